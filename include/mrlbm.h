@@ -1,0 +1,217 @@
+/*
+ * mrlbm.h - C ABI of the MI355X-native D3Q27 moment-representation regularized LBM solver.
+ *
+ * Every entry point is plain C: pointers, sizes and status codes, no C++ or torch types.
+ * The library owns all device memory; caller buffers are read or written during the call
+ * and never retained. All functions return an int status (MRLBM_OK == 0); on failure
+ * mrlbm_last_error() returns a thread-local description of the last error.
+ *
+ * Reference interfaces replaced (paths relative to the reference repository):
+ *   mrlbm_case_desc          <- compile-time case constants: MRLBM_3D/src/config.h:4-13,
+ *                               src/cases/cylinder/constants.cuh:8-54,
+ *                               src/cases/annulus/constants.cuh:8-81
+ *   mrlbm_create             <- setup_case + initialize_simulation (allocation, node types,
+ *                               case_module.initialize, constant upload, tau/Mach guards):
+ *                               src/solve_mrlbm.cu:3-24, src/simulation.cu:13-35,
+ *                               src/solver/initializeLBM.cuh:8-80
+ *   mrlbm_init               <- gpu_initialize_Moments_GhostInterface (K9):
+ *                               src/solver/initializeLBM.cu:39-121
+ *   mrlbm_step               <- one iteration of the main loop: streaming, boundary_treatment,
+ *                               post_streaming, collision, post_collision:
+ *                               src/main.cu:21-35, src/simulation.cu:81-113
+ *   mrlbm_get_forces         <- d_TotalFx/Fy/Fz/m read-back of compute_force_mass_kernel (K5):
+ *                               src/statistics/forces.cu:25-133,
+ *                               src/cases/cylinder/cylinder_kernels.cuh:578-588
+ *   mrlbm_get_moments/_set   <- copyMomentsDeviceToHost / HostToDevice:
+ *                               src/solve_mrlbm.cuh:82-147
+ *   mrlbm_get_node_types     <- nodeVar::nodeType (type << 20 | index):
+ *                               src/common/nodeTypeMap.cuh:82-98
+ *   mrlbm_get_wall_table     <- boundaryVar (boundaryList, unit normals, delta, masks):
+ *                               src/cases/cylinder/cylinder_structs.h:5-21
+ *   mrlbm_get_bcfluid_table  <- bcfluidList + h_incomingMask_bcfluid/h_outgoingMask_bcfluid:
+ *                               src/cases/cylinder/cylinder_helpers.cuh:230-299
+ *   mrlbm_checkpoint_write/read <- write_checkpoint / read_checkpoint:
+ *                               src/restart/restart_io.cu:3-92
+ *   mrlbm_destroy            <- finalize_simulation + free_case: src/simulation.cu:161-172
+ *   mrlbm_last_error         <- checkCudaErrors / exit(EXIT_FAILURE): src/utils/cudaHelpers.cuh:10-28
+ */
+#ifndef MRLBM_H
+#define MRLBM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MRLBM_ABI_VERSION 1
+
+/* ---- status codes (replace exit(EXIT_FAILURE) in the reference) ---- */
+enum mrlbm_status {
+    MRLBM_OK = 0,
+    MRLBM_ERR_INVALID = 1,     /* bad argument or descriptor field */
+    MRLBM_ERR_TAU = 2,         /* tau outside (0.5, 2.5): check_tau, initializeLBM.cuh:8-15 */
+    MRLBM_ERR_MACH = 3,        /* Ma = u/cs > 0.33: check_mach, initializeLBM.cuh:17-26 */
+    MRLBM_ERR_GEOMETRY = 4,    /* wall or tag overflow, empty wall list */
+    MRLBM_ERR_DEVICE = 5,      /* HIP runtime error */
+    MRLBM_ERR_NOMEM = 6,       /* allocation failed */
+    MRLBM_ERR_IO = 7,          /* checkpoint file missing or unreadable */
+    MRLBM_ERR_CORRUPT = 8,     /* checkpoint header, size or checksum mismatch */
+    MRLBM_ERR_NONFINITE = 9,   /* NaN/Inf watchdog tripped */
+    MRLBM_ERR_UNSUPPORTED = 10 /* feature not available for this case/precision */
+};
+
+/* ---- case kinds: the reference's CASE_ID (config.h:4-8) as a runtime value ---- */
+enum mrlbm_case_kind {
+    MRLBM_CASE_PERIODIC_BOX = 0, /* all nodes BULK, periodic in x, y, z */
+    MRLBM_CASE_CYLINDER = 1,     /* channel with one circular cylinder (setup_cylinder_case) */
+    MRLBM_CASE_ANNULUS = 2       /* two concentric walls (setup_annulus_case) */
+};
+
+/* ---- precision: storage type of the 10 moments / arithmetic type of the update ---- */
+enum mrlbm_precision {
+    MRLBM_FP64 = 0,      /* store f64, compute f64 */
+    MRLBM_FP32 = 1,      /* store f32, compute f32 (reference default: types.cuh:5) */
+    MRLBM_FP32_BF16 = 2, /* store bf16 scaled deviations, compute f32 */
+    MRLBM_FP32_FP16 = 3  /* store fp16 scaled deviations, compute f32 */
+};
+
+enum mrlbm_inlet_profile {
+    MRLBM_INLET_UNIFORM = 0,  /* u_x = u_max (reference: boundaries.cuh:217) */
+    MRLBM_INLET_PARABOLIC = 1 /* u_x(y) = 4 u_max y (H - y) / H^2, H = ny - 1 */
+};
+
+/* ---- moment fields, in the argument order of pop_reconstruction (reconstruction.cuh:5-10) ---- */
+enum mrlbm_field {
+    MRLBM_F_RHO = 0, /* stored as rho - rho0 (mlbm.cu:154) */
+    MRLBM_F_UX = 1,
+    MRLBM_F_UY = 2,
+    MRLBM_F_UZ = 3,
+    MRLBM_F_MXX = 4,
+    MRLBM_F_MYY = 5,
+    MRLBM_F_MZZ = 6,
+    MRLBM_F_MXY = 7,
+    MRLBM_F_MXZ = 8,
+    MRLBM_F_MYZ = 9,
+    MRLBM_NUM_FIELDS = 10
+};
+
+/* compat_flags bits */
+#define MRLBM_COMPAT_INIT_M_ZERO 0x1u /* initial m = 0, as the reference's K9 computes (it reads
+                                          the constant tables before they are uploaded) */
+
+/* mrlbm_step flags */
+#define MRLBM_STEP_FORCES 0x1u /* accumulate per-step Fx, Fy, Fz, mass flux (K5 window) */
+
+/*
+ * Case descriptor. Lengths are lattice units; x is the fastest index of every host array
+ * (index = x + nx * (y + ny * z)).
+ */
+typedef struct mrlbm_case_desc {
+    int32_t kind;      /* enum mrlbm_case_kind */
+    int32_t precision; /* enum mrlbm_precision */
+    int32_t nx, ny, nz;
+    int32_t periodic_x, periodic_y, periodic_z; /* X/Y/Z_PERIODIC (constants.cuh:8-10) */
+    double tau;                                  /* relaxation time; omega = 1 / tau */
+    double rho0;                                 /* RHO_0 */
+    double u_max;          /* inlet velocity scale U_MAX (cylinder); used for the Mach guard */
+    int32_t inlet_profile; /* enum mrlbm_inlet_profile */
+    uint32_t compat_flags; /* MRLBM_COMPAT_* */
+    double inlet_noise;    /* relative seeded perturbation of the inlet u_x */
+    double xc, yc;         /* wall centre (XC, YC) */
+    double r_inner;        /* cylinder radius (D/2) or annulus inner radius R_IN */
+    double r_outer;        /* annulus outer radius R_OUT (unused for the cylinder) */
+    double u_wall_inner[3]; /* wall-frame velocity (U'_x normal, U'_y tangential, U'_z) */
+    double u_wall_outer[3];
+    double init_noise; /* amplitude a of the uniform initial velocity noise U(-a, a) */
+    uint64_t seed;     /* seed of the counter-based initial-state hash */
+} mrlbm_case_desc;
+
+/* Summary of the geometry built by mrlbm_create. */
+typedef struct mrlbm_info {
+    int64_t num_nodes;         /* nx * ny * nz */
+    int32_t nb_inner, nb_outer; /* wall nodes of the inner / outer wall list */
+    int32_t nb_fluid_inner, nb_fluid_outer; /* BCFLUID nodes */
+    int32_t num_donors;        /* nodes read by the curved-wall interpolation stencils */
+    int32_t active_tiles, total_tiles; /* reference 8x8x4 tile activity (simulation.cu:174-214) */
+    int64_t steps_done;
+    int32_t precision;
+    int32_t bytes_per_value; /* storage bytes per moment value */
+} mrlbm_info;
+
+typedef struct mrlbm_sim mrlbm_sim;
+
+/* Fill desc with the defaults of the given case kind (64x64x16 box / C2 channel / C1 annulus). */
+int mrlbm_case_desc_default(int32_t kind, mrlbm_case_desc *desc);
+
+/* Validate desc, classify the geometry, build wall tables, allocate device memory. */
+int mrlbm_create(const mrlbm_case_desc *desc, mrlbm_sim **out);
+
+/*
+ * Build the geometry of desc (node types, wall and BCFLUID tables, tile activity) without
+ * touching a device: only the mrlbm_get_node_types / _wall_table / _bcfluid_table / _info
+ * queries and mrlbm_destroy are valid on the result. No reference counterpart: it exposes the
+ * host half of case_module.initialize (cases/cylinder/cylinder_kernels.cu:5-49) for callers
+ * without a GPU.
+ */
+int mrlbm_create_geometry_only(const mrlbm_case_desc *desc, mrlbm_sim **out);
+
+/* Seeded initial state: rho = rho0, u = U(-a, a) per component, m = u u (or 0, compat). */
+int mrlbm_init(mrlbm_sim *sim);
+
+/* Advance nsteps time steps (asynchronous on the simulation's stream). */
+int mrlbm_step(mrlbm_sim *sim, int32_t nsteps, uint32_t flags);
+
+/* Wait for all work queued on the simulation's stream. */
+int mrlbm_sync(mrlbm_sim *sim);
+
+/* Copy one moment field to / from a host array of nx*ny*nz doubles (x fastest). */
+int mrlbm_get_moments(mrlbm_sim *sim, int32_t field, double *host);
+int mrlbm_set_moments(mrlbm_sim *sim, int32_t field, const double *host);
+
+/*
+ * Per-step force record (Fx, Fy, Fz, mass flux) in FP64, for steps taken with
+ * MRLBM_STEP_FORCES. Records are kept in a device ring of the last `capacity` steps;
+ * `first` counts from the first step of the simulation.
+ */
+int mrlbm_get_forces(mrlbm_sim *sim, int64_t first, int32_t count, double *out4);
+
+/* Node types in the reference encoding (type << 20 | index), x-fastest. */
+int mrlbm_get_node_types(mrlbm_sim *sim, uint32_t *out);
+
+/*
+ * Wall table of wall group g (0 inner, 1 outer): global x-fastest node index, unit normal,
+ * delta, incoming and outgoing masks. Arrays must hold the count reported by mrlbm_get_info.
+ */
+int mrlbm_get_wall_table(mrlbm_sim *sim, int32_t group, int64_t *node, double *nx, double *ny,
+                         double *delta, uint32_t *in_mask, uint32_t *out_mask);
+
+/* BCFLUID list of group g (x-fastest index and tag) and the 256-entry per-tag mask tables. */
+int mrlbm_get_bcfluid_table(mrlbm_sim *sim, int32_t group, int64_t *node, uint32_t *tag,
+                            uint32_t *in_mask256, uint32_t *out_mask256);
+
+int mrlbm_get_info(mrlbm_sim *sim, mrlbm_info *info);
+
+/*
+ * Time `nsteps` steps on device with HIP events (inputs resident in HBM). Writes the wall
+ * time in ms to ms_total and, if non-null, the mean duration of one launch of each kernel
+ * (fused step, curved wall) measured by per-launch events to ms_kernels[0..1].
+ */
+int mrlbm_time_steps(mrlbm_sim *sim, int32_t nsteps, uint32_t flags, double *ms_total,
+                     double *ms_kernels);
+
+/* Layout-independent checkpoint with header and checksum. */
+int mrlbm_checkpoint_write(mrlbm_sim *sim, const char *path);
+int mrlbm_checkpoint_read(mrlbm_sim *sim, const char *path);
+
+int mrlbm_destroy(mrlbm_sim *sim);
+
+const char *mrlbm_last_error(void);
+int32_t mrlbm_abi_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MRLBM_H */

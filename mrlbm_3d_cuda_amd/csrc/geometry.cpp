@@ -1,0 +1,354 @@
+// geometry.cpp - host-side case setup (see geometry.h).
+#include "geometry.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "lattice.h"
+
+namespace mrlbm {
+
+namespace {
+
+uint64_t mix64(uint64_t x) {
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+
+// Octant b holds the seven non-zero lattice directions whose components are each 0 or carry the
+// octant's sign; bit 0/1/2 of b selects +x/+y/+z (the sets of nodeClass.cuh:67-75).
+bool in_octant(int q, int b) {
+    const int s[3] = {(b & 1) ? 1 : -1, (b & 2) ? 1 : -1, (b & 4) ? 1 : -1};
+    const int c[3] = {LATTICE.cx[q], LATTICE.cy[q], LATTICE.cz[q]};
+    if (c[0] == 0 && c[1] == 0 && c[2] == 0) return false;
+    for (int a = 0; a < 3; ++a)
+        if (c[a] != 0 && c[a] != s[a]) return false;
+    return true;
+}
+
+// Face/edge/corner type of a node from the non-periodic domain sides
+// (cases/cylinder/boundaries.cuh:8-94).
+uint8_t domain_type(const mrlbm_case_desc &d, int x, int y, int z) {
+    const bool W = !d.periodic_x && x == 0, E = !d.periodic_x && x == d.nx - 1;
+    const bool S = !d.periodic_y && y == 0, N = !d.periodic_y && y == d.ny - 1;
+    const bool B = !d.periodic_z && z == 0, F = !d.periodic_z && z == d.nz - 1;
+    const int nsides = int(W || E) + int(S || N) + int(B || F);
+    if (nsides == 3) {
+        if (N) return W ? (F ? RT_NWF : RT_NWB) : (F ? RT_NEF : RT_NEB);
+        return W ? (F ? RT_SWF : RT_SWB) : (F ? RT_SEF : RT_SEB);
+    }
+    if (nsides == 2) {
+        if (N && W) return RT_NORTH_WEST;
+        if (N && E) return RT_NORTH_EAST;
+        if (N && F) return RT_NORTH_FRONT;
+        if (N && B) return RT_NORTH_BACK;
+        if (S && W) return RT_SOUTH_WEST;
+        if (S && E) return RT_SOUTH_EAST;
+        if (S && F) return RT_SOUTH_FRONT;
+        if (S && B) return RT_SOUTH_BACK;
+        if (W && F) return RT_WEST_FRONT;
+        if (W && B) return RT_WEST_BACK;
+        if (E && F) return RT_EAST_FRONT;
+        return RT_EAST_BACK;
+    }
+    if (N) return RT_NORTH;
+    if (S) return RT_SOUTH;
+    if (W) return RT_WEST;
+    if (E) return RT_EAST;
+    if (F) return RT_FRONT;
+    if (B) return RT_BACK;
+    return RT_BULK;
+}
+
+// Node-type grid with a one-node SOLID frame in x and y and periodic wrap in z, so that the
+// neighbour look-ups of the classification (load_neighbors, nodeClass.cuh:27-60) need no
+// bounds tests.
+struct TypeGrid {
+    int nx, ny, nz, px, py;
+    std::vector<uint8_t> t;
+    TypeGrid(int nx_, int ny_, int nz_) : nx(nx_), ny(ny_), nz(nz_), px(nx_ + 2), py(ny_ + 2) {
+        t.assign(size_t(px) * py * nz, RT_SOLID);
+    }
+    uint8_t &at(int x, int y, int z) { return t[size_t(x + 1) + size_t(px) * (size_t(y + 1) + size_t(py) * z)]; }
+    uint8_t nb(int x, int y, int z, int q) const {
+        int zn = z + LATTICE.cz[q];
+        zn = zn < 0 ? zn + nz : (zn >= nz ? zn - nz : zn);
+        return t[size_t(x + 1 + LATTICE.cx[q]) + size_t(px) * (size_t(y + 1 + LATTICE.cy[q]) + size_t(py) * zn)];
+    }
+    // bitmask over octants that contain a neighbour of type `want`
+    uint32_t octants_with(int x, int y, int z, uint8_t want) const {
+        uint32_t bits = 0;
+        for (int q = 1; q < Q; ++q)
+            if (nb(x, y, z, q) == want)
+                for (int b = 0; b < 8; ++b)
+                    if (in_octant(q, b)) bits |= 1u << b;
+        return bits;
+    }
+    bool any_nb(int x, int y, int z, uint8_t want) const {
+        for (int q = 1; q < Q; ++q)
+            if (nb(x, y, z, q) == want) return true;
+        return false;
+    }
+};
+
+int popcount8(uint32_t v) {
+    int c = 0;
+    for (int b = 0; b < 8; ++b) c += (v >> b) & 1u;
+    return c;
+}
+
+}  // namespace
+
+double hash_uniform(uint64_t seed, uint64_t counter) {
+    return double(mix64(seed ^ mix64(counter)) >> 11) * 0x1.0p-53;
+}
+
+int build_geometry(const mrlbm_case_desc &d, Geometry &g, std::string &err) {
+    if (d.nx < 3 || d.ny < 3 || d.nz < 1 || int64_t(d.nx) * d.ny * d.nz > (int64_t(1) << 31)) {
+        err = "grid size out of range";
+        return MRLBM_ERR_INVALID;
+    }
+    if (d.kind != MRLBM_CASE_PERIODIC_BOX && d.kind != MRLBM_CASE_CYLINDER && d.kind != MRLBM_CASE_ANNULUS) {
+        err = "unknown case kind";
+        return MRLBM_ERR_INVALID;
+    }
+    // check_tau / check_mach (solver/initializeLBM.cuh:8-26)
+    if (!(d.tau > 0.5 && d.tau < 2.5)) {
+        err = "tau out of the stable range (0.5, 2.5)";
+        return MRLBM_ERR_TAU;
+    }
+    if (!(d.rho0 > 0.0)) {
+        err = "rho0 must be positive";
+        return MRLBM_ERR_INVALID;
+    }
+    if (d.u_max / std::sqrt(1.0 / 3.0) > 0.33) {
+        err = "Mach number above 0.33";
+        return MRLBM_ERR_MACH;
+    }
+    const bool walls = d.kind != MRLBM_CASE_PERIODIC_BOX;
+    const bool ann = d.kind == MRLBM_CASE_ANNULUS;
+    if (walls && !d.periodic_z) {
+        err = "curved-wall cases need a periodic z axis";
+        return MRLBM_ERR_UNSUPPORTED;
+    }
+    if (walls && (!(d.r_inner > 0.0) || (ann && !(d.r_outer > d.r_inner)))) {
+        err = "bad wall radii";
+        return MRLBM_ERR_GEOMETRY;
+    }
+    g.nx = d.nx;
+    g.ny = d.ny;
+    g.nz = d.nz;
+    g.n = int64_t(d.nx) * d.ny * d.nz;
+    const int NX = d.nx, NY = d.ny, NZ = d.nz;
+
+    TypeGrid tg(NX, NY, NZ);
+    for (int z = 0; z < NZ; ++z)
+        for (int y = 0; y < NY; ++y)
+            for (int x = 0; x < NX; ++x) tg.at(x, y, z) = walls ? domain_type(d, x, y, z) : RT_BULK;
+    g.ref_index.assign(g.n, 0);
+    g.ngroups = walls ? (ann ? 2 : 1) : 0;
+
+    if (walls) {
+        const double r_in = d.r_inner, r_out = d.r_outer, r_mid = 0.5 * (r_in + r_out);
+        auto radius = [&](int x, int y) {
+            const double dx = double(x) - d.xc, dy = double(y) - d.yc;
+            return std::sqrt(dx * dx + dy * dy);
+        };
+        // solid marking: every z plane is the same disc / ring
+        for (int y = 0; y < NY; ++y)
+            for (int x = 0; x < NX; ++x) {
+                const double r = radius(x, y);
+                if (r <= r_in || (ann && r >= r_out))
+                    for (int z = 0; z < NZ; ++z) tg.at(x, y, z) = RT_SOLID;
+            }
+        // classification window: cylinder bounding box +4 nodes, annulus whole domain
+        int xa = 0, xb = NX, ya = 0, yb = NY;
+        if (!ann) {
+            xa = std::max(0, int(std::ceil(d.xc - r_in)) - 4);
+            xb = std::min(NX, int(std::floor(d.xc + r_in)) + 5);
+            ya = std::max(0, int(std::ceil(d.yc - r_in)) - 4);
+            yb = std::min(NY, int(std::floor(d.yc + r_in)) + 5);
+        }
+        // wall nodes: SOLID with BULK neighbours in more than two octants. Converting a node does
+        // not change any other node's test (only SOLID centres and BULK neighbours count).
+        for (int z = 0; z < NZ; ++z)
+            for (int y = ya; y < yb; ++y)
+                for (int x = xa; x < xb; ++x) {
+                    if (tg.at(x, y, z) != RT_SOLID) continue;
+                    if (popcount8(tg.octants_with(x, y, z, RT_BULK)) > 2)
+                        tg.at(x, y, z) = (ann && radius(x, y) >= r_mid) ? RT_OUTER : RT_INNER;
+                }
+        // BCFLUID nodes: BULK with a SOLID neighbour, tagged by the octants holding SOLID
+        std::vector<uint32_t> tag(g.n, 0);
+        for (int z = 0; z < NZ; ++z)
+            for (int y = ya; y < yb; ++y)
+                for (int x = xa; x < xb; ++x) {
+                    if (tg.at(x, y, z) != RT_BULK || !tg.any_nb(x, y, z, RT_SOLID)) continue;
+                    tag[g.idx(x, y, z)] = tg.octants_with(x, y, z, RT_SOLID);
+                    tg.at(x, y, z) = (ann && radius(x, y) >= r_mid) ? RT_BCFLUID_OUTER : RT_BCFLUID_INNER;
+                }
+        g.wall[0].radius = r_in;
+        g.wall[1].radius = r_out;
+        g.wall[1].fluid_dir = -1.0;
+        for (int k = 0; k < 3; ++k) {
+            g.wall[0].uw[k] = d.u_wall_inner[k];
+            g.wall[1].uw[k] = d.u_wall_outer[k];
+        }
+        // lists in x-fastest order (the reference's z, y, x scan order)
+        for (int z = 0; z < NZ; ++z)
+            for (int y = 0; y < NY; ++y)
+                for (int x = 0; x < NX; ++x) {
+                    const uint8_t t = tg.at(x, y, z);
+                    const int64_t i = g.idx(x, y, z);
+                    if (t == RT_INNER || t == RT_OUTER) {
+                        WallTable &w = g.wall[t == RT_OUTER];
+                        g.ref_index[i] = uint32_t(w.node.size());
+                        w.node.push_back(i);
+                    } else if (t == RT_BCFLUID_INNER || t == RT_BCFLUID_OUTER) {
+                        WallTable &w = g.wall[t == RT_BCFLUID_OUTER];
+                        g.ref_index[i] = tag[i];
+                        w.bcf_node.push_back(i);
+                        w.bcf_tag.push_back(tag[i]);
+                    }
+                }
+        for (int gi = 0; gi < g.ngroups; ++gi) {
+            WallTable &w = g.wall[gi];
+            if (w.node.empty()) {
+                err = "empty wall list";
+                return MRLBM_ERR_GEOMETRY;
+            }
+            if (w.node.size() >= (size_t(1) << 20)) {
+                err = "wall list index overflows the 20-bit node-type field";
+                return MRLBM_ERR_GEOMETRY;
+            }
+            for (int64_t i : w.node) {
+                const int x = int(i % NX), y = int((i / NX) % NY), z = int(i / (int64_t(NX) * NY));
+                // unit normal from the centre, wall point and depth delta
+                // (compute_unit_vectors_boundary_nodes, cylinder_helpers.cuh:10-47)
+                const double dx = double(x) - d.xc, dy = double(y) - d.yc;
+                const double inv_r = 1.0 / std::sqrt(dx * dx + dy * dy);
+                const double nxv = dx * inv_r, nyv = dy * inv_r;
+                const double xw = d.xc + w.radius * nxv, yw = d.yc + w.radius * nyv;
+                w.nx.push_back(nxv);
+                w.ny.push_back(nyv);
+                w.delta.push_back(std::fabs((xw - double(x)) * nxv + (yw - double(y)) * nyv));
+                // incoming: opposite directions of octants free of SOLID neighbours, plus rest
+                // (find_incomings_outgoings, cylinder_helpers.cuh:134-228)
+                uint32_t in = 1u;
+                const uint32_t solid_oct = tg.octants_with(x, y, z, RT_SOLID);
+                for (int b = 0; b < 8; ++b) {
+                    if (solid_oct & (1u << b)) continue;
+                    for (int q = 1; q < Q; ++q)
+                        if (in_octant(q, b)) in |= 1u << LATTICE.opp[q];
+                }
+                uint32_t out = 1u;
+                for (int q = 0; q < Q; ++q)
+                    if (in & (1u << LATTICE.opp[q])) out |= 1u << q;
+                w.in_mask.push_back(in);
+                w.out_mask.push_back(out);
+            }
+        }
+        // per-tag BCFLUID masks from the first node of each list with that tag; groups in order,
+        // so the outer list overrides tags the inner list set (setup_bcfluid_masks,
+        // cylinder_helpers.cuh:230-299, called inner then outer by annulus_kernels.cu:37-38)
+        for (int gi = 0; gi < g.ngroups; ++gi) {
+            uint8_t seen[256] = {0};
+            const WallTable &w = g.wall[gi];
+            for (size_t k = 0; k < w.bcf_node.size(); ++k) {
+                const uint32_t t = w.bcf_tag[k];
+                if (seen[t]) continue;
+                seen[t] = 1;
+                const int64_t i = w.bcf_node[k];
+                const int x = int(i % NX), y = int((i / NX) % NY), z = int(i / (int64_t(NX) * NY));
+                uint32_t in = (1u << Q) - 1u;
+                for (int q = 0; q < Q; ++q)
+                    if (tg.nb(x, y, z, q) == RT_SOLID) in &= ~(1u << LATTICE.opp[q]);
+                uint32_t out = 0;
+                for (int q = 0; q < Q; ++q)
+                    if (in & (1u << LATTICE.opp[q])) out |= 1u << q;
+                g.bcf_in[t] = in;
+                g.bcf_out[t] = out;
+            }
+        }
+        // inlet profile per (y, z)
+        g.inlet.assign(size_t(NY) * NZ, 0.0);
+        const double H = double(NY - 1);
+        for (int z = 0; z < NZ; ++z)
+            for (int y = 0; y < NY; ++y) {
+                double u = d.u_max;
+                if (d.inlet_profile == MRLBM_INLET_PARABOLIC) u = 4.0 * d.u_max * double(y) * (H - double(y)) / (H * H);
+                const double r = hash_uniform(d.seed, uint64_t(3 * g.n) + uint64_t(y) + uint64_t(NY) * uint64_t(z));
+                g.inlet[size_t(y) + size_t(NY) * z] = u * (1.0 + d.inlet_noise * (2.0 * r - 1.0));
+            }
+    }
+
+    // copy the type grid out
+    g.ref_type.assign(g.n, RT_BULK);
+    for (int z = 0; z < NZ; ++z)
+        for (int y = 0; y < NY; ++y)
+            for (int x = 0; x < NX; ++x) g.ref_type[g.idx(x, y, z)] = tg.at(x, y, z);
+
+    // reference tile activity: 8x8x4 tiles, active if any non-SOLID node; the kernels index the
+    // mask without blockIdx.z (mlbm.cu:87), so the bz = 0 tile decides its whole column
+    const int GX = (NX + 7) / 8, GY = (NY + 7) / 8, GZ = (NZ + 3) / 4;
+    std::vector<uint8_t> col_active(size_t(GX) * GY, 0);
+    for (int y = 0; y < NY; ++y)
+        for (int x = 0; x < NX; ++x)
+            for (int z = 0; z < std::min(4, NZ); ++z)
+                if (g.ref_type[g.idx(x, y, z)] != RT_SOLID) col_active[size_t(x / 8) + size_t(GX) * (y / 8)] = 1;
+    g.total_tiles = GX * GY * GZ;
+    g.active_tiles = 0;
+    for (uint8_t a : col_active) g.active_tiles += int(a) * GZ;
+
+    // per-node info words for the fused kernel
+    g.info.assign(g.n, NC_BULK);
+    for (int z = 0; z < NZ; ++z)
+        for (int y = 0; y < NY; ++y)
+            for (int x = 0; x < NX; ++x) {
+                const int64_t i = g.idx(x, y, z);
+                if (!col_active[size_t(x / 8) + size_t(GX) * (y / 8)]) {
+                    g.info[i] = NC_FROZEN;
+                    continue;
+                }
+                uint32_t code = NC_KEEP, payload = 0;
+                const uint8_t t = g.ref_type[i];
+                switch (t) {
+                case RT_BULK: code = NC_BULK; break;
+                case RT_INNER: code = NC_WALL; payload = g.ref_index[i]; break;
+                case RT_OUTER: code = NC_WALL; payload = uint32_t(g.wall[0].node.size()) + g.ref_index[i]; break;
+                case RT_BCFLUID_INNER:
+                case RT_BCFLUID_OUTER: {
+                    const uint32_t tg8 = g.ref_index[i];
+                    if (tg8 == 17 || tg8 == 34 || tg8 == 68 || tg8 == 136) {
+                        code = NC_BCF;
+                        payload = tg8;
+                    }
+                    break;
+                }
+                default:
+                    // domain faces have closed forms only in the cylinder case; the annulus
+                    // module leaves them unchanged (annulus_kernels.cuh:54-57)
+                    if (d.kind == MRLBM_CASE_CYLINDER) {
+                        switch (t) {
+                        case RT_WEST: code = NC_WEST; break;
+                        case RT_EAST: code = NC_EAST; break;
+                        case RT_NORTH: code = NC_NORTH; break;
+                        case RT_SOUTH: code = NC_SOUTH; break;
+                        case RT_NORTH_WEST: code = NC_NORTH_WEST; break;
+                        case RT_SOUTH_WEST: code = NC_SOUTH_WEST; break;
+                        case RT_NORTH_EAST: code = NC_NORTH_EAST; break;
+                        case RT_SOUTH_EAST: code = NC_SOUTH_EAST; break;
+                        default: break;
+                        }
+                    }
+                    break;
+                }
+                g.info[i] = code | (payload << INFO_PAYLOAD_SHIFT);
+            }
+    return MRLBM_OK;
+}
+
+}  // namespace mrlbm

@@ -1,0 +1,430 @@
+// node_math.h - per-node D3Q27 moment-space arithmetic for the HIP kernels.
+//
+// Moment vector M[10] = (rho, ux, uy, uz, mxx, myy, mzz, mxy, mxz, myz) with rho absolute and
+// m_ab = Pi_ab / rho - cs2 delta_ab (common/globalStructs.cuh:6-19,
+// core/D3Q27/eval_moments.cuh:16-21). R is the arithmetic type (double or float).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "lattice.h"
+
+namespace mrlbm {
+
+// ---------------------------------------------------------------- mask helpers
+
+// Bit set of directions satisfying a predicate on (cx, cy, cz).
+template <class Pred>
+constexpr uint32_t dir_mask(Pred p) {
+    uint32_t m = 0;
+    for (int q = 0; q < Q; ++q)
+        if (p(LATTICE.cx[q], LATTICE.cy[q], LATTICE.cz[q])) m |= 1u << q;
+    return m;
+}
+
+// Sum of f over the set bits of M, in increasing q.
+template <uint32_t M, class R>
+__device__ __forceinline__ R msum(const R *f) {
+    R s = R(0);
+    bool first = true;
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        if (M & (1u << q)) {
+            s = first ? f[q] : s + f[q];
+            first = false;
+        }
+    }
+    return s;
+}
+
+// ---------------------------------------------------------------- regularised populations
+
+// All 27 populations of the second-order regularised distribution of M
+// (core/D3Q27/second_order/reconstruction.cuh:5-60), evaluated pairwise: for each pair of
+// opposite directions f = w (S +- V) with S the even (rho, m) part and V = 3 rho c.u.
+template <class R>
+__device__ __forceinline__ void reconstruct(const R *M, R *f) {
+    const R rho = M[0];
+    const R P = rho * (R(1) - R(1.5) * (M[4] + M[5] + M[6]));
+    const R r3 = R(3) * rho, r45 = R(4.5) * rho, r9 = R(9) * rho;
+    const R Ux = r3 * M[1], Uy = r3 * M[2], Uz = r3 * M[3];
+    const R Dx = r45 * M[4], Dy = r45 * M[5], Dz = r45 * M[6];
+    const R Oxy = r9 * M[7], Oxz = r9 * M[8], Oyz = r9 * M[9];
+    const R w0 = R(W_REST), w1 = R(W_AXIS), w2 = R(W_FACE), w3 = R(W_CORNER);
+    f[0] = w0 * P;
+    auto pair = [&](int q, R w, R S, R V) {
+        f[q] = w * (S + V);
+        f[q + 1] = w * (S - V);
+    };
+    pair(1, w1, P + Dx, Ux);
+    pair(3, w1, P + Dy, Uy);
+    pair(5, w1, P + Dz, Uz);
+    const R Pxy = P + Dx + Dy, Pxz = P + Dx + Dz, Pyz = P + Dy + Dz, Pxyz = Pxy + Dz;
+    pair(7, w2, Pxy + Oxy, Ux + Uy);
+    pair(9, w2, Pxz + Oxz, Ux + Uz);
+    pair(11, w2, Pyz + Oyz, Uy + Uz);
+    pair(13, w2, Pxy - Oxy, Ux - Uy);
+    pair(15, w2, Pxz - Oxz, Ux - Uz);
+    pair(17, w2, Pyz - Oyz, Uy - Uz);
+    pair(19, w3, Pxyz + Oxy + Oxz + Oyz, Ux + Uy + Uz);
+    pair(21, w3, Pxyz + Oxy - Oxz - Oyz, Ux + Uy - Uz);
+    pair(23, w3, Pxyz - Oxy + Oxz - Oyz, Ux - Uy + Uz);
+    pair(25, w3, Pxyz - Oxy - Oxz + Oyz, Uy + Uz - Ux);
+}
+
+// Moments of 27 populations (core/D3Q27/eval_moments.cuh:5-22), from the sums s and
+// differences d of opposite pairs.
+template <class R>
+__device__ __forceinline__ void moments(const R *f, R *M) {
+    R s[13], d[13];
+#pragma unroll
+    for (int k = 0; k < 13; ++k) {
+        s[k] = f[2 * k + 1] + f[2 * k + 2];
+        d[k] = f[2 * k + 1] - f[2 * k + 2];
+    }
+    // pair k: 0 x, 1 y, 2 z, 3 (1,1,0), 4 (1,0,1), 5 (0,1,1), 6 (1,-1,0), 7 (1,0,-1),
+    // 8 (0,1,-1), 9 (1,1,1), 10 (1,1,-1), 11 (1,-1,1), 12 (-1,1,1)
+    const R corners = (s[9] + s[10]) + (s[11] + s[12]);
+    const R rho = f[0] + ((s[0] + s[1]) + (s[2] + s[3])) + ((s[4] + s[5]) + (s[6] + s[7])) + (s[8] + corners);
+    const R jx = (d[0] + d[3]) + (d[4] + d[6]) + d[7] + (d[9] + d[10]) + (d[11] - d[12]);
+    const R jy = (d[1] + d[3]) + (d[5] - d[6]) + d[8] + (d[9] + d[10]) + (d[12] - d[11]);
+    const R jz = (d[2] + d[4]) + (d[5] - d[7]) - d[8] + (d[9] - d[10]) + (d[11] + d[12]);
+    const R pxx = (s[0] + s[3]) + (s[4] + s[6]) + s[7] + corners;
+    const R pyy = (s[1] + s[3]) + (s[5] + s[6]) + s[8] + corners;
+    const R pzz = (s[2] + s[4]) + (s[5] + s[7]) + s[8] + corners;
+    const R pxy = (s[3] - s[6]) + (s[9] + s[10]) - (s[11] + s[12]);
+    const R pxz = (s[4] - s[7]) + (s[9] + s[11]) - (s[10] + s[12]);
+    const R pyz = (s[5] - s[8]) + (s[9] + s[12]) - (s[10] + s[11]);
+    const R inv = R(1) / rho;
+    const R cs2 = R(1) / R(3);
+    M[0] = rho;
+    M[1] = jx * inv;
+    M[2] = jy * inv;
+    M[3] = jz * inv;
+    M[4] = pxx * inv - cs2;
+    M[5] = pyy * inv - cs2;
+    M[6] = pzz * inv - cs2;
+    M[7] = pxy * inv;
+    M[8] = pxz * inv;
+    M[9] = pyz * inv;
+}
+
+// BGK relaxation of the second moments towards u u (moment_collision, solver/mlbm.cu:3-16).
+template <class R>
+__device__ __forceinline__ void collide(R omega, R *M) {
+    const R om1 = R(1) - omega;
+    M[4] = om1 * M[4] + omega * M[1] * M[1];
+    M[5] = om1 * M[5] + omega * M[2] * M[2];
+    M[6] = om1 * M[6] + omega * M[3] * M[3];
+    M[7] = om1 * M[7] + omega * M[1] * M[2];
+    M[8] = om1 * M[8] + omega * M[1] * M[3];
+    M[9] = om1 * M[9] + omega * M[2] * M[3];
+}
+
+// Population q of the regularised distribution of M (force sums over masked directions).
+template <class R>
+__device__ __forceinline__ R population(const R *M, int q) {
+    const R cx = R(LATTICE.cx[q]), cy = R(LATTICE.cy[q]), cz = R(LATTICE.cz[q]);
+    const R third = R(1) / R(3);
+    const R cu = cx * M[1] + cy * M[2] + cz * M[3];
+    const R hd = M[4] * (cx * cx - third) + M[5] * (cy * cy - third) + M[6] * (cz * cz - third);
+    const R ho = M[7] * cx * cy + M[8] * cx * cz + M[9] * cy * cz;
+    return R(weight(q)) * M[0] * (R(1) + R(3) * cu + R(4.5) * hd + R(9) * ho);
+}
+
+// ---------------------------------------------------------------- domain-face closed forms
+// cases/cylinder/boundaries.cuh:201-843. The known ("incoming") populations of a face are those
+// whose velocity does not point out of the domain through that face; every moment below is a
+// sum over them. `old`/`old_xm1` are the node's and its x-1 neighbour's moments at the start of
+// the step.
+
+namespace face {
+constexpr uint32_t K_W = dir_mask([](int cx, int, int) { return cx <= 0; });
+constexpr uint32_t K_E = dir_mask([](int cx, int, int) { return cx >= 0; });
+constexpr uint32_t K_N = dir_mask([](int, int cy, int) { return cy >= 0; });
+constexpr uint32_t K_S = dir_mask([](int, int cy, int) { return cy <= 0; });
+constexpr uint32_t XY_P = dir_mask([](int cx, int cy, int) { return cx * cy > 0; });
+constexpr uint32_t XY_N = dir_mask([](int cx, int cy, int) { return cx * cy < 0; });
+constexpr uint32_t XZ_P = dir_mask([](int cx, int, int cz) { return cx * cz > 0; });
+constexpr uint32_t XZ_N = dir_mask([](int cx, int, int cz) { return cx * cz < 0; });
+constexpr uint32_t YZ_P = dir_mask([](int, int cy, int cz) { return cy * cz > 0; });
+constexpr uint32_t YZ_N = dir_mask([](int, int cy, int cz) { return cy * cz < 0; });
+constexpr uint32_t YY = dir_mask([](int, int cy, int) { return cy != 0; });
+constexpr uint32_t ZZ = dir_mask([](int, int, int cz) { return cz != 0; });
+}  // namespace face
+
+// sum over known & P minus sum over known & N
+template <uint32_t K, uint32_t P, uint32_t N, class R>
+__device__ __forceinline__ R mdiff(const R *f) {
+    return msum<K & P>(f) - msum<K & N>(f);
+}
+
+// Convective outlet blend (update_neumaan_density_velocity, boundaries.cuh:163-199 with
+// CONVECTIVE_OUTLET): Uc = |u_x(x-1)|, phi = (1 - Uc) phi_b + Uc phi_{x-1}.
+template <class R>
+__device__ __forceinline__ void outlet_blend(const R *old, const R *old_xm1, R *M) {
+    const R Uc = sqrt(old_xm1[1] * old_xm1[1]);
+    const R a = R(1) - Uc;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) M[k] = a * old[k] + Uc * old_xm1[k];
+}
+
+template <class R>
+__device__ __forceinline__ void bc_west(const R *f, const R *old, R u_in, R *M) {
+    using namespace face;
+    const R rI = msum<K_W>(f);
+    const R inv = R(1) / rI;
+    const R mxyI = mdiff<K_W, XY_P, XY_N>(f) * inv;
+    const R mxzI = mdiff<K_W, XZ_P, XZ_N>(f) * inv;
+    const R uo = old[1];  // previous-step u_x of the node (boundaries.cuh:216)
+    const R rho = -R(6) * rI / (-R(5) + R(3) * uo + R(3) * uo * uo);
+    M[0] = rho;
+    M[1] = u_in;
+    M[2] = R(0);
+    M[3] = R(0);
+    M[4] = u_in * u_in;
+    M[5] = R(0);
+    M[6] = R(0);
+    M[7] = R(2) * mxyI * rI / rho;
+    M[8] = R(2) * mxzI * rI / rho;
+    M[9] = R(0);
+}
+
+template <class R>
+__device__ __forceinline__ void bc_east(const R *f, const R *old, const R *old_xm1, R *M) {
+    using namespace face;
+    const R rI = msum<K_E>(f);
+    const R inv = R(1) / rI;
+    const R cs2 = R(1) / R(3);
+    const R myyI = msum<K_E & YY>(f) * inv - cs2;
+    const R mzzI = msum<K_E & ZZ>(f) * inv - cs2;
+    const R mxyI = mdiff<K_E, XY_P, XY_N>(f) * inv;
+    const R mxzI = mdiff<K_E, XZ_P, XZ_N>(f) * inv;
+    const R myzI = mdiff<K_E, YZ_P, YZ_N>(f) * inv;
+    outlet_blend(old, old_xm1, M);
+    const R rho = M[0], uy = M[2], uz = M[3];
+    M[4] = M[1] * M[1];
+    M[5] = (R(6) * myyI * rI - R(6) * mzzI * rI + R(5) * uy * uy * rho + R(5) * uz * uz * rho) / (R(10) * rho);
+    M[6] = (-R(6) * myyI * rI + R(6) * mzzI * rI + R(5) * uy * uy * rho + R(5) * uz * uz * rho) / (R(10) * rho);
+    M[7] = (R(6) * mxyI * rI - uy * rho) / (R(3) * rho);
+    M[8] = (R(6) * mxzI * rI - uz * rho) / (R(3) * rho);
+    M[9] = R(6) * myzI * rI / (R(5) * rho);
+}
+
+// no-slip y walls (NORTH/SOUTH, boundaries.cuh:272-315)
+template <uint32_t K, class R>
+__device__ __forceinline__ void bc_ywall(const R *f, R *M) {
+    using namespace face;
+    const R rI = msum<K>(f);
+    const R inv = R(1) / rI;
+    const R mxyI = mdiff<K, XY_P, XY_N>(f) * inv;
+    const R myzI = mdiff<K, YZ_P, YZ_N>(f) * inv;
+    const R rho = R(6) * rI / R(5);
+    M[0] = rho;
+    M[1] = M[2] = M[3] = R(0);
+    M[4] = M[5] = M[6] = R(0);
+    M[7] = R(2) * mxyI * rI / rho;
+    M[8] = R(0);
+    M[9] = R(2) * myzI * rI / rho;
+}
+
+// inlet corners with the y walls (NORTH_WEST sgn = -1 ... SOUTH_WEST sgn = +1,
+// boundaries.cuh:360-403): sgn is the sign of c_x c_y of the known diagonal pair.
+template <uint32_t K, int SGN, class R>
+__device__ __forceinline__ void bc_west_edge(const R *f, R omega, R *M) {
+    using namespace face;
+    const R rI = msum<K>(f);
+    const R inv = R(1) / rI;
+    const R mxyI = (SGN > 0 ? msum<K & XY_P>(f) : -msum<K & XY_N>(f)) * inv;
+    const R myzI = mdiff<K, YZ_P, YZ_N>(f) * inv;
+    R rho;
+    if (SGN > 0)
+        rho = R(36) * rI * (R(1) - mxyI + mxyI * omega) / (R(24) + omega);
+    else
+        rho = R(36) * rI * (R(1) + mxyI - mxyI * omega) / (R(24) + omega);
+    M[0] = rho;
+    M[1] = M[2] = M[3] = R(0);
+    M[4] = M[5] = M[6] = R(0);
+    M[7] = (R(36) * mxyI * rI - R(SGN) * rho) / (R(9) * rho);
+    M[8] = R(0);
+    M[9] = R(12) * myzI * rI / (R(5) * rho);
+}
+
+// outlet corners with the y walls (NORTH_EAST sgn = +1, SOUTH_EAST sgn = -1,
+// boundaries.cuh:448-513)
+template <uint32_t K, int SGN, class R>
+__device__ __forceinline__ void bc_east_edge(const R *f, const R *old, const R *old_xm1, R *M) {
+    using namespace face;
+    const R rI = msum<K>(f);
+    const R inv = R(1) / rI;
+    const R mxyI = (SGN > 0 ? msum<K & XY_P>(f) : -msum<K & XY_N>(f)) * inv;
+    const R myzI = mdiff<K, YZ_P, YZ_N>(f) * inv;
+    outlet_blend(old, old_xm1, M);
+    const R rho = M[0];
+    M[4] = M[5] = M[6] = R(0);
+    M[7] = (R(36) * mxyI * rI - R(SGN) * rho) / (R(9) * rho);
+    M[8] = R(0);
+    M[9] = (R(12) * myzI * rI) / (R(5) * rho);
+}
+
+// ---------------------------------------------------------------- BCFLUID closed forms
+// fluid_boundary_condition, cases/cylinder/boundaries.cuh:845-1037 (Equilibrium mass variant).
+// The four tags are x/y mirror images: with (sx, sy) = (+,+) for 17, (-,+) for 34, (+,-) for 68
+// and (-,-) for 136, each tag's formulas are tag 17's applied to (sx ux, sy uy, sx sy mxy,
+// sx mxz, sy myz), with the outputs mapped back the same way.
+template <class R>
+__device__ __forceinline__ void bc_fluid(uint32_t tag, uint32_t in_mask, const R *f, R *M) {
+    R rI = 0, ux = 0, uy = 0, uz = 0, mxx = 0, myy = 0, mzz = 0, mxy = 0, mxz = 0, myz = 0;
+    const R third = R(1) / R(3);
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        if (!(in_mask & (1u << q))) continue;
+        const R fq = f[q];
+        rI += fq;
+        ux += fq * R(LATTICE.cx[q]);
+        uy += fq * R(LATTICE.cy[q]);
+        uz += fq * R(LATTICE.cz[q]);
+        mxx += fq * (R(LATTICE.cx[q] * LATTICE.cx[q]) - third);
+        myy += fq * (R(LATTICE.cy[q] * LATTICE.cy[q]) - third);
+        mzz += fq * (R(LATTICE.cz[q] * LATTICE.cz[q]) - third);
+        mxy += fq * R(LATTICE.cx[q] * LATTICE.cy[q]);
+        mxz += fq * R(LATTICE.cx[q] * LATTICE.cz[q]);
+        myz += fq * R(LATTICE.cy[q] * LATTICE.cz[q]);
+    }
+    const R inv = R(1) / rI;
+    const R sx = (tag == 34u || tag == 136u) ? R(-1) : R(1);
+    const R sy = (tag == 68u || tag == 136u) ? R(-1) : R(1);
+    ux = sx * (ux * inv);
+    uy = sy * (uy * inv);
+    uz = uz * inv;
+    mxx = mxx * inv;
+    myy = myy * inv;
+    mzz = mzz * inv;
+    mxy = (sx * sy) * (mxy * inv);
+    mxz = sx * (mxz * inv);
+    myz = sy * (myz * inv);
+    // density: root of a rho^2 + b rho + c = 0 closest to rho_I (boundaries.cuh:905-915)
+    const R a = R(-9998);
+    const R b = R(6) * rI * (R(1734) + R(66) * mxx + R(198) * mxy + R(66) * myy + R(253) * ux + R(253) * uy);
+    const R c = R(3) * (rI * rI) *
+                (R(45) * (mxx * mxx) + R(405) * (mxy * mxy) + R(45) * (myy * myy) + R(345) * myy * ux +
+                 R(589) * (ux * ux) + R(345) * myy * uy + R(1467) * ux * uy + R(589) * (uy * uy) +
+                 R(45) * mxy * (R(6) * myy + R(23) * (ux + uy)) +
+                 R(15) * mxx * (R(18) * mxy + R(6) * myy + R(23) * (ux + uy)));
+    const R disc = fmax(b * b - R(4) * a * c, R(0));
+    const R sq = sqrt(disc);
+    const R rho1 = (-b + sq) / (R(2) * a);
+    const R rho2 = (-b - sq) / (R(2) * a);
+    const R rho = (rho1 > R(0) && fabs(rho1 - rI) < fabs(rho2 - rI)) ? rho1 : rho2;
+    // closed forms, boundaries.cuh:916-924
+    const R uxo = (R(3) * mxx * rI + R(9) * mxy * rI + R(3) * myy * rI + R(20) * rI * ux + R(3) * rI * uy + rho) / (R(17) * rho);
+    const R uyo = (R(3) * mxx * rI + R(9) * mxy * rI + R(3) * myy * rI + R(3) * rI * ux + R(20) * rI * uy + rho) / (R(17) * rho);
+    const R uzo = (R(3) * rI * (mxz + myz + R(10) * uz)) / (R(29) * rho);
+    const R mxxo = (R(57) * mxx * rI + R(18) * mxy * rI + R(6) * myy * rI + R(6) * rI * ux + R(6) * rI * uy + R(2) * rho) / (R(51) * rho);
+    const R myyo = (R(6) * mxx * rI + R(18) * mxy * rI + R(57) * myy * rI + R(6) * rI * ux + R(6) * rI * uy + R(2) * rho) / (R(51) * rho);
+    const R mzzo = (R(36) * mzz * rI) / (R(35) * rho);
+    const R mxyo = (R(3) * mxx * rI + R(26) * mxy * rI + R(3) * myy * rI + R(3) * rI * ux + R(3) * rI * uy + rho) / (R(17) * rho);
+    const R mxzo = (rI * (R(32) * mxz + R(3) * myz + uz)) / (R(29) * rho);
+    const R myzo = (rI * (R(3) * mxz + R(32) * myz + uz)) / (R(29) * rho);
+    M[0] = rho;
+    M[1] = sx * uxo;
+    M[2] = sy * uyo;
+    M[3] = uzo;
+    M[4] = mxxo;
+    M[5] = myyo;
+    M[6] = mzzo;
+    M[7] = (sx * sy) * mxyo;
+    M[8] = sx * mxzo;
+    M[9] = sy * myzo;
+}
+
+// ---------------------------------------------------------------- curved walls
+
+// Incoming density and rotated incoming second moments at a wall node (x' along the normal
+// (nx, ny)); evaluate_incoming_moments_rotated, boundary/curvedLBM.cu:5-59.
+template <class R>
+__device__ __forceinline__ void incoming_rotated(R nx, R ny, uint32_t in_mask, const R *f, R *M) {
+    const R third = R(1) / R(3);
+    R r = 0, axx = 0, ayy = 0, azz = 0, axy = 0, axz = 0, ayz = 0;
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        if (!(in_mask & (1u << q))) continue;
+        const R cx = R(LATTICE.cx[q]), cy = R(LATTICE.cy[q]), cz = R(LATTICE.cz[q]);
+        const R xp = cx * nx + cy * ny, yp = -cx * ny + cy * nx;
+        const R fq = f[q];
+        r += fq;
+        axx += fq * (xp * xp - third);
+        ayy += fq * (yp * yp - third);
+        azz += fq * (cz * cz - third);
+        axy += fq * (xp * yp);
+        axz += fq * (xp * cz);
+        ayz += fq * (yp * cz);
+    }
+    const R inv = R(1) / r;
+    M[0] = r;
+    M[4] = axx * inv;
+    M[5] = ayy * inv;
+    M[6] = azz * inv;
+    M[7] = axy * inv;
+    M[8] = axz * inv;
+    M[9] = ayz * inv;
+}
+
+// Wall-frame closure (numerical_solution_rhoeq_rotated, boundary/numerical_solution.cu:4-122):
+// p holds the extrapolated u'x, u'y, u'z, m'xx, m'yy, m'zz, m'yz; rhoI, mxyI, mxzI the node's
+// incoming values. Solves the 2x2 system for m'xy, m'xz, rotates back and closes rho.
+template <class R>
+__device__ __forceinline__ void wall_closure(R nx, R ny, uint32_t in_mask, const R *p, R rhoI, R mxyI, R mxzI,
+                                             R *M) {
+    const R c = nx, s = ny, third = R(1) / R(3);
+    const R s2 = R(2) * s * c, c2 = c * c - s * s;
+    const R ux = p[0], uy = p[1], uz = p[2], mxx = p[3], myy = p[4], mzz = p[5], myz = p[6];
+    R E = 0, Dxy = 0, Dxz = 0;
+    R F11a = 0, F22a = 0, F33a = 0, F12a = 0, F13a = 0, F23a = 0;
+    R F11b = 0, F22b = 0, F33b = 0, F12b = 0, F13b = 0, F23b = 0;
+    for (int q = 0; q < Q; ++q) {
+        if (!(in_mask & (1u << q))) continue;
+        const R cx = R(LATTICE.cx[q]), cy = R(LATTICE.cy[q]), cz = R(LATTICE.cz[q]);
+        const R xp = cx * c + cy * s, yp = -cx * s + cy * c, zp = cz;
+        const R Hxx = xp * xp - third, Hyy = yp * yp - third, Hzz = zp * zp - third;
+        const R Hxy = xp * yp, Hxz = xp * zp, Hyz = yp * zp;
+        const R wq = R(weight(q));
+        const R cf = R(4.5) * wq;
+        const R A = wq * (R(1) + R(3) * (ux * xp + uy * yp + uz * zp));
+        E += A + cf * (ux * ux * Hxx + uy * uy * Hyy + uz * uz * Hzz + R(2) * ux * uy * Hxy + R(2) * ux * uz * Hxz +
+                       R(2) * uy * uz * Hyz);
+        Dxy += A * Hxy;
+        Dxz += A * Hxz;
+        F11a += (cf * Hxx) * Hxy;
+        F22a += (cf * Hyy) * Hxy;
+        F33a += (cf * Hzz) * Hxy;
+        F12a += (cf * Hxy) * Hxy;
+        F13a += (cf * Hxz) * Hxy;
+        F23a += (cf * Hyz) * Hxy;
+        F11b += (cf * Hxx) * Hxz;
+        F22b += (cf * Hyy) * Hxz;
+        F33b += (cf * Hzz) * Hxz;
+        F12b += (cf * Hxy) * Hxz;
+        F13b += (cf * Hxz) * Hxz;
+        F23b += (cf * Hyz) * Hxz;
+    }
+    const R a1 = R(2) * F12a, b1 = R(2) * F13a;
+    const R c1 = (E * mxyI - Dxy) - mxx * F11a - myy * F22a - mzz * F33a - R(2) * myz * F23a;
+    const R a2 = R(2) * F12b, b2 = R(2) * F13b;
+    const R cc = (E * mxzI - Dxz) - mxx * F11b - myy * F22b - mzz * F33b - R(2) * myz * F23b;
+    const R inv = R(1) / (a2 * b1 - a1 * b2);
+    const R mxyp = (b1 * cc - b2 * c1) * inv;
+    const R mxzp = (a2 * c1 - a1 * cc) * inv;
+    M[1] = ux * c - uy * s;
+    M[2] = ux * s + uy * c;
+    M[3] = uz;
+    M[4] = mxx * c * c + myy * s * s - mxyp * s2;
+    M[5] = mxx * s * s + myy * c * c + mxyp * s2;
+    M[6] = mzz;
+    M[7] = (mxx - myy) * R(0.5) * s2 + mxyp * c2;
+    M[8] = mxzp * c - myz * s;
+    M[9] = mxzp * s + myz * c;
+    M[0] = rhoI / E;
+}
+
+}  // namespace mrlbm

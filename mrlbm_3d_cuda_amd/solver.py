@@ -1,0 +1,164 @@
+"""Python side of the C ABI: the reference driver's Simulation, case setup and step.
+
+Mirrors the reference's host flow (src/main.cu:7-40, src/simulation.cu:13-172):
+
+    sim = Simulation(configs.c1())     # setup_case + initialize_simulation + case initialize
+    sim.init()                         # gpu_initialize_Moments_GhostInterface
+    sim.step(100, forces=True)         # streaming, boundary_treatment, post_streaming,
+                                       # collision, post_collision (x100)
+    rho = sim.field("rho")             # copyMomentsDeviceToHost
+    F = sim.forces(0, 100)             # d_TotalFx/Fy/Fz/m per step
+
+All compute runs in the HIP library libmrlbm.so built from csrc/; there is no CPU fallback. If
+the library is missing this module raises at import of the first simulation.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import _abi
+from ._abi import CaseDesc, Info, as_ptr
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmrlbm.so")
+
+_lib = None
+
+
+class MrlbmError(RuntimeError):
+    def __init__(self, status, message):
+        super().__init__(f"mrlbm status {status}: {message}")
+        self.status = status
+
+
+def lib():
+    """Load libmrlbm.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise MrlbmError(_abi.ERR_DEVICE, f"{LIB_PATH} is missing: run python -m mrlbm_3d_cuda_amd.build")
+        _lib = _abi.bind(C.CDLL(LIB_PATH), _abi.PRODUCT_PROTOTYPES)
+        _abi.bind(_lib, {"mrlbm_create_geometry_only": (C.c_int, [C.POINTER(CaseDesc), C.POINTER(C.c_void_p)])})
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise MrlbmError(rc, lib().mrlbm_last_error().decode(errors="replace"))
+
+
+def default_desc(kind):
+    d = CaseDesc()
+    _check(lib().mrlbm_case_desc_default(kind, C.byref(d)))
+    return d
+
+
+class Simulation:
+    """One case on the GPU, owned by the HIP library."""
+
+    def __init__(self, desc: CaseDesc, geometry_only=False):
+        self.lib = lib()
+        self.desc = desc.copy()
+        h = C.c_void_p()
+        fn = self.lib.mrlbm_create_geometry_only if geometry_only else self.lib.mrlbm_create
+        _check(fn(C.byref(self.desc), C.byref(h)))
+        self.h = h
+        self.shape = (desc.nz, desc.ny, desc.nx)
+        self.n = desc.num_nodes
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.mrlbm_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- time stepping
+    def init(self):
+        _check(self.lib.mrlbm_init(self.h))
+        return self
+
+    def step(self, n=1, forces=False):
+        _check(self.lib.mrlbm_step(self.h, n, _abi.STEP_FORCES if forces else 0))
+        return self
+
+    def sync(self):
+        _check(self.lib.mrlbm_sync(self.h))
+
+    def time_steps(self, n, forces=False):
+        """Device time of n steps (ms) and mean per-launch ms of the fused and wall kernels."""
+        total = C.c_double()
+        per = (C.c_double * 2)()
+        _check(self.lib.mrlbm_time_steps(self.h, n, _abi.STEP_FORCES if forces else 0, C.byref(total), per))
+        return total.value, (per[0], per[1])
+
+    # -- state
+    def field(self, f):
+        fi = _abi.FIELDS.index(f) if isinstance(f, str) else int(f)
+        out = np.empty(self.n, dtype=np.float64)
+        _check(self.lib.mrlbm_get_moments(self.h, fi, as_ptr(out, C.c_double)))
+        return out.reshape(self.shape)
+
+    def fields(self):
+        return np.stack([self.field(f) for f in range(_abi.NUM_FIELDS)])
+
+    def set_field(self, f, values):
+        fi = _abi.FIELDS.index(f) if isinstance(f, str) else int(f)
+        arr = np.ascontiguousarray(values, dtype=np.float64).reshape(-1)
+        assert arr.size == self.n
+        _check(self.lib.mrlbm_set_moments(self.h, fi, as_ptr(arr, C.c_double)))
+
+    def forces(self, first, count):
+        out = np.empty((count, 4), dtype=np.float64)
+        _check(self.lib.mrlbm_get_forces(self.h, first, count, as_ptr(out, C.c_double)))
+        return out
+
+    def checkpoint_write(self, path):
+        _check(self.lib.mrlbm_checkpoint_write(self.h, os.fsencode(path)))
+
+    def checkpoint_read(self, path):
+        _check(self.lib.mrlbm_checkpoint_read(self.h, os.fsencode(path)))
+
+    # -- geometry
+    def info(self):
+        i = Info()
+        _check(self.lib.mrlbm_get_info(self.h, C.byref(i)))
+        return i
+
+    def node_types(self):
+        out = np.empty(self.n, dtype=np.uint32)
+        _check(self.lib.mrlbm_get_node_types(self.h, as_ptr(out, C.c_uint32)))
+        return out.reshape(self.shape)
+
+    def wall_table(self, group):
+        info = self.info()
+        n = info.nb_inner if group == 0 else info.nb_outer
+        node = np.empty(n, np.int64)
+        nx, ny, delta = (np.empty(n, np.float64) for _ in range(3))
+        fin, fout = np.empty(n, np.uint32), np.empty(n, np.uint32)
+        _check(self.lib.mrlbm_get_wall_table(self.h, group, as_ptr(node, C.c_int64), as_ptr(nx, C.c_double),
+                                             as_ptr(ny, C.c_double), as_ptr(delta, C.c_double),
+                                             as_ptr(fin, C.c_uint32), as_ptr(fout, C.c_uint32)))
+        return dict(node=node, nx=nx, ny=ny, delta=delta, in_mask=fin, out_mask=fout)
+
+    def bcfluid_table(self, group):
+        info = self.info()
+        n = info.nb_fluid_inner if group == 0 else info.nb_fluid_outer
+        node, tag = np.empty(n, np.int64), np.empty(n, np.uint32)
+        fin, fout = np.empty(256, np.uint32), np.empty(256, np.uint32)
+        _check(self.lib.mrlbm_get_bcfluid_table(self.h, group, as_ptr(node, C.c_int64), as_ptr(tag, C.c_uint32),
+                                                as_ptr(fin, C.c_uint32), as_ptr(fout, C.c_uint32)))
+        return dict(node=node, tag=tag, in_mask=fin, out_mask=fout)
+
+
+def algorithmic_bytes_per_update(precision):
+    """Read 10 moments + write 10 moments (B_alg)."""
+    return 20 * _abi.STORAGE_BYTES[precision]

@@ -1,0 +1,58 @@
+/*
+ * mrlbm_oracle.h - C API of the CPU oracle.
+ *
+ * TEST INFRASTRUCTURE ONLY. The oracle is a plain C++ (OpenMP) restatement of the reference's
+ * per-node update (sakthi-lbm/MRLBM_3D_CUDA, two-pass order: streaming + node BC, curved wall,
+ * incoming force, collision, outgoing force). It exists to check the HIP library and to time a
+ * CPU baseline. Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it;
+ * the product library never links, loads or calls it.
+ */
+#ifndef MRLBM_ORACLE_H
+#define MRLBM_ORACLE_H
+
+#include <stdint.h>
+
+#include "../include/mrlbm.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct oracle_sim oracle_sim;
+
+/* real_bytes: 8 runs the restatement in double, 4 in float (CPU-baseline timing only). */
+int oracle_create(const mrlbm_case_desc *desc, int32_t real_bytes, oracle_sim **out);
+int oracle_init(oracle_sim *s);
+int oracle_step(oracle_sim *s, int32_t nsteps, uint32_t flags);
+int oracle_get_moments(oracle_sim *s, int32_t field, double *host);
+int oracle_set_moments(oracle_sim *s, int32_t field, const double *host);
+int oracle_get_forces(oracle_sim *s, int64_t first, int32_t count, double *out4);
+int oracle_get_node_types(oracle_sim *s, uint32_t *out);
+int oracle_get_wall_table(oracle_sim *s, int32_t group, int64_t *node, double *nx, double *ny,
+                          double *delta, uint32_t *in_mask, uint32_t *out_mask);
+int oracle_get_bcfluid_table(oracle_sim *s, int32_t group, int64_t *node, uint32_t *tag,
+                             uint32_t *in_mask256, uint32_t *out_mask256);
+int oracle_get_info(oracle_sim *s, mrlbm_info *info);
+int oracle_destroy(oracle_sim *s);
+int32_t oracle_num_threads(void);
+
+/* Per-node building blocks, exposed for closed-form tests (double precision). */
+void oracle_pop_reconstruction(const double *m10, double *f27); /* m10[0] = rho (not rho-1) */
+void oracle_eval_moments(const double *f27, double *m10);       /* m10[0] = rho */
+void oracle_collide(double omega, double *m10);
+/* node BC of the given reference node type on 27 pulled pops; old10 holds the node's old
+ * moments (m10[0] = rho) and, for the outlet types, old10_xm1 the x-1 neighbour's. */
+int oracle_node_bc(uint32_t type, uint32_t tag, uint32_t in_mask, double omega, double u_in,
+                   const double *f27, const double *old10, const double *old10_xm1,
+                   double *out10);
+void oracle_incoming_rotated(double nx, double ny, uint32_t in_mask, const double *f27,
+                             double *rho, double *m6);
+void oracle_extrapolation_coeffs(double delta, double *c3);
+void oracle_curved_solution(double nx, double ny, uint32_t in_mask, const double *prime7,
+                            double rho_i, double mxy_i, double mxz_i, double *out10);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif
