@@ -14,7 +14,13 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libmrlbm.so")
 ARCH = os.environ.get("MRLBM_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["geometry.cpp", "kernels.hip", "capi.hip"]
+# per-source extra flags: the FP64 kernels are compiled without FMA contraction
+SOURCES = {
+    "geometry.cpp": [],
+    "kernels_f64.hip": ["-ffp-contract=off"],
+    "kernels_f32.hip": [],
+    "capi.hip": [],
+}
 
 HIPCC_FLAGS = [
     f"--offload-arch={ARCH}",
@@ -41,10 +47,10 @@ def build(verbose=False, extra_flags=()):
     objs = []
     obj_dir = os.path.join(HERE, "_build")
     os.makedirs(obj_dir, exist_ok=True)
-    for src in SOURCES:
+    for src, flags in SOURCES.items():
         obj = os.path.join(obj_dir, os.path.splitext(src)[0] + ".o")
         lang = ["-x", "hip"] if src.endswith(".cpp") else []
-        cmd = [hipcc(), *HIPCC_FLAGS, *extra_flags, *lang, "-c", os.path.join(CSRC, src), "-o", obj]
+        cmd = [hipcc(), *HIPCC_FLAGS, *flags, *extra_flags, *lang, "-c", os.path.join(CSRC, src), "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)
         subprocess.run(cmd, check=True)

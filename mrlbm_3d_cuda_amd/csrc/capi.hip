@@ -107,20 +107,14 @@ class Sim final : public SimBase {
                 return set_error(MRLBM_ERR_UNSUPPORTED, "no fused tile fits the device LDS");
         }
         tiles_x_ = (g.nx + bx_ - 1) / bx_;
-        const int tiles_y = (g.ny + by_ - 1) / by_;
-        ntiles_ = tiles_x_ * tiles_y;
-        lz_ = std::min(g.nz, 16);
-        if (const char *e = std::getenv("MRLBM_LZ")) lz_ = std::max(1, std::min(g.nz, std::atoi(e)));
-        const int nchunks = (g.nz + lz_ - 1) / lz_;
-        nblocks_ = ntiles_ * nchunks;
-        xcd_ = (nblocks_ % 8 == 0) ? 8 : 1;
-        if (const char *e = std::getenv("MRLBM_XCD_REMAP")) xcd_ = std::atoi(e) ? xcd_ : 1;
+        tiles_y_ = (g.ny + by_ - 1) / by_;
+        ntiles_ = tiles_x_ * tiles_y_;
 
         // ---- stencils of the wall nodes and donor marking
         std::vector<uint32_t> info = g.info;
         const int nwall = int(g.wall[0].node.size() + g.wall[1].node.size());
         std::vector<uint32_t> w_node, w_in, w_out, st_base;
-        std::vector<R> w_nx, w_ny, w_coef, st_frac;
+        std::vector<R> w_nx, w_ny, w_coef, st_frac, w_const;
         for (int gi = 0; gi < g.ngroups; ++gi) {
             const WallTable &w = g.wall[gi];
             for (size_t k = 0; k < w.node.size(); ++k) {
@@ -137,6 +131,9 @@ class Sim final : public SimBase {
                 w_coef.push_back(R(1) - R(0.5) * d2 + R(1.5) * dd);
                 w_coef.push_back(d2 - R(2) * dd);
                 w_coef.push_back(-R(0.5) * (d2 - dd));
+                double kc[WALL_CONSTS];
+                wall_constants(w.nx[k], w.ny[k], w.in_mask[k], kc);
+                for (int j = 0; j < WALL_CONSTS; ++j) w_const.push_back(R(kc[j]));
                 // wall point and the two fluid points along the normal (curvedLBM.cu:78-84)
                 const R xw = R(d.xc) + R(w.radius) * nx, yw = R(d.yc) + R(w.radius) * ny;
                 const R dir = R(w.fluid_dir);
@@ -170,15 +167,47 @@ class Sim final : public SimBase {
         nb_inner_ = int(g.wall[0].node.size());
         nbcf_ = int(bcf_node.size());
 
-        // per-block special flags in logical block order (chunk-major)
-        std::vector<uint8_t> special(nblocks_, 0);
-        for (int z = 0; z < g.nz; ++z)
-            for (int y = 0; y < g.ny; ++y)
-                for (int x = 0; x < g.nx; ++x)
-                    if (info[g.idx(x, y, z)] != NC_BULK) {
-                        const int tile = x / bx_ + tiles_x_ * (y / by_);
-                        special[size_t(z / lz_) * ntiles_ + tile] = 1;
+        // ---- work items of the fused pass. The z axis is cut into one slab per XCD (blocks are
+        // dispatched round-robin over the XCDs, so slab j runs on XCD j and its tiles share that
+        // L2); inside a slab every tile is cut into z chunks, shorter for tiles that hold
+        // boundary nodes (their blocks take the slow per-node path), and those chunks go first.
+        std::vector<uint8_t> tile_special(ntiles_, 0);
+        for (int64_t i = 0; i < g.n; ++i)
+            if (info[i] != NC_BULK) {
+                const int x = int(i % g.nx), y = int((i / g.nx) % g.ny);
+                tile_special[x / bx_ + tiles_x_ * (y / by_)] = 1;
+            }
+        int groups = (g.nz >= 32) ? 8 : 1;
+        if (const char *e = std::getenv("MRLBM_XCD_REMAP")) groups = std::atoi(e) ? groups : 1;
+        const int slab = (g.nz + groups - 1) / groups;
+        lz_ = std::min(slab, 32);
+        if (const char *e = std::getenv("MRLBM_LZ")) lz_ = std::max(1, std::min(slab, std::atoi(e)));
+        int lz_special = std::max(2, lz_ / 4);
+        if (const char *e = std::getenv("MRLBM_LZ_SPECIAL")) lz_special = std::max(1, std::min(slab, std::atoi(e)));
+        std::vector<std::vector<uint2>> lists(groups);
+        for (int gi = 0; gi < groups; ++gi) {
+            const int za = gi * slab, zb = std::min(g.nz, za + slab);
+            for (int pass = 0; pass < 2; ++pass)
+                for (int t = 0; t < ntiles_; ++t) {
+                    const bool sp = tile_special[t];
+                    if (sp != (pass == 0)) continue;
+                    const int step = sp ? lz_special : lz_;
+                    for (int z = za; z < zb; z += step) {
+                        const int z1 = std::min(zb, z + step);
+                        lists[gi].push_back(make_uint2(uint32_t(t) | (sp ? 0x80000000u : 0u),
+                                                       uint32_t(z) | (uint32_t(z1) << 16)));
                     }
+                }
+        }
+        size_t len = 0;
+        for (auto &l : lists) len = std::max(len, l.size());
+        std::vector<uint2> items;
+        for (auto &l : lists) {
+            l.resize(len, make_uint2(0u, 0u));
+            items.insert(items.end(), l.begin(), l.end());
+        }
+        nblocks_ = int(items.size());
+        xcd_ = groups;
 
         std::vector<R> wall_nx_all(std::max(nwall, 1)), wall_ny_all(std::max(nwall, 1));
         std::vector<uint32_t> wall_in_all(std::max(nwall, 1));
@@ -192,7 +221,8 @@ class Sim final : public SimBase {
 
         int rc;
         if ((rc = upload(d_info_, info))) return rc;
-        if ((rc = upload(d_special_, special))) return rc;
+        if ((rc = upload(d_items_, items))) return rc;
+        if ((rc = upload(d_wconst_, w_const))) return rc;
         if ((rc = upload(d_wnx_, wall_nx_all))) return rc;
         if ((rc = upload(d_wny_, wall_ny_all))) return rc;
         if ((rc = upload(d_win_, wall_in_all))) return rc;
@@ -323,13 +353,11 @@ class Sim final : public SimBase {
         a.nx = g.nx;
         a.ny = g.ny;
         a.nz = g.nz;
-        a.lz = lz_;
         a.tiles_x = tiles_x_;
-        a.ntiles = ntiles_;
         a.nblocks = nblocks_;
         a.xcd_groups = xcd_;
+        a.items = d_items_;
         a.info = d_info_;
-        a.special = d_special_;
         a.wall_nx = d_wnx_;
         a.wall_ny = d_wny_;
         a.wall_in = d_win_;
@@ -354,6 +382,7 @@ class Sim final : public SimBase {
             w.wnx = d_wnx_;
             w.wny = d_wny_;
             w.coef = d_wcoef_;
+            w.wconst = d_wconst_;
             w.in_mask = d_win_;
             w.out_mask = d_wout_;
             w.st_base = d_stbase_;
@@ -417,7 +446,7 @@ class Sim final : public SimBase {
 
     void release() {
         if (stream_) (void)hipStreamSynchronize(stream_);
-        void *ptrs[] = {buf_[0], buf_[1], donor_, d_info_, d_special_, d_wnx_, d_wny_, d_win_, d_wout_, d_wnode_,
+        void *ptrs[] = {buf_[0], buf_[1], donor_, d_info_, d_items_, d_wconst_, d_wnx_, d_wny_, d_win_, d_wout_, d_wnode_,
                         d_wcoef_, d_stbase_, d_stfrac_, d_bcfnode_, d_bcftag_, d_bcfin_, d_bcfout_, d_inlet_,
                         d_partials_, d_ticket_, d_ring_};
         for (void *p : ptrs)
@@ -433,10 +462,11 @@ class Sim final : public SimBase {
     int cur_ = 0;
     R omega_ = 0, rho0_ = 1;
     StoreScale<R> scale_{};
-    int bx_ = 32, by_ = 8, lz_ = 16, tiles_x_ = 1, ntiles_ = 1, nblocks_ = 1, xcd_ = 1;
+    int bx_ = 32, by_ = 8, lz_ = 16, tiles_x_ = 1, tiles_y_ = 1, ntiles_ = 1, nblocks_ = 1, xcd_ = 1;
     int nwall_ = 0, nb_inner_ = 0, nbcf_ = 0;
     uint32_t *d_info_ = nullptr;
-    uint8_t *d_special_ = nullptr;
+    uint2 *d_items_ = nullptr;
+    R *d_wconst_ = nullptr;
     R *d_wnx_ = nullptr, *d_wny_ = nullptr, *d_wcoef_ = nullptr, *d_stfrac_ = nullptr, *d_inlet_ = nullptr;
     uint32_t *d_win_ = nullptr, *d_wout_ = nullptr, *d_wnode_ = nullptr, *d_stbase_ = nullptr;
     uint32_t *d_bcfnode_ = nullptr, *d_bcfin_ = nullptr, *d_bcfout_ = nullptr;

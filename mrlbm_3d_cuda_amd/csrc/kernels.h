@@ -19,13 +19,13 @@ struct FusedArgs {
     S *__restrict__ donor;  // pre-collision snapshot of donor nodes (same layout)
     size_t stride;
     int nx, ny, nz;
-    int lz;          // z extent of one block's march
     int tiles_x;     // tiles along x
-    int ntiles;      // tiles per z chunk
-    int nblocks;     // tiles * chunks
+    int nblocks;     // work items, one block each
     int xcd_groups;  // blocks are spread round-robin over this many XCDs (1: no remap)
-    const uint32_t *__restrict__ info;     // per-node NodeCode | flags | payload
-    const uint8_t *__restrict__ special;   // per block: 1 if any node needs info[]
+    // work item b: x = tile index | (special << 31), y = z0 | (z1 << 16); special items (tiles
+    // with boundary nodes, which need info[]) come first and march shorter z ranges
+    const uint2 *__restrict__ items;
+    const uint32_t *__restrict__ info;  // per-node NodeCode | flags | payload
     const R *__restrict__ wall_nx;
     const R *__restrict__ wall_ny;
     const uint32_t *__restrict__ wall_in;
@@ -46,6 +46,7 @@ struct WallArgs {
     const R *__restrict__ wnx;
     const R *__restrict__ wny;
     const R *__restrict__ coef;        // 3 extrapolation coefficients per wall
+    const R *__restrict__ wconst;      // WALL_CONSTS closure constants per wall
     const uint32_t *__restrict__ in_mask;
     const uint32_t *__restrict__ out_mask;
     const uint32_t *__restrict__ st_base;  // 2 per wall: index of the (x0, y0) stencil corner
@@ -83,7 +84,10 @@ inline bool fused_tile_supported(int bx, int by) {
            (bx == 64 && by == 4) || (bx == 16 && by == 4);
 }
 
-// Bytes of dynamic LDS one fused block uses.
-size_t fused_lds_bytes(int bx, int by, size_t real_bytes);
+// Bytes of dynamic LDS one fused block uses: three planes of the c_z=+1 group, two of the
+// c_z=0 group and one of the c_z=-1 group, 9 populations each, over the tile and its ring.
+inline size_t fused_lds_bytes(int bx, int by, size_t real_bytes) {
+    return size_t(54) * size_t(bx + 2) * size_t(by + 2) * real_bytes;
+}
 
 }  // namespace mrlbm

@@ -1,4 +1,5 @@
-// kernels.hip - the two kernels of one time step.
+// kernels_impl.h - the kernels of one time step (instantiated by kernels_f64.hip and
+// kernels_f32.hip).
 //
 // fused_step: one pass per node that replaces the reference's streaming_and_evaluate_Mom and
 // collision_halo_update (solver/mlbm.cu:83-242). A block owns a BX x BY column of nodes and
@@ -15,6 +16,8 @@
 // cases/cylinder/cylinder_kernels.cu:68-128, cases/annulus/annulus_kernels.cu:75-139) with
 // deferred collision, and the momentum-exchange force / mass-flux reduction
 // (statistics/forces.cu:25-133) in FP64.
+#pragma once
+
 #include "kernels.h"
 
 namespace mrlbm {
@@ -45,17 +48,18 @@ __global__ __launch_bounds__(BX *BY) void fused_step_kernel(FusedArgs<R, S> a) {
     R *const g0 = lds + 27 * NH;
     R *const gm = lds + 45 * NH;
 
-    // block -> (tile, chunk); consecutive logical blocks share an XCD so that neighbouring
-    // tiles reuse each other's ring loads from the same L2
+    // block -> work item; consecutive items share an XCD so that neighbouring tiles reuse each
+    // other's ring loads from the same L2
     int b = blockIdx.x;
     if (a.xcd_groups > 1) b = (b % a.xcd_groups) * (a.nblocks / a.xcd_groups) + b / a.xcd_groups;
-    const int tile = b % a.ntiles;
-    const int chunk = b / a.ntiles;
+    const uint2 item = a.items[b];
+    const int tile = int(item.x & 0x7FFFFFFFu);
+    const bool special = (item.x >> 31) != 0;
     const int x0 = (tile % a.tiles_x) * BX;
     const int y0 = (tile / a.tiles_x) * BY;
-    const int z0 = chunk * a.lz;
-    const int z1 = min(a.nz, z0 + a.lz);
-    const bool special = a.special[b] != 0;
+    const int z0 = int(item.y & 0xFFFFu);
+    const int z1 = int(item.y >> 16);
+    if (z0 >= z1) return;  // padding item (uniform per block)
 
     const int tid = threadIdx.x;
     const int tx = tid % BX, ty = tid / BX;
@@ -147,39 +151,38 @@ __global__ __launch_bounds__(BX *BY) void fused_step_kernel(FusedArgs<R, S> a) {
                 bool do_collide = true;
                 if (code == NC_BULK) {
                     moments(f, M);
-                } else {
+                } else if (code == NC_KEEP || code == NC_FROZEN) {
+                    load_node<R, S>(a.src, a.stride, idx, a.scale, a.rho0, M);
+                    do_collide = code == NC_KEEP;
+                } else if (code == NC_WALL) {
+                    // u keeps its old value; rho and m become the incoming rotated moments
+                    load_node<R, S>(a.src, a.stride, idx, a.scale, a.rho0, M);
+                    incoming_rotated(a.wall_nx[payload], a.wall_ny[payload], a.wall_in[payload], f, M);
+                    do_collide = false;  // finished by wall_step
+                } else if (code == NC_BCF) {
+                    bc_fluid(payload, a.bcf_in[payload], f, M);
+                } else if (code == NC_WEST) {
                     R old[10];
                     load_node<R, S>(a.src, a.stride, idx, a.scale, a.rho0, old);
-#pragma unroll
-                    for (int k = 0; k < 10; ++k) M[k] = old[k];
-                    if (code == NC_FROZEN) {
-                        do_collide = false;
-                    } else if (code == NC_WALL) {
-                        incoming_rotated(a.wall_nx[payload], a.wall_ny[payload], a.wall_in[payload], f, M);
-                        do_collide = false;  // finished by wall_step
-                    } else if (code == NC_BCF) {
-                        bc_fluid(payload, a.bcf_in[payload], f, M);
-                    } else if (code == NC_WEST) {
-                        bc_west(f, old, a.inlet[uint32_t(y) + uint32_t(a.ny) * uint32_t(z)], M);
-                    } else if (code == NC_NORTH) {
-                        bc_ywall<face::K_N>(f, M);
-                    } else if (code == NC_SOUTH) {
-                        bc_ywall<face::K_S>(f, M);
-                    } else if (code == NC_NORTH_WEST) {
-                        bc_west_edge<face::K_W & face::K_N, -1>(f, a.omega, M);
-                    } else if (code == NC_SOUTH_WEST) {
-                        bc_west_edge<face::K_W & face::K_S, +1>(f, a.omega, M);
-                    } else if (code == NC_EAST || code == NC_NORTH_EAST || code == NC_SOUTH_EAST) {
-                        R oxm[10];
-                        load_node<R, S>(a.src, a.stride, idx - 1, a.scale, a.rho0, oxm);
-                        if (code == NC_EAST)
-                            bc_east(f, old, oxm, M);
-                        else if (code == NC_NORTH_EAST)
-                            bc_east_edge<face::K_E & face::K_N, +1>(f, old, oxm, M);
-                        else
-                            bc_east_edge<face::K_E & face::K_S, -1>(f, old, oxm, M);
-                    }
-                    // NC_KEEP: moments stay as loaded
+                    bc_west(f, old, a.inlet[uint32_t(y) + uint32_t(a.ny) * uint32_t(z)], M);
+                } else if (code == NC_NORTH) {
+                    bc_ywall<face::K_N>(f, M);
+                } else if (code == NC_SOUTH) {
+                    bc_ywall<face::K_S>(f, M);
+                } else if (code == NC_NORTH_WEST) {
+                    bc_west_edge<face::K_W & face::K_N, -1>(f, a.omega, M);
+                } else if (code == NC_SOUTH_WEST) {
+                    bc_west_edge<face::K_W & face::K_S, +1>(f, a.omega, M);
+                } else {  // NC_EAST, NC_NORTH_EAST, NC_SOUTH_EAST
+                    R old[10], oxm[10];
+                    load_node<R, S>(a.src, a.stride, idx, a.scale, a.rho0, old);
+                    load_node<R, S>(a.src, a.stride, idx - 1, a.scale, a.rho0, oxm);
+                    if (code == NC_EAST)
+                        bc_east(f, old, oxm, M);
+                    else if (code == NC_NORTH_EAST)
+                        bc_east_edge<face::K_E & face::K_N, +1>(f, old, oxm, M);
+                    else
+                        bc_east_edge<face::K_E & face::K_S, -1>(f, old, oxm, M);
                 }
                 if (info & INFO_DONOR) store_node<R, S>(a.donor, a.stride, idx, a.scale, a.rho0, M);
                 if (do_collide) collide(a.omega, M);
@@ -245,19 +248,6 @@ __device__ __forceinline__ void interp_rotate(const WallArgs<R, S> &a, uint32_t 
     vm[8] = myz * c - mxz * s;
 }
 
-template <class R>
-__device__ __forceinline__ void force_add(const R *M, uint32_t mask, double sign, double *F) {
-#pragma unroll
-    for (int q = 0; q < Q; ++q) {
-        if (!(mask & (1u << q))) continue;
-        const double fq = sign * double(population(M, q));
-        F[0] += fq * LATTICE.cx[q];
-        F[1] += fq * LATTICE.cy[q];
-        F[2] += fq * LATTICE.cz[q];
-        F[3] += fq;
-    }
-}
-
 constexpr int WALL_THREADS = 256;
 
 template <class R, class S>
@@ -281,12 +271,18 @@ __global__ __launch_bounds__(WALL_THREADS) void wall_step_kernel(WallArgs<R, S> 
         R p[7];
 #pragma unroll
         for (int j = 0; j < 7; ++j) p[j] = c0 * vw[j] + c1 * vm1[sel[j]] + c2 * vm2[sel[j]];
-        const uint32_t in = a.in_mask[i];
         R out[10];
-        wall_closure(c, s, in, p, M[0], M[7], M[8], out);
-        if (a.forces) force_add(out, in, 1.0, F);
+        if constexpr (kExactOrder<R>) {
+            wall_closure_loop(c, s, a.in_mask[i], p, M[0], M[7], M[8], out);
+        } else {
+            R k[WALL_CONSTS];
+#pragma unroll
+            for (int j = 0; j < WALL_CONSTS; ++j) k[j] = a.wconst[size_t(i) * WALL_CONSTS + j];
+            wall_closure(c, s, k, p, M[0], M[7], M[8], out);
+        }
+        if (a.forces) masked_force(out, a.in_mask[i], 1.0, F);
         collide(a.omega, out);
-        if (a.forces) force_add(out, a.out_mask[i], -1.0, F);
+        if (a.forces) masked_force(out, a.out_mask[i], -1.0, F);
         store_node<R, S>(a.mom, a.stride, idx, a.scale, a.rho0, out);
     } else if (a.forces && i < a.nwall + a.nbcf) {
         const int k = i - a.nwall;
@@ -294,9 +290,9 @@ __global__ __launch_bounds__(WALL_THREADS) void wall_step_kernel(WallArgs<R, S> 
         const uint32_t t = a.bcf_tag[k];
         R M[10];
         load_node<R, S>(a.donor, a.stride, idx, a.scale, a.rho0, M);
-        force_add(M, a.bcf_in[t], 1.0, F);
+        masked_force(M, a.bcf_in[t], 1.0, F);
         collide(a.omega, M);
-        force_add(M, a.bcf_out[t], -1.0, F);
+        masked_force(M, a.bcf_out[t], -1.0, F);
     }
     if (!a.forces) return;
     // block reduction in FP64: wave shuffles, then across the four waves through LDS
@@ -377,10 +373,6 @@ __global__ void decode_kernel(const S *__restrict__ in, size_t stride, double *_
 
 }  // namespace
 
-size_t fused_lds_bytes(int bx, int by, size_t real_bytes) {
-    return size_t(54) * size_t(bx + 2) * size_t(by + 2) * real_bytes;
-}
-
 template <class R, class S>
 void launch_fused(const FusedArgs<R, S> &a, int bx, int by, hipStream_t stream) {
     const size_t lds = fused_lds_bytes(bx, by, sizeof(R));
@@ -424,15 +416,10 @@ void launch_decode(const S *in, size_t stride, double *out, uint32_t n, const St
     hipLaunchKernelGGL((decode_kernel<R, S>), dim3((n + 255) / 256), dim3(256), 0, stream, in, stride, out, n, sc);
 }
 
-#define MRLBM_INSTANTIATE(R, S)                                                                            \
-    template void launch_fused<R, S>(const FusedArgs<R, S> &, int, int, hipStream_t);                      \
-    template void launch_wall<R, S>(const WallArgs<R, S> &, hipStream_t);                                  \
+}  // namespace mrlbm
+
+#define MRLBM_INSTANTIATE(R, S)                                                                             \
+    template void launch_fused<R, S>(const FusedArgs<R, S> &, int, int, hipStream_t);                       \
+    template void launch_wall<R, S>(const WallArgs<R, S> &, hipStream_t);                                   \
     template void launch_encode<R, S>(const double *, S *, size_t, uint32_t, const StoreScale<R> &, hipStream_t); \
     template void launch_decode<R, S>(const S *, size_t, double *, uint32_t, const StoreScale<R> &, hipStream_t);
-MRLBM_INSTANTIATE(double, double)
-MRLBM_INSTANTIATE(float, float)
-MRLBM_INSTANTIATE(float, __hip_bfloat16)
-MRLBM_INSTANTIATE(float, _Float16)
-#undef MRLBM_INSTANTIATE
-
-}  // namespace mrlbm

@@ -39,16 +39,15 @@ constexpr LatticeTables make_lattice() {
 
 constexpr LatticeTables LATTICE = make_lattice();
 
-// Weights w0..w3 by the number of non-zero components (8/27, 2/27, 1/54, 1/216).
-constexpr double W_REST = 8.0 / 27.0;
-constexpr double W_AXIS = 2.0 / 27.0;
-constexpr double W_FACE = 1.0 / 54.0;
-constexpr double W_CORNER = 1.0 / 216.0;
-
-constexpr double weight(int q) {
-    const int nz = (LATTICE.cx[q] != 0) + (LATTICE.cy[q] != 0) + (LATTICE.cz[q] != 0);
-    return nz == 0 ? W_REST : nz == 1 ? W_AXIS : nz == 2 ? W_FACE : W_CORNER;
-}
+// Weights as the product of the 1-D weights, w = w1(cx) w1(cy) w1(cz) with w1(0) = 2/3 and
+// w1(+-1) = 1/6 (8/27, 2/27, 1/54, 1/216 of latticeProperties.cuh:7-10). Evaluated left to right
+// in double, so every user of the table gets the same rounded values.
+constexpr double weight1(int c) { return c == 0 ? 2.0 / 3.0 : 1.0 / 6.0; }
+constexpr double weight(int q) { return weight1(LATTICE.cx[q]) * weight1(LATTICE.cy[q]) * weight1(LATTICE.cz[q]); }
+constexpr double W_REST = weight(0);
+constexpr double W_AXIS = weight(1);
+constexpr double W_FACE = weight(7);
+constexpr double W_CORNER = weight(19);
 
 // Populations grouped by the z component of their velocity. A population with c_z = +1 at
 // plane z was pulled from plane z-1, c_z = 0 from plane z, c_z = -1 from plane z+1.

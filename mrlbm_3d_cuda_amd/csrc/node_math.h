@@ -7,9 +7,30 @@
 
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "lattice.h"
 
 namespace mrlbm {
+
+// FP64 arithmetic follows one fixed expression order per quantity (the order of the plain
+// per-direction loops the reference writes), and the FP64 kernels are compiled without FMA
+// contraction, so FP64 results are reproducible bit for bit by any IEEE implementation of the
+// same loops. FP32 and the 16-bit storage forms use the faster pairwise forms below.
+template <class R>
+constexpr bool kExactOrder = std::is_same<R, double>::value;
+
+// Population q of the regularised distribution of M, per-direction loop form
+// (core/D3Q27/second_order/reconstruction.cuh:5-60 as a Hermite sum).
+template <class R>
+__device__ __forceinline__ R population(const R *M, int q) {
+    const R cx = R(LATTICE.cx[q]), cy = R(LATTICE.cy[q]), cz = R(LATTICE.cz[q]);
+    const R third = R(1) / R(3);
+    const R cu = cx * M[1] + cy * M[2] + cz * M[3];
+    const R hd = M[4] * (cx * cx - third) + M[5] * (cy * cy - third) + M[6] * (cz * cz - third);
+    const R ho = M[7] * cx * cy + M[8] * cx * cz + M[9] * cy * cz;
+    return R(weight(q)) * M[0] * (R(1) + R(3) * cu + R(4.5) * hd + R(9) * ho);
+}
 
 // ---------------------------------------------------------------- mask helpers
 
@@ -37,6 +58,21 @@ __device__ __forceinline__ R msum(const R *f) {
     return s;
 }
 
+// Sequential signed sum in increasing q: + over K & P, - over K & N (first term keeps its sign).
+template <uint32_t K, uint32_t P, uint32_t N, class R>
+__device__ __forceinline__ R mseq(const R *f) {
+    R s = R(0);
+    bool first = true;
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        const bool pos = (K & P) & (1u << q), neg = (K & N) & (1u << q);
+        if (!pos && !neg) continue;
+        s = first ? (pos ? f[q] : -f[q]) : (pos ? s + f[q] : s - f[q]);
+        first = false;
+    }
+    return s;
+}
+
 // ---------------------------------------------------------------- regularised populations
 
 // All 27 populations of the second-order regularised distribution of M
@@ -44,6 +80,11 @@ __device__ __forceinline__ R msum(const R *f) {
 // opposite directions f = w (S +- V) with S the even (rho, m) part and V = 3 rho c.u.
 template <class R>
 __device__ __forceinline__ void reconstruct(const R *M, R *f) {
+    if constexpr (kExactOrder<R>) {
+#pragma unroll
+        for (int q = 0; q < Q; ++q) f[q] = population(M, q);
+        return;
+    }
     const R rho = M[0];
     const R P = rho * (R(1) - R(1.5) * (M[4] + M[5] + M[6]));
     const R r3 = R(3) * rho, r45 = R(4.5) * rho, r9 = R(9) * rho;
@@ -76,6 +117,36 @@ __device__ __forceinline__ void reconstruct(const R *M, R *f) {
 // differences d of opposite pairs.
 template <class R>
 __device__ __forceinline__ void moments(const R *f, R *M) {
+    if constexpr (kExactOrder<R>) {
+        R rho = 0, jx = 0, jy = 0, jz = 0, pxx = 0, pyy = 0, pzz = 0, pxy = 0, pxz = 0, pyz = 0;
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            const int cx = LATTICE.cx[q], cy = LATTICE.cy[q], cz = LATTICE.cz[q];
+            rho += f[q];
+            jx += f[q] * R(cx);
+            jy += f[q] * R(cy);
+            jz += f[q] * R(cz);
+            pxx += f[q] * R(cx * cx);
+            pyy += f[q] * R(cy * cy);
+            pzz += f[q] * R(cz * cz);
+            pxy += f[q] * R(cx * cy);
+            pxz += f[q] * R(cx * cz);
+            pyz += f[q] * R(cy * cz);
+        }
+        const R inv = R(1) / rho;
+        const R cs2 = R(1) / R(3);
+        M[0] = rho;
+        M[1] = jx * inv;
+        M[2] = jy * inv;
+        M[3] = jz * inv;
+        M[4] = pxx * inv - cs2;
+        M[5] = pyy * inv - cs2;
+        M[6] = pzz * inv - cs2;
+        M[7] = pxy * inv;
+        M[8] = pxz * inv;
+        M[9] = pyz * inv;
+        return;
+    }
     R s[13], d[13];
 #pragma unroll
     for (int k = 0; k < 13; ++k) {
@@ -119,17 +190,6 @@ __device__ __forceinline__ void collide(R omega, R *M) {
     M[7] = om1 * M[7] + omega * M[1] * M[2];
     M[8] = om1 * M[8] + omega * M[1] * M[3];
     M[9] = om1 * M[9] + omega * M[2] * M[3];
-}
-
-// Population q of the regularised distribution of M (force sums over masked directions).
-template <class R>
-__device__ __forceinline__ R population(const R *M, int q) {
-    const R cx = R(LATTICE.cx[q]), cy = R(LATTICE.cy[q]), cz = R(LATTICE.cz[q]);
-    const R third = R(1) / R(3);
-    const R cu = cx * M[1] + cy * M[2] + cz * M[3];
-    const R hd = M[4] * (cx * cx - third) + M[5] * (cy * cy - third) + M[6] * (cz * cz - third);
-    const R ho = M[7] * cx * cy + M[8] * cx * cz + M[9] * cy * cz;
-    return R(weight(q)) * M[0] * (R(1) + R(3) * cu + R(4.5) * hd + R(9) * ho);
 }
 
 // ---------------------------------------------------------------- domain-face closed forms
@@ -200,7 +260,7 @@ __device__ __forceinline__ void bc_east(const R *f, const R *old, const R *old_x
     const R mzzI = msum<K_E & ZZ>(f) * inv - cs2;
     const R mxyI = mdiff<K_E, XY_P, XY_N>(f) * inv;
     const R mxzI = mdiff<K_E, XZ_P, XZ_N>(f) * inv;
-    const R myzI = mdiff<K_E, YZ_P, YZ_N>(f) * inv;
+    const R myzI = mseq<K_E, YZ_P, YZ_N>(f) * inv;  // boundaries.cuh:241 order
     outlet_blend(old, old_xm1, M);
     const R rho = M[0], uy = M[2], uz = M[3];
     M[4] = M[1] * M[1];
@@ -236,7 +296,7 @@ __device__ __forceinline__ void bc_west_edge(const R *f, R omega, R *M) {
     const R rI = msum<K>(f);
     const R inv = R(1) / rI;
     const R mxyI = (SGN > 0 ? msum<K & XY_P>(f) : -msum<K & XY_N>(f)) * inv;
-    const R myzI = mdiff<K, YZ_P, YZ_N>(f) * inv;
+    const R myzI = mseq<K, YZ_P, YZ_N>(f) * inv;
     R rho;
     if (SGN > 0)
         rho = R(36) * rI * (R(1) - mxyI + mxyI * omega) / (R(24) + omega);
@@ -258,13 +318,43 @@ __device__ __forceinline__ void bc_east_edge(const R *f, const R *old, const R *
     const R rI = msum<K>(f);
     const R inv = R(1) / rI;
     const R mxyI = (SGN > 0 ? msum<K & XY_P>(f) : -msum<K & XY_N>(f)) * inv;
-    const R myzI = mdiff<K, YZ_P, YZ_N>(f) * inv;
+    const R myzI = mseq<K, YZ_P, YZ_N>(f) * inv;
     outlet_blend(old, old_xm1, M);
     const R rho = M[0];
     M[4] = M[5] = M[6] = R(0);
     M[7] = (R(36) * mxyI * rI - R(SGN) * rho) / (R(9) * rho);
     M[8] = R(0);
     M[9] = (R(12) * myzI * rI) / (R(5) * rho);
+}
+
+// ---------------------------------------------------------------- masked lattice-frame sums
+
+// Zeroth, first and second lattice-frame moments of the populations selected by `mask`
+// (runtime bit set): S0 = sum f, J = sum f c, P = sum f c c (xx, yy, zz, xy, xz, yz).
+template <class R>
+struct MaskedSums {
+    R s0, jx, jy, jz, pxx, pyy, pzz, pxy, pxz, pyz;
+};
+
+template <class R>
+__device__ __forceinline__ MaskedSums<R> masked_sums(uint32_t mask, const R *f) {
+    MaskedSums<R> m{R(0), R(0), R(0), R(0), R(0), R(0), R(0), R(0), R(0), R(0)};
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        const R v = ((mask >> q) & 1u) ? f[q] : R(0);
+        const int cx = LATTICE.cx[q], cy = LATTICE.cy[q], cz = LATTICE.cz[q];
+        m.s0 += v;
+        if (cx) m.jx += cx > 0 ? v : -v;
+        if (cy) m.jy += cy > 0 ? v : -v;
+        if (cz) m.jz += cz > 0 ? v : -v;
+        if (cx) m.pxx += v;
+        if (cy) m.pyy += v;
+        if (cz) m.pzz += v;
+        if (cx && cy) m.pxy += cx * cy > 0 ? v : -v;
+        if (cx && cz) m.pxz += cx * cz > 0 ? v : -v;
+        if (cy && cz) m.pyz += cy * cz > 0 ? v : -v;
+    }
+    return m;
 }
 
 // ---------------------------------------------------------------- BCFLUID closed forms
@@ -274,35 +364,53 @@ __device__ __forceinline__ void bc_east_edge(const R *f, const R *old, const R *
 // sx mxz, sy myz), with the outputs mapped back the same way.
 template <class R>
 __device__ __forceinline__ void bc_fluid(uint32_t tag, uint32_t in_mask, const R *f, R *M) {
-    R rI = 0, ux = 0, uy = 0, uz = 0, mxx = 0, myy = 0, mzz = 0, mxy = 0, mxz = 0, myz = 0;
     const R third = R(1) / R(3);
-#pragma unroll
-    for (int q = 0; q < Q; ++q) {
-        if (!(in_mask & (1u << q))) continue;
-        const R fq = f[q];
-        rI += fq;
-        ux += fq * R(LATTICE.cx[q]);
-        uy += fq * R(LATTICE.cy[q]);
-        uz += fq * R(LATTICE.cz[q]);
-        mxx += fq * (R(LATTICE.cx[q] * LATTICE.cx[q]) - third);
-        myy += fq * (R(LATTICE.cy[q] * LATTICE.cy[q]) - third);
-        mzz += fq * (R(LATTICE.cz[q] * LATTICE.cz[q]) - third);
-        mxy += fq * R(LATTICE.cx[q] * LATTICE.cy[q]);
-        mxz += fq * R(LATTICE.cx[q] * LATTICE.cz[q]);
-        myz += fq * R(LATTICE.cy[q] * LATTICE.cz[q]);
-    }
-    const R inv = R(1) / rI;
     const R sx = (tag == 34u || tag == 136u) ? R(-1) : R(1);
     const R sy = (tag == 68u || tag == 136u) ? R(-1) : R(1);
-    ux = sx * (ux * inv);
-    uy = sy * (uy * inv);
-    uz = uz * inv;
-    mxx = mxx * inv;
-    myy = myy * inv;
-    mzz = mzz * inv;
-    mxy = (sx * sy) * (mxy * inv);
-    mxz = sx * (mxz * inv);
-    myz = sy * (myz * inv);
+    R rI, ux, uy, uz, mxx, myy, mzz, mxy, mxz, myz;
+    if constexpr (kExactOrder<R>) {
+        // per-direction accumulation of the incoming moments (boundaries.cuh:861-888)
+        R a[10] = {R(0), R(0), R(0), R(0), R(0), R(0), R(0), R(0), R(0), R(0)};
+        for (int q = 0; q < Q; ++q) {
+            if (!(in_mask & (1u << q))) continue;
+            const int cx = LATTICE.cx[q], cy = LATTICE.cy[q], cz = LATTICE.cz[q];
+            const R fq = f[q];
+            a[0] += fq;
+            a[1] += fq * R(cx);
+            a[2] += fq * R(cy);
+            a[3] += fq * R(cz);
+            a[4] += fq * (R(cx * cx) - third);
+            a[5] += fq * (R(cy * cy) - third);
+            a[6] += fq * (R(cz * cz) - third);
+            a[7] += fq * R(cx * cy);
+            a[8] += fq * R(cx * cz);
+            a[9] += fq * R(cy * cz);
+        }
+        rI = a[0];
+        const R inv = R(1) / rI;
+        ux = sx * (a[1] * inv);
+        uy = sy * (a[2] * inv);
+        uz = a[3] * inv;
+        mxx = a[4] * inv;
+        myy = a[5] * inv;
+        mzz = a[6] * inv;
+        mxy = (sx * sy) * (a[7] * inv);
+        mxz = sx * (a[8] * inv);
+        myz = sy * (a[9] * inv);
+    } else {
+        const MaskedSums<R> S = masked_sums(in_mask, f);
+        rI = S.s0;
+        const R inv = R(1) / rI;
+        ux = sx * (S.jx * inv);
+        uy = sy * (S.jy * inv);
+        uz = S.jz * inv;
+        mxx = S.pxx * inv - third;
+        myy = S.pyy * inv - third;
+        mzz = S.pzz * inv - third;
+        mxy = (sx * sy) * (S.pxy * inv);
+        mxz = sx * (S.pxz * inv);
+        myz = sy * (S.pyz * inv);
+    }
     // density: root of a rho^2 + b rho + c = 0 closest to rho_I (boundaries.cuh:905-915)
     const R a = R(-9998);
     const R b = R(6) * rI * (R(1734) + R(66) * mxx + R(198) * mxy + R(66) * myy + R(253) * ux + R(253) * uy);
@@ -316,16 +424,23 @@ __device__ __forceinline__ void bc_fluid(uint32_t tag, uint32_t in_mask, const R
     const R rho1 = (-b + sq) / (R(2) * a);
     const R rho2 = (-b - sq) / (R(2) * a);
     const R rho = (rho1 > R(0) && fabs(rho1 - rI) < fabs(rho2 - rI)) ? rho1 : rho2;
-    // closed forms, boundaries.cuh:916-924
-    const R uxo = (R(3) * mxx * rI + R(9) * mxy * rI + R(3) * myy * rI + R(20) * rI * ux + R(3) * rI * uy + rho) / (R(17) * rho);
-    const R uyo = (R(3) * mxx * rI + R(9) * mxy * rI + R(3) * myy * rI + R(3) * rI * ux + R(20) * rI * uy + rho) / (R(17) * rho);
-    const R uzo = (R(3) * rI * (mxz + myz + R(10) * uz)) / (R(29) * rho);
-    const R mxxo = (R(57) * mxx * rI + R(18) * mxy * rI + R(6) * myy * rI + R(6) * rI * ux + R(6) * rI * uy + R(2) * rho) / (R(51) * rho);
-    const R myyo = (R(6) * mxx * rI + R(18) * mxy * rI + R(57) * myy * rI + R(6) * rI * ux + R(6) * rI * uy + R(2) * rho) / (R(51) * rho);
+    // closed forms, boundaries.cuh:916-924 (FP64 divides as written, FP32 by reciprocals)
+    const R d17 = R(17) * rho, d29 = R(29) * rho, d51 = R(51) * rho;
+    auto div = [](R num, R den) {
+        if constexpr (kExactOrder<R>)
+            return num / den;
+        else
+            return num * (R(1) / den);
+    };
+    const R uxo = div(R(3) * mxx * rI + R(9) * mxy * rI + R(3) * myy * rI + R(20) * rI * ux + R(3) * rI * uy + rho, d17);
+    const R uyo = div(R(3) * mxx * rI + R(9) * mxy * rI + R(3) * myy * rI + R(3) * rI * ux + R(20) * rI * uy + rho, d17);
+    const R uzo = div(R(3) * rI * (mxz + myz + R(10) * uz), d29);
+    const R mxxo = div(R(57) * mxx * rI + R(18) * mxy * rI + R(6) * myy * rI + R(6) * rI * ux + R(6) * rI * uy + R(2) * rho, d51);
+    const R myyo = div(R(6) * mxx * rI + R(18) * mxy * rI + R(57) * myy * rI + R(6) * rI * ux + R(6) * rI * uy + R(2) * rho, d51);
     const R mzzo = (R(36) * mzz * rI) / (R(35) * rho);
-    const R mxyo = (R(3) * mxx * rI + R(26) * mxy * rI + R(3) * myy * rI + R(3) * rI * ux + R(3) * rI * uy + rho) / (R(17) * rho);
-    const R mxzo = (rI * (R(32) * mxz + R(3) * myz + uz)) / (R(29) * rho);
-    const R myzo = (rI * (R(3) * mxz + R(32) * myz + uz)) / (R(29) * rho);
+    const R mxyo = div(R(3) * mxx * rI + R(26) * mxy * rI + R(3) * myy * rI + R(3) * rI * ux + R(3) * rI * uy + rho, d17);
+    const R mxzo = div(rI * (R(32) * mxz + R(3) * myz + uz), d29);
+    const R myzo = div(rI * (R(3) * mxz + R(32) * myz + uz), d29);
     M[0] = rho;
     M[1] = sx * uxo;
     M[2] = sy * uyo;
@@ -340,42 +455,125 @@ __device__ __forceinline__ void bc_fluid(uint32_t tag, uint32_t in_mask, const R
 
 // ---------------------------------------------------------------- curved walls
 
-// Incoming density and rotated incoming second moments at a wall node (x' along the normal
-// (nx, ny)); evaluate_incoming_moments_rotated, boundary/curvedLBM.cu:5-59.
+// Incoming density and rotated incoming second moments at a wall node, x' along the normal
+// (nx, ny) and y' = (-ny, nx) (evaluate_incoming_moments_rotated, boundary/curvedLBM.cu:5-59).
+// The rotated sums are the rotation of the lattice-frame masked second-moment tensor.
 template <class R>
 __device__ __forceinline__ void incoming_rotated(R nx, R ny, uint32_t in_mask, const R *f, R *M) {
-    const R third = R(1) / R(3);
-    R r = 0, axx = 0, ayy = 0, azz = 0, axy = 0, axz = 0, ayz = 0;
-#pragma unroll
-    for (int q = 0; q < Q; ++q) {
-        if (!(in_mask & (1u << q))) continue;
-        const R cx = R(LATTICE.cx[q]), cy = R(LATTICE.cy[q]), cz = R(LATTICE.cz[q]);
-        const R xp = cx * nx + cy * ny, yp = -cx * ny + cy * nx;
-        const R fq = f[q];
-        r += fq;
-        axx += fq * (xp * xp - third);
-        ayy += fq * (yp * yp - third);
-        azz += fq * (cz * cz - third);
-        axy += fq * (xp * yp);
-        axz += fq * (xp * cz);
-        ayz += fq * (yp * cz);
+    if constexpr (kExactOrder<R>) {
+        const R third = R(1) / R(3);
+        R r = 0, axx = 0, ayy = 0, azz = 0, axy = 0, axz = 0, ayz = 0;
+        for (int q = 0; q < Q; ++q) {
+            if (!(in_mask & (1u << q))) continue;
+            const R cx = R(LATTICE.cx[q]), cy = R(LATTICE.cy[q]), cz = R(LATTICE.cz[q]);
+            const R xp = cx * nx + cy * ny, yp = -cx * ny + cy * nx, zp = cz;
+            const R fq = f[q];
+            r += fq;
+            axx += fq * (xp * xp - third);
+            ayy += fq * (yp * yp - third);
+            azz += fq * (zp * zp - third);
+            axy += fq * (xp * yp);
+            axz += fq * (xp * zp);
+            ayz += fq * (yp * zp);
+        }
+        const R inv = R(1) / r;
+        M[0] = r;
+        M[4] = axx * inv;
+        M[5] = ayy * inv;
+        M[6] = azz * inv;
+        M[7] = axy * inv;
+        M[8] = axz * inv;
+        M[9] = ayz * inv;
+        return;
     }
-    const R inv = R(1) / r;
-    M[0] = r;
-    M[4] = axx * inv;
-    M[5] = ayy * inv;
-    M[6] = azz * inv;
-    M[7] = axy * inv;
-    M[8] = axz * inv;
-    M[9] = ayz * inv;
+    const MaskedSums<R> S = masked_sums(in_mask, f);
+    const R inv = R(1) / S.s0;
+    const R third = R(1) / R(3);
+    const R nn = nx * ny, cc = nx * nx, ss = ny * ny;
+    const R txx = cc * S.pxx + R(2) * nn * S.pxy + ss * S.pyy;
+    const R tyy = ss * S.pxx - R(2) * nn * S.pxy + cc * S.pyy;
+    const R txy = nn * (S.pyy - S.pxx) + (cc - ss) * S.pxy;
+    const R txz = nx * S.pxz + ny * S.pyz;
+    const R tyz = nx * S.pyz - ny * S.pxz;
+    M[0] = S.s0;
+    M[4] = txx * inv - third;
+    M[5] = tyy * inv - third;
+    M[6] = S.pzz * inv - third;
+    M[7] = txy * inv;
+    M[8] = txz * inv;
+    M[9] = tyz * inv;
 }
 
-// Wall-frame closure (numerical_solution_rhoeq_rotated, boundary/numerical_solution.cu:4-122):
-// p holds the extrapolated u'x, u'y, u'z, m'xx, m'yy, m'zz, m'yz; rhoI, mxyI, mxzI the node's
-// incoming values. Solves the 2x2 system for m'xy, m'xz, rotates back and closes rho.
+// Geometry constants of the wall-frame closure of one wall node: masked sums over its incoming
+// directions of the weights times wall-frame Hermite tensors (c' rotated by the normal). With
+// them, numerical_solution_rhoeq_rotated (boundary/numerical_solution.cu:4-122) is
+//   E'   = G0 + 3 u'.G1 + 4.5 u'u' : G2
+//   D'ab = G2ab + 3 u'.Dab      (ab = xy, xz)
+//   F'ab,cd = 4.5 sum w H'cd H'ab
+// and the 2x2 solve for m'xy, m'xz. Computed on the host once per wall node.
+constexpr int WALL_CONSTS = 28;
+// layout: [0] G0, [1..3] G1, [4..9] G2 (xx yy zz xy xz yz), [10..12] Dxy, [13..15] Dxz,
+//         [16..21] Fxy (xx yy zz xy xz yz), [22..27] Fxz
+inline void wall_constants(double nx, double ny, uint32_t in_mask, double *k) {
+    for (int i = 0; i < WALL_CONSTS; ++i) k[i] = 0.0;
+    const double third = 1.0 / 3.0;
+    for (int q = 0; q < Q; ++q) {
+        if (!(in_mask & (1u << q))) continue;
+        const double cx = LATTICE.cx[q], cy = LATTICE.cy[q], cz = LATTICE.cz[q];
+        const double xp = cx * nx + cy * ny, yp = -cx * ny + cy * nx, zp = cz;
+        const double H[6] = {xp * xp - third, yp * yp - third, zp * zp - third, xp * yp, xp * zp, yp * zp};
+        const double w = weight(q);
+        const double cp[3] = {xp, yp, zp};
+        k[0] += w;
+        for (int a = 0; a < 3; ++a) {
+            k[1 + a] += w * cp[a];
+            k[10 + a] += w * cp[a] * H[3];
+            k[13 + a] += w * cp[a] * H[4];
+        }
+        for (int b = 0; b < 6; ++b) {
+            k[4 + b] += w * H[b];
+            k[16 + b] += 4.5 * w * H[b] * H[3];
+            k[22 + b] += 4.5 * w * H[b] * H[4];
+        }
+    }
+}
+
+// Wall-frame closure with precomputed constants k (see wall_constants): p holds the extrapolated
+// u'x, u'y, u'z, m'xx, m'yy, m'zz, m'yz; rhoI, mxyI, mxzI the node's incoming values. Solves for
+// m'xy, m'xz, rotates back to the lattice frame and closes rho = rho_I / E'.
 template <class R>
-__device__ __forceinline__ void wall_closure(R nx, R ny, uint32_t in_mask, const R *p, R rhoI, R mxyI, R mxzI,
-                                             R *M) {
+__device__ __forceinline__ void wall_closure(R nx, R ny, const R *k, const R *p, R rhoI, R mxyI, R mxzI, R *M) {
+    const R c = nx, s = ny;
+    const R s2 = R(2) * s * c, c2 = c * c - s * s;
+    const R ux = p[0], uy = p[1], uz = p[2], mxx = p[3], myy = p[4], mzz = p[5], myz = p[6];
+    const R E = k[0] + R(3) * (ux * k[1] + uy * k[2] + uz * k[3]) +
+                R(4.5) * (ux * ux * k[4] + uy * uy * k[5] + uz * uz * k[6] +
+                          R(2) * (ux * uy * k[7] + ux * uz * k[8] + uy * uz * k[9]));
+    const R Dxy = k[7] + R(3) * (ux * k[10] + uy * k[11] + uz * k[12]);
+    const R Dxz = k[8] + R(3) * (ux * k[13] + uy * k[14] + uz * k[15]);
+    const R a1 = R(2) * k[19], b1 = R(2) * k[20];
+    const R c1 = (E * mxyI - Dxy) - mxx * k[16] - myy * k[17] - mzz * k[18] - R(2) * myz * k[21];
+    const R a2 = R(2) * k[25], b2 = R(2) * k[26];
+    const R cc = (E * mxzI - Dxz) - mxx * k[22] - myy * k[23] - mzz * k[24] - R(2) * myz * k[27];
+    const R inv = R(1) / (a2 * b1 - a1 * b2);
+    const R mxyp = (b1 * cc - b2 * c1) * inv;
+    const R mxzp = (a2 * c1 - a1 * cc) * inv;
+    M[1] = ux * c - uy * s;
+    M[2] = ux * s + uy * c;
+    M[3] = uz;
+    M[4] = mxx * c * c + myy * s * s - mxyp * s2;
+    M[5] = mxx * s * s + myy * c * c + mxyp * s2;
+    M[6] = mzz;
+    M[7] = (mxx - myy) * R(0.5) * s2 + mxyp * c2;
+    M[8] = mxzp * c - myz * s;
+    M[9] = mxzp * s + myz * c;
+    M[0] = rhoI / E;
+}
+
+// Per-direction loop form of the same closure (numerical_solution.cu:37-121), used in FP64.
+template <class R>
+__device__ __forceinline__ void wall_closure_loop(R nx, R ny, uint32_t in_mask, const R *p, R rhoI, R mxyI, R mxzI,
+                                                  R *M) {
     const R c = nx, s = ny, third = R(1) / R(3);
     const R s2 = R(2) * s * c, c2 = c * c - s * s;
     const R ux = p[0], uy = p[1], uz = p[2], mxx = p[3], myy = p[4], mzz = p[5], myz = p[6];
@@ -425,6 +623,19 @@ __device__ __forceinline__ void wall_closure(R nx, R ny, uint32_t in_mask, const
     M[8] = mxzp * c - myz * s;
     M[9] = mxzp * s + myz * c;
     M[0] = rhoI / E;
+}
+
+// Momentum and mass carried by the populations of M selected by mask (compute_forces_mass,
+// statistics/forces.cu:3-23), accumulated in FP64 with the given sign.
+template <class R>
+__device__ __forceinline__ void masked_force(const R *M, uint32_t mask, double sign, double *F) {
+    R f[Q];
+    reconstruct(M, f);
+    const MaskedSums<R> S = masked_sums(mask, f);
+    F[0] += sign * double(S.jx);
+    F[1] += sign * double(S.jy);
+    F[2] += sign * double(S.jz);
+    F[3] += sign * double(S.s0);
 }
 
 }  // namespace mrlbm

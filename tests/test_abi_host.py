@@ -1,0 +1,125 @@
+"""CPU checks of the C ABI and the host-side case setup (no GPU needed).
+
+* libmrlbm.so loads and exports every entry point include/mrlbm.h declares;
+* the ctypes structures match the C header's layout (sizeof/offsetof from a gcc-compiled probe);
+* the host geometry (node types, wall tables, BCFLUID masks) equals the oracle's bit for bit;
+* descriptor validation returns the documented status codes instead of exiting.
+"""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from mrlbm_3d_cuda_amd import _abi, configs
+from mrlbm_3d_cuda_amd.solver import MrlbmError, Simulation, lib
+from oracle.binding import Oracle
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "mrlbm.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?[\w\s\*]+?\b(mrlbm_\w+)\s*\(", text, flags=re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    names = declared_functions()
+    assert len(names) >= 19
+    L = lib()
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+    assert L.mrlbm_abi_version() == _abi.ABI_VERSION
+
+
+def test_struct_layout_matches_header(tmp_path):
+    probe = tmp_path / "probe.c"
+    fields = [n for n, _ in _abi.CaseDesc._fields_]
+    ifields = [n for n, _ in _abi.Info._fields_]
+    lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', 'int main(void) {',
+             'printf("%zu %zu\\n", sizeof(mrlbm_case_desc), sizeof(mrlbm_info));']
+    lines += [f'printf("d {f} %zu\\n", offsetof(mrlbm_case_desc, {f}));' for f in fields]
+    lines += [f'printf("i {f} %zu\\n", offsetof(mrlbm_info, {f}));' for f in ifields]
+    lines += ['return 0; }']
+    probe.write_text("\n".join(lines))
+    exe = tmp_path / "probe"
+    subprocess.run(["gcc", "-std=c11", str(probe), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    sizes = out[0].split()
+    assert int(sizes[0]) == C.sizeof(_abi.CaseDesc)
+    assert int(sizes[1]) == C.sizeof(_abi.Info)
+    for line in out[1:]:
+        if not line:
+            continue
+        kind, name, off = line.split()
+        struct = _abi.CaseDesc if kind == "d" else _abi.Info
+        assert getattr(struct, name).offset == int(off), name
+
+
+CASES = {
+    "c1_annulus": configs.c1(),
+    "c2_channel": configs.c2(precision=_abi.FP64),
+    "small_annulus": configs.annulus(nx=40, ny=40, nz=8, xc=20.0, yc=20.0, r_in=7.0, r_out=17.0),
+    "offcentre_annulus": configs.annulus(nx=48, ny=44, nz=4, xc=23.5, yc=21.25, r_in=8.3, r_out=19.6),
+    "small_channel": configs.cylinder(nx=96, ny=48, nz=8, diameter=10.0, xc=24.0, yc=24.0,
+                                      precision=_abi.FP64),
+    "periodic_y_channel": configs.cylinder(nx=64, ny=40, nz=4, diameter=9.0, xc=20.5, yc=19.0,
+                                           periodic_y=1, precision=_abi.FP64),
+    "box": configs.periodic_box(17, 9, 5),
+}
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_host_geometry_equals_oracle(name):
+    d = CASES[name]
+    s = Simulation(d, geometry_only=True)
+    o = Oracle(d)
+    assert np.array_equal(s.node_types(), o.node_types())
+    si, oi = s.info(), o.info()
+    for f in ("nb_inner", "nb_outer", "nb_fluid_inner", "nb_fluid_outer", "active_tiles", "total_tiles"):
+        assert getattr(si, f) == getattr(oi, f), f
+    groups = {0: [], 1: [0], 2: [0, 1]}[d.kind]
+    for g in groups:
+        ws, wo = s.wall_table(g), o.wall_table(g)
+        for k in ws:
+            assert np.array_equal(ws[k], wo[k]), (g, k)
+        bs, bo = s.bcfluid_table(g), o.bcfluid_table(g)
+        for k in bs:
+            assert np.array_equal(bs[k], bo[k]), (g, k)
+
+
+@pytest.mark.parametrize("field,value,status", [
+    ("tau", 0.5, _abi.ERR_TAU),
+    ("tau", 2.6, _abi.ERR_TAU),
+    ("u_max", 0.3, _abi.ERR_MACH),
+    ("kind", 7, _abi.ERR_INVALID),
+    ("nx", 2, _abi.ERR_INVALID),
+    ("periodic_z", 0, _abi.ERR_UNSUPPORTED),
+    ("r_outer", 5.0, _abi.ERR_GEOMETRY),
+])
+def test_descriptor_validation_status(field, value, status):
+    d = configs.c1()
+    setattr(d, field, value)
+    with pytest.raises(MrlbmError) as e:
+        Simulation(d, geometry_only=True)
+    assert e.value.status == status
+    assert lib().mrlbm_last_error()
+
+
+def test_geometry_only_rejects_device_calls():
+    s = Simulation(configs.c1(), geometry_only=True)
+    with pytest.raises(MrlbmError) as e:
+        s.init()
+    assert e.value.status == _abi.ERR_INVALID
+
+
+def test_default_descriptors_match_configs():
+    from mrlbm_3d_cuda_amd.solver import default_desc
+    for kind, ref in ((_abi.CASE_ANNULUS, configs.c1()), (_abi.CASE_CYLINDER, configs.c2())):
+        d = default_desc(kind)
+        for name in ("nx", "ny", "nz", "xc", "yc", "r_inner", "r_outer", "tau", "u_max", "precision",
+                     "periodic_x", "periodic_y", "periodic_z", "inlet_profile", "seed"):
+            assert getattr(d, name) == pytest.approx(getattr(ref, name)), name
