@@ -38,6 +38,23 @@ def read_traffic(path, workload_key):
     return None
 
 
+def copy_calibration(nbytes, reps=5):
+    """Device-to-device copies of a known size, for scaling the FETCH_SIZE / WRITE_SIZE counters."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(a, 1, C.c_size_t(nbytes)) == 0
+    for _ in range(reps):
+        assert hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps({"copy_calibration_bytes": nbytes, "reps": reps}), flush=True)
+    return 0
+
+
 def cpu_baseline(desc, steps=20, warmup=2):
     from oracle import binding
 
@@ -66,7 +83,14 @@ def main():
     ap.add_argument("--precision", default="f32", choices=["f64", "f32", "bf16", "fp16"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=20)
+    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "box"],
+                    help="c2: confined cylinder 256x128x64 (default); c3: wake 512x256x256; "
+                         "box: periodic 512x256x256 (no boundaries)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only run device copies of this size (counter calibration)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        return copy_calibration(args.copy_calibration)
 
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: --gpus {args.gpus} is not supported (this build runs the "
@@ -76,15 +100,27 @@ def main():
     from mrlbm_3d_cuda_amd.solver import Simulation, algorithmic_bytes_per_update
 
     prec = {"f64": _abi.FP64, "f32": _abi.FP32, "bf16": _abi.FP32_BF16, "fp16": _abi.FP32_FP16}[args.precision]
-    desc = configs.c2(precision=prec)
+    if args.workload == "c2":
+        desc = configs.c2(precision=prec)
+        workload = WORKLOAD
+    elif args.workload == "c3":
+        desc = configs.c3(precision=prec)
+        workload = "cylinder wake 512x256x256 D3Q27, per-step Cd/Cl"
+    else:
+        desc = configs.periodic_box(512, 256, 256, precision=prec)
+        workload = "periodic box 512x256x256 D3Q27"
     n = desc.num_nodes
-    print(f"[bench] workload={WORKLOAD} precision={args.precision} nodes={n} warmup={args.warmup} "
+    print(f"[bench] workload={workload} precision={args.precision} nodes={n} warmup={args.warmup} "
           f"steps={args.steps}", file=sys.stderr, flush=True)
     sim = Simulation(desc).init()
     sim.step(args.warmup, forces=True)
     sim.sync()
-    ms_total, (ms_fused, ms_wall) = sim.time_steps(args.steps, forces=True)
-    steps_done = args.warmup + args.steps
+    # timed region: K steps as the library runs them (captured step graphs), HIP events around
+    ms_total, _ = sim.time_steps(args.steps, forces=True, per_kernel=False)
+    # per-launch kernel durations: the same K steps queued one by one with HIP events around
+    # every launch on the simulation's stream
+    ms_eager, (ms_fused, ms_wall) = sim.time_steps(args.steps, forces=True, per_kernel=True)
+    steps_done = args.warmup + 2 * args.steps
     forces = sim.forces(steps_done - min(args.steps, 200), min(args.steps, 200))
     sim.close()
 
@@ -93,10 +129,10 @@ def main():
     b_alg = algorithmic_bytes_per_update(prec)
     bytes_per_launch = b_alg * n
     achieved = bytes_per_launch / (ms_fused / 1e3) / 1e9
-    key = f"c2_{args.precision}"
-    traffic = read_traffic(os.path.join(ROOT, "profiles", "fused_step_c2_cylinder.md"), key)
+    key = f"{args.workload}_{args.precision}"
+    traffic = read_traffic(os.path.join(ROOT, "profiles", "fused_step_traffic.md"), key)
     # drag and lift coefficients: Cd = 2 Fx / (rho0 U^2 D Lz), U = u_max
-    U, D, Lz = desc.u_max, 2 * desc.r_inner, desc.nz
+    U, D, Lz = desc.u_max, max(2 * desc.r_inner, 1.0), desc.nz
     cd = 2 * forces[:, 0] / (desc.rho0 * U * U * D * Lz)
     cl = 2 * forces[:, 1] / (desc.rho0 * U * U * D * Lz)
 
@@ -115,13 +151,14 @@ def main():
             args.precision],
         "data": "synthetic (seeded initial velocity noise 1e-4; no dataset)",
         "config": {
-            "workload": WORKLOAD if args.precision == "f32" else WORKLOAD.replace("FP32", args.precision.upper()),
+            "workload": workload + f", {args.precision} storage",
             "grid": [desc.nx, desc.ny, desc.nz],
             "nodes": n,
             "tau": desc.tau,
             "precision": args.precision,
             "parallelism": "single GPU",
             "hbm_pct_of_peak_whole_step": round(100 * mlups * 1e6 * b_alg / (HBM_PEAK_GBS * 1e9), 2),
+            "ms_per_step_eager_with_events": round(ms_eager / args.steps, 5),
             "ms_fused_kernel": round(ms_fused, 5),
             "ms_wall_kernel": round(ms_wall, 5),
             "cd_mean_last200": float(cd.mean()),

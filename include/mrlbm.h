@@ -201,7 +201,12 @@ int mrlbm_get_info(mrlbm_sim *sim, mrlbm_info *info);
 int mrlbm_time_steps(mrlbm_sim *sim, int32_t nsteps, uint32_t flags, double *ms_total,
                      double *ms_kernels);
 
-/* Layout-independent checkpoint with header and checksum. */
+/*
+ * Checkpoint: a header (magic, version, precision, grid, step count, FNV-1a checksum) and the
+ * moments exactly as stored on the device, field-major without padding, so a restart continues
+ * bit for bit. Reading rejects a file of another precision or grid, trailing bytes or a bad
+ * checksum with MRLBM_ERR_CORRUPT, and a missing file with MRLBM_ERR_IO.
+ */
 int mrlbm_checkpoint_write(mrlbm_sim *sim, const char *path);
 int mrlbm_checkpoint_read(mrlbm_sim *sim, const char *path);
 

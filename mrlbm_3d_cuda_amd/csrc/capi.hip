@@ -33,7 +33,7 @@ int set_error(int code, const std::string &msg) {
             return set_error(MRLBM_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
-constexpr int FORCE_RING = 1 << 16;  // steps of force history kept on the device
+using mrlbm::FORCE_RING_STEPS;
 
 }  // namespace
 
@@ -47,9 +47,13 @@ class SimBase {
     virtual int step(int n, uint32_t flags) = 0;
     virtual int get(int f, double *h) = 0;
     virtual int set(int f, const double *h) = 0;
+    // the current moments in the storage type, field-major, 10 * n values (checkpoint payload)
+    virtual int get_raw(void *h) = 0;
+    virtual int set_raw(const void *h) = 0;
     virtual int get_forces(int64_t first, int count, double *out4) = 0;
     virtual int time_steps(int n, uint32_t flags, double *ms_total, double *ms_k) = 0;
     virtual int sync() = 0;
+    virtual int set_steps(int64_t n) = 0;
     mrlbm_case_desc d{};
     Geometry g;
     int64_t steps = 0;
@@ -97,13 +101,16 @@ class Sim final : public SimBase {
         int dev = 0, max_lds = 0;
         HIP_TRY(hipGetDevice(&dev));
         HIP_TRY(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
-        if (fused_lds_bytes(bx_, by_, sizeof(R)) > size_t(max_lds)) {
+        if (const char *e = std::getenv("MRLBM_FUSED_VARIANT")) variant_ = std::atoi(e) == 1 ? 1 : 0;
+        if (const char *e = std::getenv("MRLBM_STOCHASTIC_ROUNDING")) stochastic_ = Conv<S>::scaled && std::atoi(e) != 0;
+        if (const char *e = std::getenv("MRLBM_GRAPH_STEPS")) graph_steps_ = std::max(0, std::atoi(e) / 2 * 2);
+        if (fused_lds_bytes(bx_, by_, sizeof(R), variant_) > size_t(max_lds)) {
             if (std::getenv("MRLBM_TILE"))
                 return set_error(MRLBM_ERR_UNSUPPORTED, "fused tile needs more LDS than a block may use");
             // default shape too large for this device: halve the tile height
             bx_ = sizeof(R) == 8 ? 16 : 32;
             by_ = 4;
-            if (fused_lds_bytes(bx_, by_, sizeof(R)) > size_t(max_lds))
+            if (fused_lds_bytes(bx_, by_, sizeof(R), variant_) > size_t(max_lds))
                 return set_error(MRLBM_ERR_UNSUPPORTED, "no fused tile fits the device LDS");
         }
         tiles_x_ = (g.nx + bx_ - 1) / bx_;
@@ -241,8 +248,10 @@ class Sim final : public SimBase {
         HIP_TRY(hipMalloc(&d_partials_, size_t(wall_blocks) * 4 * sizeof(double)));
         HIP_TRY(hipMalloc(&d_ticket_, sizeof(unsigned)));
         HIP_TRY(hipMemsetAsync(d_ticket_, 0, sizeof(unsigned), stream_));
-        HIP_TRY(hipMalloc(&d_ring_, size_t(FORCE_RING) * 4 * sizeof(double)));
-        HIP_TRY(hipMemsetAsync(d_ring_, 0, size_t(FORCE_RING) * 4 * sizeof(double), stream_));
+        HIP_TRY(hipMalloc(&d_ring_, size_t(FORCE_RING_STEPS) * 4 * sizeof(double)));
+        HIP_TRY(hipMalloc(&d_step_counter_, sizeof(unsigned long long)));
+        HIP_TRY(hipMemsetAsync(d_step_counter_, 0, sizeof(unsigned long long), stream_));
+        HIP_TRY(hipMemsetAsync(d_ring_, 0, size_t(FORCE_RING_STEPS) * 4 * sizeof(double), stream_));
         HIP_TRY(hipStreamSynchronize(stream_));
         return MRLBM_OK;
     }
@@ -263,13 +272,32 @@ class Sim final : public SimBase {
         }
         int rc = put_all(h);
         if (rc) return rc;
-        steps = 0;
+        return set_steps(0);
+    }
+
+    // Steps are queued eagerly for short runs; longer runs replay a captured graph of
+    // graph_steps_ steps (an even number, so the ping-pong parity is unchanged).
+    int step(int nsteps, uint32_t flags) override {
+        int done = 0;
+        if (graph_steps_ > 0 && launched_) {
+            while (nsteps - done >= graph_steps_) {
+                hipGraphExec_t exec = nullptr;
+                int rc = graph_for(flags, exec);
+                if (rc) return rc;
+                HIP_TRY(hipGraphLaunch(exec, stream_));
+                steps += graph_steps_;
+                done += graph_steps_;
+            }
+        }
+        for (; done < nsteps; ++done) queue_step(flags, nullptr);
+        HIP_TRY(hipGetLastError());
         return MRLBM_OK;
     }
 
-    int step(int nsteps, uint32_t flags) override {
-        for (int s = 0; s < nsteps; ++s) queue_step(flags, nullptr);
-        HIP_TRY(hipGetLastError());
+    int set_steps(int64_t n) override {
+        steps = n;
+        const unsigned long long v = static_cast<unsigned long long>(n);
+        HIP_TRY(hipMemcpy(d_step_counter_, &v, sizeof(v), hipMemcpyHostToDevice));
         return MRLBM_OK;
     }
 
@@ -286,6 +314,20 @@ class Sim final : public SimBase {
         return MRLBM_OK;
     }
 
+    int get_raw(void *h) override {
+        HIP_TRY(hipStreamSynchronize(stream_));
+        HIP_TRY(hipMemcpy2D(h, size_t(g.n) * sizeof(S), buf_[cur_], stride_ * sizeof(S), size_t(g.n) * sizeof(S), 10,
+                            hipMemcpyDeviceToHost));
+        return MRLBM_OK;
+    }
+
+    int set_raw(const void *h) override {
+        HIP_TRY(hipStreamSynchronize(stream_));
+        HIP_TRY(hipMemcpy2D(buf_[cur_], stride_ * sizeof(S), h, size_t(g.n) * sizeof(S), size_t(g.n) * sizeof(S), 10,
+                            hipMemcpyHostToDevice));
+        return MRLBM_OK;
+    }
+
     int set(int f, const double *h) override {
         std::vector<double> all;
         int rc = get_all(all);
@@ -295,11 +337,11 @@ class Sim final : public SimBase {
     }
 
     int get_forces(int64_t first, int count, double *out4) override {
-        if (count < 0 || first < 0 || first + count > steps || first < steps - FORCE_RING)
+        if (count < 0 || first < 0 || first + count > steps || first < steps - FORCE_RING_STEPS)
             return set_error(MRLBM_ERR_INVALID, "force record range not available");
         HIP_TRY(hipStreamSynchronize(stream_));
         for (int k = 0; k < count; ++k) {
-            const int slot = int((first + k) % FORCE_RING);
+            const int slot = int((first + k) % FORCE_RING_STEPS);
             HIP_TRY(hipMemcpy(out4 + 4 * k, d_ring_ + 4 * slot, 4 * sizeof(double), hipMemcpyDeviceToHost));
         }
         return MRLBM_OK;
@@ -311,7 +353,12 @@ class Sim final : public SimBase {
         for (auto &e : ev) HIP_TRY(hipEventCreate(&e));
         HIP_TRY(hipStreamSynchronize(stream_));
         HIP_TRY(hipEventRecord(ev[0], stream_));
-        for (int s = 0; s < n; ++s) queue_step(flags, ms_k ? &ev[2 + 4 * size_t(s)] : nullptr);
+        if (ms_k) {
+            for (int s = 0; s < n; ++s) queue_step(flags, &ev[2 + 4 * size_t(s)]);
+        } else {
+            int rc = step(n, flags);
+            if (rc) return rc;
+        }
         HIP_TRY(hipEventRecord(ev[1], stream_));
         HIP_TRY(hipEventSynchronize(ev[1]));
         HIP_TRY(hipGetLastError());
@@ -339,8 +386,32 @@ class Sim final : public SimBase {
     }
 
   private:
+    // The wall pass also advances the device step counter, which seeds the stochastic rounding of
+    // the 16-bit forms, so those run it even without walls.
     bool has_wall_launch(uint32_t flags) const {
-        return nwall_ > 0 || ((flags & MRLBM_STEP_FORCES) && nbcf_ > 0);
+        return nwall_ > 0 || ((flags & MRLBM_STEP_FORCES) && nbcf_ > 0) || Conv<S>::scaled;
+    }
+
+    // Executable graph of graph_steps_ steps for these flags from the current buffer parity.
+    int graph_for(uint32_t flags, hipGraphExec_t &exec) {
+        const int key = ((flags & MRLBM_STEP_FORCES) ? 1 : 0) * 2 + cur_;
+        if (!graphs_[key]) {
+            const int cur0 = cur_;
+            const int64_t steps0 = steps;
+            HIP_TRY(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+            for (int s = 0; s < graph_steps_; ++s) queue_step(flags, nullptr);
+            hipGraph_t graph = nullptr;
+            HIP_TRY(hipStreamEndCapture(stream_, &graph));
+            cur_ = cur0;
+            steps = steps0;
+            hipGraphExec_t e = nullptr;
+            const hipError_t err = hipGraphInstantiate(&e, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            HIP_TRY(err);
+            graphs_[key] = e;
+        }
+        exec = graphs_[key];
+        return MRLBM_OK;
     }
 
     // Queue one time step: fused pass buf[cur] -> buf[cur^1], then the wall pass on buf[cur^1].
@@ -363,11 +434,13 @@ class Sim final : public SimBase {
         a.wall_in = d_win_;
         a.bcf_in = d_bcfin_;
         a.inlet = d_inlet_;
+        a.step_counter = stochastic_ ? d_step_counter_ : nullptr;
         a.scale = scale_;
         a.omega = omega_;
         a.rho0 = rho0_;
         if (ev) (void)hipEventRecord(ev[0], stream_);
-        launch_fused<R, S>(a, bx_, by_, stream_);
+        launch_fused<R, S>(a, bx_, by_, variant_, stream_);
+        launched_ = true;
         if (ev) (void)hipEventRecord(ev[1], stream_);
         const bool forces = (flags & MRLBM_STEP_FORCES) && g.ngroups > 0;
         if (has_wall_launch(flags)) {
@@ -401,7 +474,7 @@ class Sim final : public SimBase {
             w.partials = d_partials_;
             w.ticket = d_ticket_;
             w.ring = d_ring_;
-            w.slot = int(steps % FORCE_RING);
+            w.step_counter = d_step_counter_;
             if (ev) (void)hipEventRecord(ev[2], stream_);
             launch_wall<R, S>(w, stream_);
             if (ev) (void)hipEventRecord(ev[3], stream_);
@@ -446,9 +519,11 @@ class Sim final : public SimBase {
 
     void release() {
         if (stream_) (void)hipStreamSynchronize(stream_);
+        for (auto &g : graphs_)
+            if (g) (void)hipGraphExecDestroy(g);
         void *ptrs[] = {buf_[0], buf_[1], donor_, d_info_, d_items_, d_wconst_, d_wnx_, d_wny_, d_win_, d_wout_, d_wnode_,
                         d_wcoef_, d_stbase_, d_stfrac_, d_bcfnode_, d_bcftag_, d_bcfin_, d_bcfout_, d_inlet_,
-                        d_partials_, d_ticket_, d_ring_};
+                        d_partials_, d_ticket_, d_ring_, d_step_counter_};
         for (void *p : ptrs)
             if (p) (void)hipFree(p);
         if (stream_) (void)hipStreamDestroy(stream_);
@@ -462,6 +537,11 @@ class Sim final : public SimBase {
     int cur_ = 0;
     R omega_ = 0, rho0_ = 1;
     StoreScale<R> scale_{};
+    int variant_ = 1;
+    bool stochastic_ = Conv<S>::scaled;
+    int graph_steps_ = 16;
+    bool launched_ = false;
+    hipGraphExec_t graphs_[4] = {nullptr, nullptr, nullptr, nullptr};
     int bx_ = 32, by_ = 8, lz_ = 16, tiles_x_ = 1, tiles_y_ = 1, ntiles_ = 1, nblocks_ = 1, xcd_ = 1;
     int nwall_ = 0, nb_inner_ = 0, nbcf_ = 0;
     uint32_t *d_info_ = nullptr;
@@ -472,6 +552,7 @@ class Sim final : public SimBase {
     uint32_t *d_bcfnode_ = nullptr, *d_bcfin_ = nullptr, *d_bcfout_ = nullptr;
     uint8_t *d_bcftag_ = nullptr;
     double *d_partials_ = nullptr, *d_ring_ = nullptr;
+    unsigned long long *d_step_counter_ = nullptr;
     unsigned *d_ticket_ = nullptr;
 };
 
@@ -722,21 +803,20 @@ int mrlbm_checkpoint_write(mrlbm_sim *s, const char *path) {
     DEV_CHECK(s);
     if (!path) return set_error(MRLBM_ERR_INVALID, "null path");
     const mrlbm::Geometry &g = s->impl->g;
-    std::vector<double> payload(size_t(10) * g.n);
-    for (int f = 0; f < 10; ++f) {
-        int rc = s->impl->get(f, payload.data() + size_t(f) * g.n);
-        if (rc) return rc;
-    }
+    // payload: the moments exactly as stored (rho - rho0, u, m, or the scaled deviations of the
+    // 16-bit forms), so a restart continues bit for bit
+    std::vector<unsigned char> payload(size_t(10) * g.n * s->impl->bytes_per_value);
+    if (int rc = s->impl->get_raw(payload.data())) return rc;
     CheckpointHeader h{};
     std::memcpy(h.magic, "MRLBMCK1", 8);
-    h.version = 1;
+    h.version = 2;
     h.precision = uint32_t(s->impl->d.precision);
     h.nx = g.nx;
     h.ny = g.ny;
     h.nz = g.nz;
     h.fields = 10;
     h.steps = s->impl->steps;
-    h.payload_bytes = payload.size() * sizeof(double);
+    h.payload_bytes = payload.size();
     h.checksum = fnv1a(payload.data(), h.payload_bytes);
     FILE *fp = std::fopen(path, "wb");
     if (!fp) return set_error(MRLBM_ERR_IO, std::string("cannot open ") + path);
@@ -753,25 +833,22 @@ int mrlbm_checkpoint_read(mrlbm_sim *s, const char *path) {
     FILE *fp = std::fopen(path, "rb");
     if (!fp) return set_error(MRLBM_ERR_IO, std::string("cannot open ") + path);
     CheckpointHeader h{};
-    std::vector<double> payload(size_t(10) * g.n);
+    std::vector<unsigned char> payload(size_t(10) * g.n * s->impl->bytes_per_value);
     const bool head = std::fread(&h, sizeof(h), 1, fp) == 1;
-    bool ok = head && std::memcmp(h.magic, "MRLBMCK1", 8) == 0 && h.version == 1 && h.nx == g.nx &&
-              h.ny == g.ny && h.nz == g.nz && h.fields == 10 && h.payload_bytes == payload.size() * sizeof(double);
+    bool ok = head && std::memcmp(h.magic, "MRLBMCK1", 8) == 0 && h.version == 2 &&
+              h.precision == uint32_t(s->impl->d.precision) && h.nx == g.nx && h.ny == g.ny && h.nz == g.nz &&
+              h.fields == 10 && h.payload_bytes == payload.size();
     if (ok) ok = std::fread(payload.data(), 1, h.payload_bytes, fp) == h.payload_bytes;
     if (ok) {
         char extra;
         ok = std::fread(&extra, 1, 1, fp) == 0;  // no trailing bytes
     }
     std::fclose(fp);
-    if (!ok) return set_error(MRLBM_ERR_CORRUPT, "checkpoint header or size mismatch");
+    if (!ok) return set_error(MRLBM_ERR_CORRUPT, "checkpoint header, precision or size mismatch");
     if (fnv1a(payload.data(), h.payload_bytes) != h.checksum)
         return set_error(MRLBM_ERR_CORRUPT, "checkpoint checksum mismatch");
-    for (int f = 0; f < 10; ++f) {
-        int rc = s->impl->set(f, payload.data() + size_t(f) * g.n);
-        if (rc) return rc;
-    }
-    s->impl->steps = h.steps;
-    return MRLBM_OK;
+    if (int rc = s->impl->set_raw(payload.data())) return rc;
+    return s->impl->set_steps(h.steps);
 }
 
 int mrlbm_destroy(mrlbm_sim *s) {

@@ -31,6 +31,9 @@ struct FusedArgs {
     const uint32_t *__restrict__ wall_in;
     const uint32_t *__restrict__ bcf_in;  // 256 per-tag incoming masks
     const R *__restrict__ inlet;          // u_x per (y, z)
+    // steps completed on the device (advanced by wall_step); seeds stochastic rounding of the
+    // 16-bit forms, null for round-to-nearest
+    const unsigned long long *__restrict__ step_counter;
     StoreScale<R> scale;
     R omega, rho0;
 };
@@ -62,12 +65,16 @@ struct WallArgs {
     int forces;
     double *__restrict__ partials;  // 4 per block
     unsigned *__restrict__ ticket;
-    double *__restrict__ ring;      // 4 per step
-    int slot;
+    double *__restrict__ ring;      // 4 per step, FORCE_RING_STEPS records
+    unsigned long long *__restrict__ step_counter;  // steps completed on the device
 };
 
+// Steps of force history kept in the device ring.
+constexpr int FORCE_RING_STEPS = 1 << 16;
+
+// variant 0: register-window kernel (default); 1: LDS ring-buffer kernel
 template <class R, class S>
-void launch_fused(const FusedArgs<R, S> &a, int bx, int by, hipStream_t stream);
+void launch_fused(const FusedArgs<R, S> &a, int bx, int by, int variant, hipStream_t stream);
 
 template <class R, class S>
 void launch_wall(const WallArgs<R, S> &a, hipStream_t stream);
@@ -84,10 +91,11 @@ inline bool fused_tile_supported(int bx, int by) {
            (bx == 64 && by == 4) || (bx == 16 && by == 4);
 }
 
-// Bytes of dynamic LDS one fused block uses: three planes of the c_z=+1 group, two of the
-// c_z=0 group and one of the c_z=-1 group, 9 populations each, over the tile and its ring.
-inline size_t fused_lds_bytes(int bx, int by, size_t real_bytes) {
-    return size_t(54) * size_t(bx + 2) * size_t(by + 2) * real_bytes;
+// Bytes of dynamic LDS one fused block uses over the tile and its ring: one plane of 27 pulled
+// populations (variant 0), or three planes of the c_z=+1 group, two of the c_z=0 group and one of
+// the c_z=-1 group (variant 1).
+inline size_t fused_lds_bytes(int bx, int by, size_t real_bytes, int variant = 0) {
+    return size_t(variant == 1 ? 54 : 27) * size_t(bx + 2) * size_t(by + 2) * real_bytes;
 }
 
 }  // namespace mrlbm

@@ -93,12 +93,15 @@ class Simulation:
     def sync(self):
         _check(self.lib.mrlbm_sync(self.h))
 
-    def time_steps(self, n, forces=False):
-        """Device time of n steps (ms) and mean per-launch ms of the fused and wall kernels."""
+    def time_steps(self, n, forces=False, per_kernel=True):
+        """Device time of n steps (ms). With per_kernel, steps are queued one by one with HIP
+        events around each launch and the mean per-launch ms of the fused and wall kernels is
+        returned too; without, the steps run as captured graph replays (as mrlbm_step does)."""
         total = C.c_double()
         per = (C.c_double * 2)()
-        _check(self.lib.mrlbm_time_steps(self.h, n, _abi.STEP_FORCES if forces else 0, C.byref(total), per))
-        return total.value, (per[0], per[1])
+        _check(self.lib.mrlbm_time_steps(self.h, n, _abi.STEP_FORCES if forces else 0, C.byref(total),
+                                         per if per_kernel else None))
+        return total.value, ((per[0], per[1]) if per_kernel else None)
 
     # -- state
     def field(self, f):

@@ -1,0 +1,91 @@
+"""Reduced-precision storage (FP32 arithmetic, BF16/FP16 moments as scaled deviations) and the
+checkpoint round trip, on the GPU.
+
+Every non-FP64 form reports its velocity L2 error and drag error against the FP64 run of the
+same case; the bounds asserted here are the ones stated in each test.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from mrlbm_3d_cuda_amd import _abi, configs
+
+pytestmark = pytest.mark.gpu
+
+
+def channel(precision):
+    return configs.cylinder(nx=128, ny=64, nz=8, diameter=12.0, xc=32.0, yc=32.0, u_max=0.05, re=60.0,
+                            precision=precision)
+
+
+def run(desc, steps):
+    from mrlbm_3d_cuda_amd.solver import Simulation
+    s = Simulation(desc).init()
+    s.step(steps, forces=True)
+    F = s.fields()
+    forces = s.forces(0, steps)
+    s.close()
+    return F, forces
+
+
+@pytest.fixture(scope="module")
+def fp64_reference():
+    print("[gpu precision] FP64 reference run", flush=True)
+    return run(channel(_abi.FP64), 1500)
+
+
+@pytest.mark.parametrize("precision,u_l2_bound,cd_bound", [
+    (_abi.FP32, 1e-4, 1e-4),
+    (_abi.FP32_FP16, 3e-3, 1e-3),
+    (_abi.FP32_BF16, 2e-2, 5e-3),
+])
+def test_storage_precision_errors(fp64_reference, precision, u_l2_bound, cd_bound):
+    F64, f64 = fp64_reference
+    print(f"[gpu precision] {_abi.PRECISION_NAMES[precision]}", flush=True)
+    F, f = run(channel(precision), 1500)
+    u, u64 = F[1:4], F64[1:4]
+    u_l2 = float(np.sqrt(np.sum((u - u64) ** 2) / np.sum(u64 ** 2)))
+    cd = f[-500:, 0].mean()
+    cd64 = f64[-500:, 0].mean()
+    cd_err = abs(cd - cd64) / abs(cd64)
+    print(f"[gpu precision] {_abi.PRECISION_NAMES[precision]}: velocity L2 error {u_l2:.3e}, "
+          f"drag error {cd_err:.3e}", flush=True)
+    assert np.isfinite(F).all()
+    assert u_l2 < u_l2_bound
+    assert cd_err < cd_bound
+
+
+@pytest.mark.parametrize("precision", [_abi.FP32, _abi.FP32_BF16])
+def test_checkpoint_round_trip_bitwise(tmp_path, precision):
+    """Write after 40 steps, continue 30; a fresh simulation that reads the checkpoint and runs 30
+    steps ends bit for bit at the same state (for BF16 storage this includes the stochastic
+    rounding, whose random bits follow the restored step count); a truncated file is rejected."""
+    from mrlbm_3d_cuda_amd.solver import MrlbmError, Simulation
+    d = channel(precision)
+    a = Simulation(d).init()
+    a.step(40, forces=True)
+    path = str(tmp_path / "ck.bin")
+    a.checkpoint_write(path)
+    a.step(30, forces=True)
+    b = Simulation(d).init()
+    b.checkpoint_read(path)
+    assert b.info().steps_done == 40
+    b.step(30, forces=True)
+    assert np.array_equal(a.fields(), b.fields())
+    assert np.array_equal(a.forces(40, 30), b.forces(40, 30))
+    data = open(path, "rb").read()
+    bad = str(tmp_path / "bad.bin")
+    open(bad, "wb").write(data[:-8])
+    with pytest.raises(MrlbmError) as e:
+        b.checkpoint_read(bad)
+    assert e.value.status == _abi.ERR_CORRUPT
+    flipped = bytearray(data)
+    flipped[len(flipped) // 2] ^= 0x40
+    open(bad, "wb").write(bytes(flipped))
+    with pytest.raises(MrlbmError) as e:
+        b.checkpoint_read(bad)
+    assert e.value.status == _abi.ERR_CORRUPT
+    with pytest.raises(MrlbmError) as e:
+        b.checkpoint_read(str(tmp_path / "missing.bin"))
+    assert e.value.status == _abi.ERR_IO
