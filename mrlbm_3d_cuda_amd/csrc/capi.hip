@@ -340,9 +340,13 @@ class Sim final : public SimBase {
         if (count < 0 || first < 0 || first + count > steps || first < steps - FORCE_RING_STEPS)
             return set_error(MRLBM_ERR_INVALID, "force record range not available");
         HIP_TRY(hipStreamSynchronize(stream_));
-        for (int k = 0; k < count; ++k) {
+        // at most two contiguous pieces of the ring
+        for (int k = 0; k < count;) {
             const int slot = int((first + k) % FORCE_RING_STEPS);
-            HIP_TRY(hipMemcpy(out4 + 4 * k, d_ring_ + 4 * slot, 4 * sizeof(double), hipMemcpyDeviceToHost));
+            const int len = std::min(count - k, FORCE_RING_STEPS - slot);
+            HIP_TRY(hipMemcpy(out4 + 4 * size_t(k), d_ring_ + 4 * size_t(slot), size_t(len) * 4 * sizeof(double),
+                              hipMemcpyDeviceToHost));
+            k += len;
         }
         return MRLBM_OK;
     }
