@@ -16,6 +16,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -39,19 +41,30 @@ def read_traffic(path, workload_key):
 
 
 def copy_calibration(nbytes, reps=5):
-    """Device-to-device copies of a known size, for scaling the FETCH_SIZE / WRITE_SIZE counters."""
+    """Device-to-device copies of a known size: scales the FETCH_SIZE / WRITE_SIZE counters and
+    gives the copy bandwidth (read + write bytes per second) this GPU sustains."""
     import ctypes as C
     hip = C.CDLL("libamdhip64.so")
     a, b = C.c_void_p(), C.c_void_p()
     assert hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) == 0
     assert hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)) == 0
     assert hip.hipMemset(a, 1, C.c_size_t(nbytes)) == 0
+    e0, e1 = C.c_void_p(), C.c_void_p()
+    assert hip.hipEventCreate(C.byref(e0)) == 0 and hip.hipEventCreate(C.byref(e1)) == 0
+    assert hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3) == 0  # hipMemcpyDeviceToDevice, warm-up
+    assert hip.hipEventRecord(e0, None) == 0
     for _ in range(reps):
-        assert hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3) == 0  # hipMemcpyDeviceToDevice
+        assert hip.hipMemcpyAsync(b, a, C.c_size_t(nbytes), 3, None) == 0
+    assert hip.hipEventRecord(e1, None) == 0
+    assert hip.hipEventSynchronize(e1) == 0
+    ms = C.c_float()
+    assert hip.hipEventElapsedTime(C.byref(ms), e0, e1) == 0
     assert hip.hipDeviceSynchronize() == 0
     hip.hipFree(a)
     hip.hipFree(b)
-    print(json.dumps({"copy_calibration_bytes": nbytes, "reps": reps}), flush=True)
+    gbs = 2 * nbytes * reps / (ms.value * 1e-3) / 1e9
+    print(json.dumps({"copy_calibration_bytes": nbytes, "reps": reps + 1, "copy_gbs_read_plus_write": round(gbs, 1)}),
+          flush=True)
     return 0
 
 
@@ -75,6 +88,96 @@ def cpu_baseline(desc, steps=20, warmup=2):
     }
 
 
+def mean_cd(forces, desc):
+    """Mean drag coefficient over the second half of the records, Cd = 2 Fx / (rho0 U^2 D Lz)."""
+    half = forces[len(forces) // 2:, 0]
+    return float(half.mean() * 2.0 / (desc.rho0 * desc.u_max ** 2 * 2.0 * desc.r_inner * desc.nz))
+
+
+def bench_batch(args):
+    """Batched sweeps: one fused + one wall launch per step over every case, tuned launch
+    configuration, graph replays. cases/s counts whole cases at the sweep's steps per case
+    (C4 5000, C5 10000) from the measured step time."""
+    from mrlbm_3d_cuda_amd.solver import Batch, algorithmic_bytes_per_update
+
+    if args.workload == "c4":
+        descs = configs.c4_cases()
+        steps_per_case, name = 5000, "64 batched annuli (seeded sweep), FP32"
+    else:
+        descs = configs.c5_cases()
+        steps_per_case, name = 10000, "256 batched channels with cylinders (seeded sweep), FP32 compute / BF16 storage"
+    prec = descs[0].precision
+    print(f"[bench] batch workload={args.workload} cases={len(descs)} warmup={args.warmup} steps={args.steps}",
+          file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    b = Batch(descs)
+    setup_s = time.perf_counter() - t0
+    tune = None
+    if not args.no_tune:
+        tune = b.autotune(trial_steps=5)
+        print(f"[bench] tuned launch {tune}", file=sys.stderr, flush=True)
+    b.init()
+    b.step(args.warmup, forces=True)
+    b.sync()
+    ms = b.time_steps(args.steps, forces=True)
+    n = b.num_nodes
+    mlups = n * args.steps / 1e3 / ms
+    ms_step = ms / args.steps
+    cases_per_s = b.admitted / (ms_step * 1e-3 * steps_per_case)
+    launch = b.launch()
+    b.close()
+    out = {
+        "metric": METRIC,
+        "value": round(mlups, 1),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if prec == _abi.FP32 else "f32-compute/bf16-store",
+        "data": "synthetic (seeded case sweep and initial noise; no dataset)",
+        "config": {
+            "workload": name,
+            "cases": b.admitted,
+            "nodes_total": n,
+            "steps_per_case": steps_per_case,
+            "cases_per_s": round(cases_per_s, 3),
+            "launch": launch,
+            "tuner": tune,
+            "setup_s": round(setup_s, 2),
+            "hbm_pct_of_peak_whole_step": round(100 * mlups * 1e6 * algorithmic_bytes_per_update(prec)
+                                                / (HBM_PEAK_GBS * 1e9), 2),
+            "parallelism": "single GPU",
+        },
+    }
+    if args.workload == "c5" and not args.no_cpu_baseline:
+        out["config"]["cd_error_vs_fp64"] = c5_cd_errors(descs[:16], args.cd_steps)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def c5_cd_errors(descs, steps):
+    """Drag error of BF16 storage against FP64 on a subset of the C5 cases, both run as batches
+    for `steps` steps; mean Cd over the second half."""
+    from mrlbm_3d_cuda_amd.solver import Batch
+    cds = {}
+    for prec in (_abi.FP64, _abi.FP32_BF16):
+        ds = []
+        for d in descs:
+            d = d.copy()
+            d.precision = prec
+            ds.append(d)
+        with Batch(ds) as b:
+            b.init().step(steps, forces=True)
+            cds[prec] = [mean_cd(c.forces(0, steps), ds[i]) for i, c in enumerate(b.cases)]
+    err = [abs(a - r) / abs(r) for a, r in zip(cds[_abi.FP32_BF16], cds[_abi.FP64])]
+    return {"cases": len(descs), "steps": steps, "max_rel": float(max(err)), "mean_rel": float(np.mean(err)),
+            "cd_fp64_mean": float(np.mean(cds[_abi.FP64]))}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -83,9 +186,12 @@ def main():
     ap.add_argument("--precision", default="f32", choices=["f64", "f32", "bf16", "fp16"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=20)
-    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "box"],
+    ap.add_argument("--cd-steps", type=int, default=2000, help="c5: steps of the BF16-vs-FP64 drag check")
+    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "box", "c4", "c5"],
                     help="c2: confined cylinder 256x128x64 (default); c3: wake 512x256x256; "
-                         "box: periodic 512x256x256 (no boundaries)")
+                         "box: periodic 512x256x256 (no boundaries); c4: 64 batched annuli (FP32); "
+                         "c5: 256 batched channels with cylinders (BF16 storage)")
+    ap.add_argument("--no-tune", action="store_true", help="batched workloads: skip the launch tuner")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run device copies of this size (counter calibration)")
     args = ap.parse_args()
@@ -96,6 +202,9 @@ def main():
         print(f"bench.py measures one GPU only: --gpus {args.gpus} is not supported (this build runs the "
               "solver on a single MI355X; multi-GPU z-slab sharding is a design note only).", file=sys.stderr)
         return 2
+
+    if args.workload in ("c4", "c5"):
+        return bench_batch(args)
 
     from mrlbm_3d_cuda_amd.solver import Simulation, algorithmic_bytes_per_update
 

@@ -212,6 +212,46 @@ int mrlbm_checkpoint_read(mrlbm_sim *sim, const char *path);
 
 int mrlbm_destroy(mrlbm_sim *sim);
 
+/*
+ * Batches: many cases of one precision stepped together -- one fused launch over the work items
+ * of every case and one wall launch over the wall nodes of every case per step, replayed from
+ * captured HIP graphs. No counterpart in the reference, which compiles one case per run
+ * (src/solve_mrlbm.cu:3-24, src/main.cu:7-40); each case keeps the semantics of mrlbm_create.
+ *
+ * mrlbm_batch_create admits cases from the front of descs while their device bytes stay within
+ * hbm_budget_bytes (0: free device memory less 1 GiB) and reports the count in *admitted.
+ * mrlbm_batch_plan does the same admission on the host only (explicit budget).
+ * mrlbm_batch_case returns a handle owned by the batch: get/set moments, forces, tables, info and
+ * checkpoint_write work on it; init, step, time_steps, checkpoint_read and destroy are refused.
+ * mrlbm_batch_autotune times trial_steps steps of each launch configuration (tile, march
+ * variant, z chunk), keeps the fastest and re-initialises every case.
+ */
+typedef struct mrlbm_batch mrlbm_batch;
+
+typedef struct mrlbm_tune_result {
+    int32_t bx, by;         /* fused tile */
+    int32_t variant;        /* 1 ring march, 3 register-window march */
+    int32_t lz;             /* z chunk */
+    int32_t candidates;     /* configurations timed */
+    int32_t reserved;
+    double ms_per_step;         /* chosen configuration */
+    double ms_per_step_default; /* default configuration */
+} mrlbm_tune_result;
+
+int mrlbm_batch_plan(const mrlbm_case_desc *descs, int32_t n, uint64_t hbm_budget_bytes, int32_t *admitted,
+                     uint64_t *bytes);
+int mrlbm_batch_create(const mrlbm_case_desc *descs, int32_t n, uint64_t hbm_budget_bytes,
+                       mrlbm_batch **out, int32_t *admitted);
+int mrlbm_batch_init(mrlbm_batch *batch);
+int mrlbm_batch_step(mrlbm_batch *batch, int32_t nsteps, uint32_t flags);
+int mrlbm_batch_sync(mrlbm_batch *batch);
+int mrlbm_batch_num_cases(mrlbm_batch *batch, int32_t *n);
+int mrlbm_batch_case(mrlbm_batch *batch, int32_t index, mrlbm_sim **out);
+int mrlbm_batch_time_steps(mrlbm_batch *batch, int32_t nsteps, uint32_t flags, double *ms_total);
+int mrlbm_batch_autotune(mrlbm_batch *batch, int32_t trial_steps, mrlbm_tune_result *out);
+int mrlbm_batch_get_launch(mrlbm_batch *batch, mrlbm_tune_result *out);
+int mrlbm_batch_destroy(mrlbm_batch *batch);
+
 const char *mrlbm_last_error(void);
 int32_t mrlbm_abi_version(void);
 

@@ -115,6 +115,19 @@ class Info(C.Structure):
     ]
 
 
+class TuneResult(C.Structure):
+    _fields_ = [
+        ("bx", C.c_int32),
+        ("by", C.c_int32),
+        ("variant", C.c_int32),
+        ("lz", C.c_int32),
+        ("candidates", C.c_int32),
+        ("reserved", C.c_int32),
+        ("ms_per_step", C.c_double),
+        ("ms_per_step_default", C.c_double),
+    ]
+
+
 P = C.POINTER
 _dp = P(C.c_double)
 _u32p = P(C.c_uint32)
@@ -140,6 +153,20 @@ PRODUCT_PROTOTYPES = {
     "mrlbm_destroy": (C.c_int, [C.c_void_p]),
     "mrlbm_last_error": (C.c_char_p, []),
     "mrlbm_abi_version": (C.c_int32, []),
+}
+
+BATCH_PROTOTYPES = {
+    "mrlbm_batch_plan": (C.c_int, [P(CaseDesc), C.c_int32, C.c_uint64, P(C.c_int32), P(C.c_uint64)]),
+    "mrlbm_batch_create": (C.c_int, [P(CaseDesc), C.c_int32, C.c_uint64, P(C.c_void_p), P(C.c_int32)]),
+    "mrlbm_batch_init": (C.c_int, [C.c_void_p]),
+    "mrlbm_batch_step": (C.c_int, [C.c_void_p, C.c_int32, C.c_uint32]),
+    "mrlbm_batch_sync": (C.c_int, [C.c_void_p]),
+    "mrlbm_batch_num_cases": (C.c_int, [C.c_void_p, P(C.c_int32)]),
+    "mrlbm_batch_case": (C.c_int, [C.c_void_p, C.c_int32, P(C.c_void_p)]),
+    "mrlbm_batch_time_steps": (C.c_int, [C.c_void_p, C.c_int32, C.c_uint32, _dp]),
+    "mrlbm_batch_autotune": (C.c_int, [C.c_void_p, C.c_int32, P(TuneResult)]),
+    "mrlbm_batch_get_launch": (C.c_int, [C.c_void_p, P(TuneResult)]),
+    "mrlbm_batch_destroy": (C.c_int, [C.c_void_p]),
 }
 
 

@@ -20,6 +20,7 @@ SOURCES = {
     "kernels_f64.hip": ["-ffp-contract=off"],
     "kernels_f32.hip": [],
     "capi.hip": [],
+    "batch.hip": [],
 }
 
 HIPCC_FLAGS = [

@@ -98,11 +98,52 @@ def c3(precision=FP32_BF16):
                     noise=1e-4, inlet_noise=1e-3, seed=SEED_BASE | 0x03)
 
 
+def c4_cases(n=64, precision=FP32, seed=SEED_BASE | 0x04):
+    """C4: seeded annulus sweep. R_in ~ U[8, 20], R_out = R_in + U[10, 30], nz ~ U{16..64},
+    nx = ny = ceil(2 R_out + 8) rounded up to a multiple of 8, centre at the grid middle, inner
+    wall speed ~ U[0.02, 0.08] (outer at rest), tau ~ U[0.55, 1.2]."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        r_in = float(rng.uniform(8.0, 20.0))
+        r_out = r_in + float(rng.uniform(10.0, 30.0))
+        nz = int(rng.integers(16, 65))
+        nxy = int(np.ceil(2.0 * r_out + 8.0))
+        nxy = (nxy + 7) // 8 * 8
+        u = float(rng.uniform(0.02, 0.08))
+        tau = float(rng.uniform(0.55, 1.2))
+        c = 0.5 * (nxy - 1)
+        out.append(annulus(nx=nxy, ny=nxy, nz=nz, xc=c, yc=c, r_in=r_in, r_out=r_out, u_inner=u, tau=tau,
+                           precision=precision, seed=seed + 0x100 * (i + 1)))
+    return out
+
+
+def c5_cases(n=256, precision=FP32_BF16, seed=SEED_BASE | 0x05):
+    """C5: seeded cylinder sweep in confined channels as C2: nx ~ U{64..192}, ny ~ U{64..96},
+    nz ~ U{32..96} (multiples of 8), D ~ U[8, 24], Re ~ U[20, 300], u_max ~ U[0.03, 0.06]; the
+    cylinder sits at x = max(nx / 4, D + 8), y = ny / 2."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        nx = int(rng.integers(8, 25)) * 8
+        ny = int(rng.integers(8, 13)) * 8
+        nz = int(rng.integers(4, 13)) * 8
+        dia = float(rng.uniform(8.0, 24.0))
+        re = float(rng.uniform(20.0, 300.0))
+        u = float(rng.uniform(0.03, 0.06))
+        xc = float(max(nx // 4, int(dia) + 8))
+        out.append(cylinder(nx=nx, ny=ny, nz=nz, diameter=dia, xc=xc, yc=float(ny // 2), u_max=u, re=re,
+                            precision=precision, seed=seed + 0x100 * (i + 1)))
+    return out
+
+
 CONFIGS = {
     "c1_annulus_64x64x16_f64": c1,
     "c2_cylinder_256x128x64_f32": c2,
     "c3_wake_512x256x256": c3,
 }
 
-__all__ = ["periodic_box", "annulus", "cylinder", "c1", "c2", "c3", "tau_from_re", "CONFIGS",
-           "FP64", "FP32", "FP32_BF16", "FP32_FP16"]
+__all__ = ["periodic_box", "annulus", "cylinder", "c1", "c2", "c3", "c4_cases", "c5_cases", "tau_from_re",
+           "CONFIGS", "FP64", "FP32", "FP32_BF16", "FP32_FP16"]

@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 
 #include "geometry.h"
@@ -14,10 +15,10 @@ namespace mrlbm {
 // Arguments of the fused stream/collide pass over one ping-pong buffer pair.
 template <class R, class S>
 struct FusedArgs {
-    const S *__restrict__ src;  // 10 fields, field f at src + f * stride
-    S *__restrict__ dst;
-    S *__restrict__ donor;  // pre-collision snapshot of donor nodes (same layout)
-    size_t stride;
+    FieldSet<const S> src;  // moments at the start of the step
+    FieldSet<S> dst;        // moments after the step
+    FieldSet<S> donor;      // pre-collision snapshot of donor nodes
+    uint32_t pad;           // index of a padding slot after the last node (discarded stores)
     int nx, ny, nz;
     int tiles_x;     // tiles along x
     int nblocks;     // work items, one block each
@@ -41,9 +42,8 @@ struct FusedArgs {
 // Arguments of the curved-wall + force pass (one thread per wall node, then one per BCFLUID).
 template <class R, class S>
 struct WallArgs {
-    S *__restrict__ mom;          // buffer the fused pass just wrote
-    const S *__restrict__ donor;  // its pre-collision snapshot
-    size_t stride;
+    FieldSet<S> mom;          // buffer the fused pass just wrote
+    FieldSet<const S> donor;  // its pre-collision snapshot
     int nwall, nb_inner, nbcf;
     const uint32_t *__restrict__ node;
     const R *__restrict__ wnx;
@@ -63,6 +63,7 @@ struct WallArgs {
     StoreScale<R> scale;
     R omega, rho0;
     int forces;
+    int nblocks;                    // blocks of this case in the launch (ticket target)
     double *__restrict__ partials;  // 4 per block
     unsigned *__restrict__ ticket;
     double *__restrict__ ring;      // 4 per step, FORCE_RING_STEPS records
@@ -72,18 +73,52 @@ struct WallArgs {
 // Steps of force history kept in the device ring.
 constexpr int FORCE_RING_STEPS = 1 << 16;
 
-// variant 0: register-window kernel (default); 1: LDS ring-buffer kernel
+// Threads per block of the wall pass (one wall or BCFLUID node each); the host sizes the
+// per-block force partials with it.
+constexpr int WALL_THREADS = 64;
+
+inline int wall_blocks(int nwall, int nbcf) { return std::max(1, (nwall + nbcf + WALL_THREADS - 1) / WALL_THREADS); }
+
+// variant 1: ring march; 3, 4: register-window march with one / two LDS planes
 template <class R, class S>
 void launch_fused(const FusedArgs<R, S> &a, int bx, int by, int variant, hipStream_t stream);
 
 template <class R, class S>
 void launch_wall(const WallArgs<R, S> &a, hipStream_t stream);
 
+// Batched launches: one fused pass over the work items of many cases (item {case, tile |
+// special << 31, z0 | z1 << 16, 0}; tiles BX x BY in {32x8, 16x8, 32x4}), and one wall pass
+// whose block b runs block blocks[b].y of case blocks[b].x. Argument arrays live on the device.
 template <class R, class S>
-void launch_encode(const double *in, S *out, size_t stride, uint32_t n, const StoreScale<R> &sc, hipStream_t stream);
+void launch_fused_batch(const FusedArgs<R, S> *cases, const uint4 *items, int nitems, int groups, int bx, int by,
+                        int variant, hipStream_t stream);
 
 template <class R, class S>
-void launch_decode(const S *in, size_t stride, double *out, uint32_t n, const StoreScale<R> &sc, hipStream_t stream);
+void launch_wall_batch(const WallArgs<R, S> *cases, const int2 *blocks, int nblocks, hipStream_t stream);
+
+// Sets the dynamic-LDS limit of the batched fused kernel for this configuration (once, before any
+// launch or graph capture); 0 on success.
+template <class R, class S>
+int prepare_fused_batch(int bx, int by, int variant);
+
+inline bool batch_tile_supported(int bx, int by) {
+    return (bx == 32 && by == 8) || (bx == 16 && by == 8) || (bx == 32 && by == 4);
+}
+
+template <class R, class S>
+void launch_encode(const double *in, const FieldSet<S> &out, uint32_t n, const StoreScale<R> &sc, hipStream_t stream);
+
+template <class R, class S>
+void launch_decode(const FieldSet<const S> &in, double *out, uint32_t n, const StoreScale<R> &sc, hipStream_t stream);
+
+// Launch configuration of the fused pass (chosen by defaults, the environment or the tuner).
+struct LaunchConfig {
+    int bx = 32, by = 8;    // tile
+    int variant = 1;        // 1 ring march, 3 / 4 register-window march with 1 / 2 LDS planes
+    int lz = 32;            // z chunk of a work item
+    int lz_special = 0;     // z chunk of boundary tiles (0: lz)
+    bool xcd_remap = true;  // one z slab per XCD
+};
 
 // Tile shapes (BX x BY) the fused kernel is instantiated for.
 inline bool fused_tile_supported(int bx, int by) {
@@ -91,11 +126,13 @@ inline bool fused_tile_supported(int bx, int by) {
            (bx == 64 && by == 4) || (bx == 16 && by == 4);
 }
 
-// Bytes of dynamic LDS one fused block uses over the tile and its ring: one plane of 27 pulled
-// populations (variant 0), or three planes of the c_z=+1 group, two of the c_z=0 group and one of
-// the c_z=-1 group (variant 1).
-inline size_t fused_lds_bytes(int bx, int by, size_t real_bytes, int variant = 0) {
-    return size_t(variant == 1 ? 54 : 27) * size_t(bx + 2) * size_t(by + 2) * real_bytes;
+// Bytes of dynamic LDS one fused block uses over the tile and its ring: three planes of the
+// c_z=+1 population group, two of the c_z=0 group and one of the c_z=-1 group (ring march,
+// variant 1), or one / two planes of all 27 pulled populations (register-window march,
+// variants 3 / 4).
+inline size_t fused_lds_bytes(int bx, int by, size_t real_bytes, int variant) {
+    const size_t planes = variant == 3 ? 27 : 54;
+    return planes * size_t(bx + 2) * size_t(by + 2) * real_bytes;
 }
 
 }  // namespace mrlbm

@@ -19,6 +19,8 @@
 #pragma once
 
 #include <algorithm>
+#include <mutex>
+#include <vector>
 
 #include "kernels.h"
 
@@ -38,13 +40,14 @@ __device__ __forceinline__ int wrap(int v, int n) {
     return v >= n ? v - n : v;
 }
 
-// Post-stream moments of one node from its 27 pulled populations f, node boundary condition,
-// donor snapshot and collision, written to dst (the per-node part of streaming_and_evaluate_Mom
-// and collision_halo_update, solver/mlbm.cu:83-164, 186-242). `special` is uniform per block:
-// blocks without boundary nodes skip the node-type look-up.
+// Post-stream moments of one node from its 27 pulled populations f (in the LDS form of
+// reconstruct_lds), node boundary condition, donor snapshot and collision, written to dst (the
+// per-node part of streaming_and_evaluate_Mom and collision_halo_update,
+// solver/mlbm.cu:83-164, 186-242). `special` is uniform per block: blocks without boundary nodes
+// skip the node-type look-up.
 template <class R, class S>
-__device__ __forceinline__ void update_node(const FusedArgs<R, S> &a, const R *f, int x, int y, int z,
-                                            uint32_t plane, bool special) {
+__device__ __forceinline__ void update_node(const FusedArgs<R, S> &a, R *f, int x, int y, int z, uint32_t plane,
+                                            bool special) {
     const uint32_t idx = uint32_t(x) + uint32_t(a.nx) * uint32_t(y) + uint32_t(z) * plane;
     uint32_t seed = 0;
     if constexpr (Conv<S>::scaled) {
@@ -52,10 +55,11 @@ __device__ __forceinline__ void update_node(const FusedArgs<R, S> &a, const R *f
     }
     R M[10];
     if (!special) {
-        moments(f, M);
+        moments_lds(f, M);
         collide(a.omega, M);
-        store_node<R, S>(a.dst, a.stride, idx, a.scale, a.rho0, M, seed);
+        store_node<R, S>(a.dst, idx, a.scale, a.rho0, M, seed);
     } else {
+        lds_to_pop(f);
         const uint32_t info = a.info[idx];
         const uint32_t code = info & INFO_CODE_MASK;
         const uint32_t payload = info >> INFO_PAYLOAD_SHIFT;
@@ -63,18 +67,18 @@ __device__ __forceinline__ void update_node(const FusedArgs<R, S> &a, const R *f
         if (code == NC_BULK) {
             moments(f, M);
         } else if (code == NC_KEEP || code == NC_FROZEN) {
-            load_node<R, S>(a.src, a.stride, idx, a.scale, a.rho0, M);
+            load_node<R, S>(a.src, idx, a.scale, a.rho0, M);
             do_collide = code == NC_KEEP;
         } else if (code == NC_WALL) {
             // u keeps its old value; rho and m become the incoming rotated moments
-            load_node<R, S>(a.src, a.stride, idx, a.scale, a.rho0, M);
+            load_node<R, S>(a.src, idx, a.scale, a.rho0, M);
             incoming_rotated(a.wall_nx[payload], a.wall_ny[payload], a.wall_in[payload], f, M);
             do_collide = false;  // finished by wall_step
         } else if (code == NC_BCF) {
             bc_fluid(payload, a.bcf_in[payload], f, M);
         } else if (code == NC_WEST) {
             R old[10];
-            load_node<R, S>(a.src, a.stride, idx, a.scale, a.rho0, old);
+            load_node<R, S>(a.src, idx, a.scale, a.rho0, old);
             bc_west(f, old, a.inlet[uint32_t(y) + uint32_t(a.ny) * uint32_t(z)], M);
         } else if (code == NC_NORTH) {
             bc_ywall<face::K_N>(f, M);
@@ -86,8 +90,8 @@ __device__ __forceinline__ void update_node(const FusedArgs<R, S> &a, const R *f
             bc_west_edge<face::K_W & face::K_S, +1>(f, a.omega, M);
         } else {  // NC_EAST, NC_NORTH_EAST, NC_SOUTH_EAST
             R old[10], oxm[10];
-            load_node<R, S>(a.src, a.stride, idx, a.scale, a.rho0, old);
-            load_node<R, S>(a.src, a.stride, idx - 1, a.scale, a.rho0, oxm);
+            load_node<R, S>(a.src, idx, a.scale, a.rho0, old);
+            load_node<R, S>(a.src, idx - 1, a.scale, a.rho0, oxm);
             if (code == NC_EAST)
                 bc_east(f, old, oxm, M);
             else if (code == NC_NORTH_EAST)
@@ -95,39 +99,36 @@ __device__ __forceinline__ void update_node(const FusedArgs<R, S> &a, const R *f
             else
                 bc_east_edge<face::K_E & face::K_S, -1>(f, old, oxm, M);
         }
-        if (info & INFO_DONOR) store_node<R, S>(a.donor, a.stride, idx, a.scale, a.rho0, M, seed ^ 0x5A5A5A5Au);
+        if (info & INFO_DONOR) store_node<R, S>(a.donor, idx, a.scale, a.rho0, M, seed ^ 0x5A5A5A5Au);
         if (do_collide) collide(a.omega, M);
-        store_node<R, S>(a.dst, a.stride, idx, a.scale, a.rho0, M, seed);
+        store_node<R, S>(a.dst, idx, a.scale, a.rho0, M, seed);
     }
 }
 
-// Ring-buffer variant: populations of the last planes kept in LDS (3 planes of the c_z=+1
-// group, 2 of c_z=0, 1 of c_z=-1); one thread per tile node plus the ring spread over threads.
-template <class R, class S, int BX, int BY>
-__global__ __launch_bounds__(BX *BY) void fused_step_ring6_kernel(FusedArgs<R, S> a) {
+// z march of one work item. Populations of the last planes are kept in LDS grouped by c_z
+// (3 planes of the c_z=+1 group, 2 of c_z=0, 1 of c_z=-1); one thread per tile node, and the
+// (BX+2)(BY+2) - BX*BY ring positions spread over the first threads as a second position. The
+// stored values of the next plane are in flight in registers while a plane is updated.
+//
+// Vector-memory loads and stores share one counter, and a wait for a load while stores are
+// outstanding waits for everything. Bulk tiles (SPECIAL false) therefore keep a plane's results
+// in registers and store them at the start of the next iteration, after the wait for the
+// prefetched plane and before the next prefetch is issued, so a prefetch is never queued behind
+// stores of the same iteration. In bulk tiles the prefetch past the item wraps around in z and
+// threads outside the grid store into the padding slot after the last node, so the stores are
+// not predicated.
+template <class R, class S, int BX, int BY, bool SPECIAL>
+__device__ __forceinline__ void fused_march(const FusedArgs<R, S> &a, R *lds, int tile, int z0, int z1) {
     constexpr int NT = BX * BY;
     constexpr int W = BX + 2;
     constexpr int NH = W * (BY + 2);
-    extern __shared__ unsigned char lds_raw[];
-    R *lds = reinterpret_cast<R *>(lds_raw);
     // group planes: zp[3][9][NH], z0[2][9][NH], zm[9][NH]
     R *const gp = lds;
     R *const g0 = lds + 27 * NH;
     R *const gm = lds + 45 * NH;
 
-    // block -> work item; consecutive items share an XCD so that neighbouring tiles reuse each
-    // other's ring loads from the same L2
-    int b = blockIdx.x;
-    if (a.xcd_groups > 1) b = (b % a.xcd_groups) * (a.nblocks / a.xcd_groups) + b / a.xcd_groups;
-    const uint2 item = a.items[b];
-    const int tile = int(item.x & 0x7FFFFFFFu);
-    const bool special = (item.x >> 31) != 0;
     const int x0 = (tile % a.tiles_x) * BX;
     const int y0 = (tile / a.tiles_x) * BY;
-    const int z0 = int(item.y & 0xFFFFu);
-    const int z1 = int(item.y >> 16);
-    if (z0 >= z1) return;  // padding item (uniform per block)
-
     const int tid = threadIdx.x;
     const int tx = tid % BX, ty = tid / BX;
     const uint32_t plane = uint32_t(a.nx) * uint32_t(a.ny);
@@ -135,26 +136,28 @@ __global__ __launch_bounds__(BX *BY) void fused_step_ring6_kernel(FusedArgs<R, S
     // ring/tile positions loaded by this thread: p0 = tid, p1 = tid + NT (if inside the ring)
     const int p1 = tid + NT;
     const bool has1 = p1 < NH;
-    uint32_t off0, off1 = 0;
+    uint32_t off0, off1;
     {
         const int hx = tid % W, hy = tid / W;
         off0 = uint32_t(wrap(x0 - 1 + hx, a.nx)) + uint32_t(a.nx) * uint32_t(wrap(y0 - 1 + hy, a.ny));
+        off1 = off0;
         if (has1) {
             const int hx1 = p1 % W, hy1 = p1 / W;
             off1 = uint32_t(wrap(x0 - 1 + hx1, a.nx)) + uint32_t(a.nx) * uint32_t(wrap(y0 - 1 + hy1, a.ny));
         }
     }
 
-    R m0[10], m1[10];
+    S m0[10], m1[10];  // stored values of the plane in flight, per position
     auto load_plane = [&](int z) {
         const uint32_t base = uint32_t(wrap(z, a.nz)) * plane;
-        load_node<R, S>(a.src, a.stride, base + off0, a.scale, a.rho0, m0);
-        if (has1) load_node<R, S>(a.src, a.stride, base + off1, a.scale, a.rho0, m1);
+        load_raw(a.src, base + off0, m0);
+        if (has1) load_raw(a.src, base + off1, m1);
     };
-    // write the populations of the loaded plane into the requested group slots
-    auto put = [&](const R *m, int p, int sp, int s0, bool want0, bool wantm) {
-        R f[Q];
-        reconstruct(m, f);
+    // write the populations of a loaded plane into the requested group slots
+    auto put = [&](const S *raw, int p, int sp, int s0, bool want0, bool wantm) {
+        R m[10], f[Q];
+        decode_node<R, S>(raw, a.scale, a.rho0, m);
+        reconstruct_lds(m, f);
 #pragma unroll
         for (int i = 0; i < 9; ++i) gp[(sp * 9 + i) * NH + p] = f[GROUP_ZP[i]];
         if (want0) {
@@ -171,7 +174,7 @@ __global__ __launch_bounds__(BX *BY) void fused_step_ring6_kernel(FusedArgs<R, S
         if (has1) put(m1, p1, sp, s0, want0, wantm);
     };
 
-    // prologue: plane z0-1 (c_z=+1 group) and plane z0 (c_z=+1 and 0 groups)
+    // prologue: plane z0-1 (c_z=+1 group) and plane z0 (c_z=+1 and 0 groups), then z0+1 in flight
     load_plane(z0 - 1);
     put_plane(0, 0, false, false);
     load_plane(z0);
@@ -183,163 +186,283 @@ __global__ __launch_bounds__(BX *BY) void fused_step_ring6_kernel(FusedArgs<R, S
     const int h = (ty + 1) * W + (tx + 1);
     int sp_prev = 0, sp_cur = 1, sp_next = 2;  // slots of planes z-1, z, z+1 in the +1 group
     int s0_cur = 0;
+    uint32_t seed = 0;  // stochastic rounding of the 16-bit forms
+    if constexpr (Conv<S>::scaled) {
+        if (a.step_counter) seed = uint32_t(*a.step_counter) * 2u + 1u;
+    }
+    R Mp[10];           // bulk: results of the previous plane, stored one iteration late
+    uint32_t idxp = a.pad;
 
-    for (int z = z0; z < z1; ++z) {
-        put_plane(sp_next, s0_cur ^ 1, true, true);  // plane z+1
-        if (z + 1 < z1) load_plane(z + 2);            // prefetch, consumed next iteration
-        lds_barrier();
-        if (inside) {
-            R f[Q];
+    auto pull = [&](R *f) {
 #pragma unroll
-            for (int i = 0; i < 9; ++i) {
-                const int q = GROUP_ZP[i];
-                f[q] = gp[(sp_prev * 9 + i) * NH + h - LATTICE.cx[q] - LATTICE.cy[q] * W];
-            }
-#pragma unroll
-            for (int i = 0; i < 9; ++i) {
-                const int q = GROUP_Z0[i];
-                f[q] = g0[(s0_cur * 9 + i) * NH + h - LATTICE.cx[q] - LATTICE.cy[q] * W];
-            }
-#pragma unroll
-            for (int i = 0; i < 9; ++i) {
-                const int q = GROUP_ZM[i];
-                f[q] = gm[i * NH + h - LATTICE.cx[q] - LATTICE.cy[q] * W];
-            }
-            update_node(a, f, x, y, z, plane, special);
+        for (int i = 0; i < 9; ++i) {
+            const int q = GROUP_ZP[i];
+            f[q] = gp[(sp_prev * 9 + i) * NH + h - LATTICE.cx[q] - LATTICE.cy[q] * W];
         }
-        lds_barrier();
+#pragma unroll
+        for (int i = 0; i < 9; ++i) {
+            const int q = GROUP_Z0[i];
+            f[q] = g0[(s0_cur * 9 + i) * NH + h - LATTICE.cx[q] - LATTICE.cy[q] * W];
+        }
+#pragma unroll
+        for (int i = 0; i < 9; ++i) {
+            const int q = GROUP_ZM[i];
+            f[q] = gm[i * NH + h - LATTICE.cx[q] - LATTICE.cy[q] * W];
+        }
+    };
+    auto rotate = [&]() {
         const int t = sp_prev;
         sp_prev = sp_cur;
         sp_cur = sp_next;
         sp_next = t;
         s0_cur ^= 1;
-        if (z + 1 < z1) {
-            // registers now hold plane z+2; nothing to do until the next put
+    };
+
+    if constexpr (SPECIAL) {
+        for (int z = z0; z < z1; ++z) {
+            put_plane(sp_next, s0_cur ^ 1, true, true);  // plane z+1
+            if (z + 1 < z1) load_plane(z + 2);            // prefetch, consumed next iteration
+            lds_barrier();
+            if (inside) {
+                R f[Q];
+                pull(f);
+                update_node(a, f, x, y, z, plane, true);
+            }
+            lds_barrier();
+            rotate();
         }
+    } else {
+        auto iteration = [&](int z, bool store_prev) {
+            put_plane(sp_next, s0_cur ^ 1, true, true);  // plane z+1
+            if (store_prev) store_node<R, S>(a.dst, idxp, a.scale, a.rho0, Mp, seed);
+            load_plane(z + 2);  // prefetch, consumed next iteration (wraps past the item)
+            lds_barrier();
+            R f[Q];
+            pull(f);
+            moments_lds(f, Mp);
+            collide(a.omega, Mp);
+            idxp = inside ? uint32_t(x) + uint32_t(a.nx) * uint32_t(y) + uint32_t(z) * plane : a.pad;
+            lds_barrier();
+            rotate();
+        };
+        iteration(z0, false);
+        for (int z = z0 + 1; z < z1; ++z) iteration(z, true);
+        store_node<R, S>(a.dst, idxp, a.scale, a.rho0, Mp, seed);
     }
 }
 
-// Register-window variant (default): one thread per tile-or-ring position. Each thread keeps
-// the c_z=+1 populations of its column for planes z-1 and z, the c_z=0 ones for plane z and the
-// moments of plane z+1 in registers, so LDS holds one plane of 27 pulled populations per
-// position; ring positions sit in their own waves after the tile's.
-template <int BX, int BY>
-constexpr int fused_v2_threads() {
-    return ((BX + 2) * (BY + 2) + 63) / 64 * 64;
-}
-
-template <class R, class S, int BX, int BY>
-__global__ __launch_bounds__((fused_v2_threads<BX, BY>())) void fused_step_kernel(FusedArgs<R, S> a) {
+// Register-window march: LDS holds NBUF planes of the 27 populations pulled in one iteration;
+// every position keeps in registers the c_z=+1 populations of its last two planes and the c_z=0
+// populations of its last plane, and writes them into LDS in the iteration that pulls them.
+// NBUF = 1 needs two barriers per plane and a quarter of the LDS of the ring march (more blocks
+// per CU); NBUF = 2 alternates two LDS planes and needs one barrier per plane. The z loop is
+// unrolled by two so the two c_z=+1 windows swap roles without register moves. Stores of bulk
+// tiles are delayed by one iteration as in fused_march.
+template <class R, class S, int BX, int BY, int NBUF, bool SPECIAL>
+__device__ __forceinline__ void window_march(const FusedArgs<R, S> &a, R *lds, int tile, int z0, int z1) {
     constexpr int NT = BX * BY;
     constexpr int W = BX + 2;
     constexpr int NH = W * (BY + 2);
-    extern __shared__ unsigned char lds_raw[];
-    R *const G = reinterpret_cast<R *>(lds_raw);  // G[q * NH + p]: population q pulled from p
 
+    const int x0 = (tile % a.tiles_x) * BX;
+    const int y0 = (tile / a.tiles_x) * BY;
+    const int tid = threadIdx.x;
+    const int tx = tid % BX, ty = tid / BX;
+    const uint32_t plane = uint32_t(a.nx) * uint32_t(a.ny);
+
+    const int p1 = tid + NT;
+    const bool has1 = p1 < NH;
+    uint32_t off0, off1 = 0;
+    {
+        const int hx = tid % W, hy = tid / W;
+        off0 = uint32_t(wrap(x0 - 1 + hx, a.nx)) + uint32_t(a.nx) * uint32_t(wrap(y0 - 1 + hy, a.ny));
+        if (has1) {
+            const int hx1 = p1 % W, hy1 = p1 / W;
+            off1 = uint32_t(wrap(x0 - 1 + hx1, a.nx)) + uint32_t(a.nx) * uint32_t(wrap(y0 - 1 + hy1, a.ny));
+        }
+    }
+    S r0[10], r1[10];  // stored values of the plane in flight, per position
+    auto load_plane = [&](int z) {
+        const uint32_t base = uint32_t(wrap(z, a.nz)) * plane;
+        load_raw(a.src, base + off0, r0);
+        if (has1) load_raw(a.src, base + off1, r1);
+    };
+    // windows, per position: c_z=+1 groups of two planes (A, B), c_z=0 group of one plane
+    R pa0[9], pb0[9], q0[9], pa1[9], pb1[9], q1[9];
+    auto recon = [&](const S *raw, R *f) {
+        R m[10];
+        decode_node<R, S>(raw, a.scale, a.rho0, m);
+        reconstruct_lds(m, f);
+    };
+
+    // prologue: +1 group of plane z0-1 into A, +1 and 0 groups of plane z0 into B and q
+    {
+        R f[Q];
+        load_plane(z0 - 1);
+        recon(r0, f);
+#pragma unroll
+        for (int i = 0; i < 9; ++i) pa0[i] = f[GROUP_ZP[i]];
+        if (has1) {
+            recon(r1, f);
+#pragma unroll
+            for (int i = 0; i < 9; ++i) pa1[i] = f[GROUP_ZP[i]];
+        }
+        load_plane(z0);
+        recon(r0, f);
+#pragma unroll
+        for (int i = 0; i < 9; ++i) {
+            pb0[i] = f[GROUP_ZP[i]];
+            q0[i] = f[GROUP_Z0[i]];
+        }
+        if (has1) {
+            recon(r1, f);
+#pragma unroll
+            for (int i = 0; i < 9; ++i) {
+                pb1[i] = f[GROUP_ZP[i]];
+                q1[i] = f[GROUP_Z0[i]];
+            }
+        }
+        load_plane(z0 + 1);
+    }
+
+    const int x = x0 + tx, y = y0 + ty;
+    const bool inside = x < a.nx && y < a.ny;
+    const int h = (ty + 1) * W + (tx + 1);
+    uint32_t seed = 0;
+    if constexpr (Conv<S>::scaled) {
+        if (a.step_counter) seed = uint32_t(*a.step_counter) * 2u + 1u;
+    }
+    R Mp[10];
+    uint32_t idxp = a.pad;
+
+    // one position: reconstruct plane z+1, write the pulled plane's groups, advance the windows
+    auto put = [&](const S *raw, int p, R *G, R *pold, R *q) {
+        R f[Q];
+        recon(raw, f);
+#pragma unroll
+        for (int i = 0; i < 9; ++i) {
+            G[GROUP_ZP[i] * NH + p] = pold[i];  // plane z-1
+            G[GROUP_Z0[i] * NH + p] = q[i];     // plane z
+            G[GROUP_ZM[i] * NH + p] = f[GROUP_ZM[i]];  // plane z+1
+        }
+#pragma unroll
+        for (int i = 0; i < 9; ++i) {
+            pold[i] = f[GROUP_ZP[i]];  // becomes plane z+1's window
+            q[i] = f[GROUP_Z0[i]];
+        }
+    };
+    auto iteration = [&](int z, R *G, R *w0, R *w1, bool store_prev) {
+        if (NBUF == 1 && z > z0) lds_barrier();  // previous plane's pulls are done
+        put(r0, tid, G, w0, q0);
+        if (has1) put(r1, p1, G, w1, q1);
+        if constexpr (!SPECIAL) {
+            if (store_prev) store_node<R, S>(a.dst, idxp, a.scale, a.rho0, Mp, seed);
+            load_plane(z + 2);  // wraps past the item
+        } else {
+            if (z + 1 < z1) load_plane(z + 2);
+        }
+        lds_barrier();
+        R f[Q];
+#pragma unroll
+        for (int q = 0; q < Q; ++q) f[q] = G[q * NH + h - LATTICE.cx[q] - LATTICE.cy[q] * W];
+        if constexpr (SPECIAL) {
+            if (inside) update_node(a, f, x, y, z, plane, true);
+        } else {
+            moments_lds(f, Mp);
+            collide(a.omega, Mp);
+            idxp = inside ? uint32_t(x) + uint32_t(a.nx) * uint32_t(y) + uint32_t(z) * plane : a.pad;
+        }
+    };
+    R *const G0 = lds;
+    R *const G1 = NBUF == 2 ? lds + 27 * NH : lds;
+    int z = z0;
+    iteration(z, G0, pa0, pa1, false);
+    ++z;
+    while (z < z1) {
+        iteration(z, G1, pb0, pb1, true);
+        if (++z >= z1) break;
+        iteration(z, G0, pa0, pa1, true);
+        ++z;
+    }
+    if constexpr (!SPECIAL) store_node<R, S>(a.dst, idxp, a.scale, a.rho0, Mp, seed);
+    if (NBUF == 1) lds_barrier();  // the block's LDS is free again for the next item
+}
+
+// One LDS plane leaves room for three blocks per CU: registers are capped to match.
+template <class R, class S, int BX, int BY, int NBUF>
+__global__ __launch_bounds__(BX *BY, (NBUF == 1 ? 3 : 2)) void fused_window_kernel(FusedArgs<R, S> a) {
+    extern __shared__ unsigned char lds_raw[];
+    R *lds = reinterpret_cast<R *>(lds_raw);
     int b = blockIdx.x;
     if (a.xcd_groups > 1) b = (b % a.xcd_groups) * (a.nblocks / a.xcd_groups) + b / a.xcd_groups;
     const uint2 item = a.items[b];
     const int tile = int(item.x & 0x7FFFFFFFu);
     const bool special = (item.x >> 31) != 0;
-    const int x0 = (tile % a.tiles_x) * BX;
-    const int y0 = (tile / a.tiles_x) * BY;
     const int z0 = int(item.y & 0xFFFFu);
     const int z1 = int(item.y >> 16);
     if (z0 >= z1) return;  // padding item (uniform per block)
+    if (special)
+        window_march<R, S, BX, BY, NBUF, true>(a, lds, tile, z0, z1);
+    else
+        window_march<R, S, BX, BY, NBUF, false>(a, lds, tile, z0, z1);
+}
 
-    const int tid = threadIdx.x;
-    const uint32_t plane = uint32_t(a.nx) * uint32_t(a.ny);
-    // position of this thread in the (BX+2) x (BY+2) frame: tile nodes first, then the ring
-    int hx = 0, hy = 0;
-    const bool interior = tid < NT;
-    const bool has_pos = tid < NH;
-    if (interior) {
-        hx = tid % BX + 1;
-        hy = tid / BX + 1;
-    } else if (has_pos) {
-        const int r = tid - NT;
-        if (r < BX) {
-            hx = r + 1;
-            hy = 0;
-        } else if (r < 2 * BX) {
-            hx = r - BX + 1;
-            hy = BY + 1;
-        } else if (r < 2 * BX + BY) {
-            hx = 0;
-            hy = r - 2 * BX + 1;
-        } else if (r < 2 * BX + 2 * BY) {
-            hx = BX + 1;
-            hy = r - 2 * BX - BY + 1;
-        } else {
-            const int k = r - 2 * BX - 2 * BY;
-            hx = (k & 1) ? BX + 1 : 0;
-            hy = (k & 2) ? BY + 1 : 0;
-        }
-    }
-    const int p = hy * W + hx;
-    const uint32_t off =
-        uint32_t(wrap(x0 - 1 + hx, a.nx)) + uint32_t(a.nx) * uint32_t(wrap(y0 - 1 + hy, a.ny));
+template <class R, class S, int BX, int BY>
+__global__ __launch_bounds__(BX *BY) void fused_step_ring6_kernel(FusedArgs<R, S> a) {
+    extern __shared__ unsigned char lds_raw[];
+    R *lds = reinterpret_cast<R *>(lds_raw);
+    // block -> work item; consecutive items share an XCD so that neighbouring tiles reuse each
+    // other's ring loads from the same L2
+    int b = blockIdx.x;
+    if (a.xcd_groups > 1) b = (b % a.xcd_groups) * (a.nblocks / a.xcd_groups) + b / a.xcd_groups;
+    const uint2 item = a.items[b];
+    const int tile = int(item.x & 0x7FFFFFFFu);
+    const bool special = (item.x >> 31) != 0;
+    const int z0 = int(item.y & 0xFFFFu);
+    const int z1 = int(item.y >> 16);
+    if (z0 >= z1) return;  // padding item (uniform per block)
+    if (special)
+        fused_march<R, S, BX, BY, true>(a, lds, tile, z0, z1);
+    else
+        fused_march<R, S, BX, BY, false>(a, lds, tile, z0, z1);
+}
 
-    R mnext[10];
-    auto load_plane = [&](int z) {
-        load_node<R, S>(a.src, a.stride, uint32_t(wrap(z, a.nz)) * plane + off, a.scale, a.rho0, mnext);
-    };
-    R pprev[9], pcur[9], zcur[9];  // c_z=+1 of planes z-1, z; c_z=0 of plane z
-    if (has_pos) {
-        R f[Q];
-        load_plane(z0 - 1);
-        reconstruct(mnext, f);
-#pragma unroll
-        for (int i = 0; i < 9; ++i) pprev[i] = f[GROUP_ZP[i]];
-        load_plane(z0);
-        reconstruct(mnext, f);
-#pragma unroll
-        for (int i = 0; i < 9; ++i) {
-            pcur[i] = f[GROUP_ZP[i]];
-            zcur[i] = f[GROUP_Z0[i]];
-        }
-        load_plane(z0 + 1);
-    }
-
-    const int tx = hx - 1, ty = hy - 1;
-    const int x = x0 + tx, y = y0 + ty;
-    const bool update = interior && x < a.nx && y < a.ny;
-
-    for (int z = z0; z < z1; ++z) {
-        if (has_pos) {
-            R f[Q];
-            reconstruct(mnext, f);  // plane z+1
-#pragma unroll
-            for (int i = 0; i < 9; ++i) {
-                G[GROUP_ZP[i] * NH + p] = pprev[i];
-                G[GROUP_Z0[i] * NH + p] = zcur[i];
-                G[GROUP_ZM[i] * NH + p] = f[GROUP_ZM[i]];
-            }
-#pragma unroll
-            for (int i = 0; i < 9; ++i) {
-                pprev[i] = pcur[i];
-                pcur[i] = f[GROUP_ZP[i]];
-                zcur[i] = f[GROUP_Z0[i]];
-            }
-            if (z + 1 < z1) load_plane(z + 2);  // consumed next iteration
-        }
-        lds_barrier();
-        if (update) {
-            R f[Q];
-#pragma unroll
-            for (int q = 0; q < Q; ++q) f[q] = G[q * NH + p - LATTICE.cx[q] - LATTICE.cy[q] * W];
-            update_node(a, f, x, y, z, plane, special);
-        }
-        lds_barrier();
+// Fused pass of a batch: work item {case, tile | special << 31, z0 | z1 << 16, 0}; the case's
+// arguments come from a device array. VARIANT as in launch_fused.
+template <class R, class S, int BX, int BY, int VARIANT>
+__global__ __launch_bounds__(BX *BY, (VARIANT == 3 ? 3 : 2)) void fused_batch_kernel(
+    const FusedArgs<R, S> *__restrict__ cases, const uint4 *__restrict__ items, int nitems, int groups) {
+    extern __shared__ unsigned char lds_raw[];
+    R *lds = reinterpret_cast<R *>(lds_raw);
+    int b = blockIdx.x;
+    if (groups > 1) b = (b % groups) * (nitems / groups) + b / groups;
+    const uint4 item = items[b];
+    const int z0 = int(item.z & 0xFFFFu);
+    const int z1 = int(item.z >> 16);
+    if (z0 >= z1) return;  // padding item (uniform per block)
+    const FusedArgs<R, S> a = cases[item.x];
+    const int tile = int(item.y & 0x7FFFFFFFu);
+    const bool special = (item.y >> 31) != 0;
+    if constexpr (VARIANT == 1) {
+        if (special)
+            fused_march<R, S, BX, BY, true>(a, lds, tile, z0, z1);
+        else
+            fused_march<R, S, BX, BY, false>(a, lds, tile, z0, z1);
+    } else {
+        constexpr int NBUF = VARIANT == 4 ? 2 : 1;
+        if (special)
+            window_march<R, S, BX, BY, NBUF, true>(a, lds, tile, z0, z1);
+        else
+            window_march<R, S, BX, BY, NBUF, false>(a, lds, tile, z0, z1);
     }
 }
 
 template <class R, class S>
 __device__ __forceinline__ R bilinear(const S *__restrict__ field, uint32_t base, uint32_t pitch, R xd, R yd) {
-    const R q00 = R(Conv<S>::to(field[base]));
-    const R q10 = R(Conv<S>::to(field[base + 1]));
-    const R q01 = R(Conv<S>::to(field[base + pitch]));
-    const R q11 = R(Conv<S>::to(field[base + pitch + 1]));
+    const R q00 = R(Conv<S>::to(ld_field(field, base)));
+    const R q10 = R(Conv<S>::to(ld_field(field, base + 1)));
+    const R q01 = R(Conv<S>::to(ld_field(field, base + pitch)));
+    const R q11 = R(Conv<S>::to(ld_field(field, base + pitch + 1)));
     const R q0 = (R(1) - xd) * q00 + xd * q10;
     const R q1 = (R(1) - xd) * q01 + xd * q11;
     return (R(1) - yd) * q0 + yd * q1;
@@ -353,7 +476,7 @@ __device__ __forceinline__ void interp_rotate(const WallArgs<R, S> &a, uint32_t 
     if constexpr (!Conv<S>::scaled) {
 #pragma unroll
         for (int k = 0; k < 9; ++k)
-            v[k] = bilinear<R, S>(a.donor + size_t(k + 1) * a.stride, base, uint32_t(a.nx), xd, yd);
+            v[k] = bilinear<R, S>(a.donor.f[k + 1], base, uint32_t(a.nx), xd, yd);
     } else {
         // stored second moments are non-equilibrium parts: m = (m - u u) + u u is restored at each
         // corner before the corners are blended
@@ -366,7 +489,7 @@ __device__ __forceinline__ void interp_rotate(const WallArgs<R, S> &a, uint32_t 
             R q[9];
 #pragma unroll
             for (int k = 0; k < 9; ++k)
-                q[k] = R(Conv<S>::to(a.donor[size_t(k + 1) * a.stride + corner[j]])) * a.scale.dec[field_class(k + 1)];
+                q[k] = R(Conv<S>::to(ld_field(a.donor.f[k + 1], corner[j]))) * a.scale.dec[field_class(k + 1)];
             q[3] += q[0] * q[0];
             q[4] += q[1] * q[1];
             q[5] += q[2] * q[2];
@@ -390,16 +513,16 @@ __device__ __forceinline__ void interp_rotate(const WallArgs<R, S> &a, uint32_t 
     vm[8] = myz * c - mxz * s;
 }
 
-constexpr int WALL_THREADS = 256;
 
+// One block of the wall pass of one case: block `lb` of the case's a.nblocks blocks.
 template <class R, class S>
-__global__ __launch_bounds__(WALL_THREADS) void wall_step_kernel(WallArgs<R, S> a) {
-    const int i = blockIdx.x * WALL_THREADS + threadIdx.x;
+__device__ __forceinline__ void wall_body(const WallArgs<R, S> &a, int lb) {
+    const int i = lb * WALL_THREADS + threadIdx.x;
     double F[4] = {0.0, 0.0, 0.0, 0.0};
     if (i < a.nwall) {
         const uint32_t idx = a.node[i];
         R M[10];
-        load_node<R, S>(a.mom, a.stride, idx, a.scale, a.rho0, M);  // rho_I, u_old, m'_I
+        load_node<R, S>(as_const(a.mom), idx, a.scale, a.rho0, M);  // rho_I, u_old, m'_I
         const R c = a.wnx[i], s = a.wny[i];
         R vm1[9], vm2[9];
         interp_rotate(a, a.st_base[2 * i], a.st_frac[4 * i], a.st_frac[4 * i + 1], c, s, vm1);
@@ -427,30 +550,33 @@ __global__ __launch_bounds__(WALL_THREADS) void wall_step_kernel(WallArgs<R, S> 
         if (a.forces) masked_force(out, a.out_mask[i], -1.0, F);
         uint32_t seed = 0;
         if constexpr (Conv<S>::scaled) seed = uint32_t(*a.step_counter) * 2u + 2u;
-        store_node<R, S>(a.mom, a.stride, idx, a.scale, a.rho0, out, seed);
+        store_node<R, S>(a.mom, idx, a.scale, a.rho0, out, seed);
     } else if (a.forces && i < a.nwall + a.nbcf) {
         const int k = i - a.nwall;
         const uint32_t idx = a.bcf_node[k];
         const uint32_t t = a.bcf_tag[k];
         R M[10];
-        load_node<R, S>(a.donor, a.stride, idx, a.scale, a.rho0, M);
+        load_node<R, S>(a.donor, idx, a.scale, a.rho0, M);
         masked_force(M, a.bcf_in[t], 1.0, F);
         collide(a.omega, M);
         masked_force(M, a.bcf_out[t], -1.0, F);
     }
-    // block reduction in FP64: wave shuffles, then across the four waves through LDS. The last
-    // block to finish (atomic ticket) adds the block partials in block order, so the per-step
-    // record is deterministic, files it under the device step counter and advances the counter.
-    __shared__ double red[4][WALL_THREADS / 64];
+    // block reduction in FP64: wave shuffles, then across the waves through LDS. The last block
+    // to finish (atomic ticket) sums the block partials: each thread adds the partials of blocks
+    // t, t + WALL_THREADS, ... in order, then a fixed shuffle tree combines the threads, so the
+    // per-step record is deterministic. It is filed under the device step counter, which the
+    // last block then advances.
+    constexpr int NW = WALL_THREADS / 64;
+    __shared__ double red[4][NW];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    auto wave_sum = [](double v) {
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+        return v;
+    };
     if (a.forces) {
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            double v = F[k];
-#pragma unroll
-            for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-            F[k] = v;
-        }
-        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+        for (int k = 0; k < 4; ++k) F[k] = wave_sum(F[k]);
         if (lane == 0)
             for (int k = 0; k < 4; ++k) red[k][wave] = F[k];
     }
@@ -460,35 +586,61 @@ __global__ __launch_bounds__(WALL_THREADS) void wall_step_kernel(WallArgs<R, S> 
         if (a.forces) {
             for (int k = 0; k < 4; ++k) {
                 double tot = 0.0;
-                for (int w = 0; w < WALL_THREADS / 64; ++w) tot += red[k][w];
-                a.partials[4 * blockIdx.x + k] = tot;
+                for (int w = 0; w < NW; ++w) tot += red[k][w];
+                a.partials[4 * lb + k] = tot;
             }
         }
         __threadfence();
         const unsigned ticket = atomicAdd(a.ticket, 1u);
-        last = ticket == gridDim.x - 1;
+        last = ticket == unsigned(a.nblocks) - 1;
     }
     __syncthreads();
-    if (last && threadIdx.x == 0) {
-        __threadfence();
-        const unsigned long long step = __hip_atomic_load(a.step_counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (a.forces) {
-            double tot[4] = {0.0, 0.0, 0.0, 0.0};
-            for (unsigned blk = 0; blk < gridDim.x; ++blk)
-                for (int k = 0; k < 4; ++k)
-                    tot[k] += __hip_atomic_load(&a.partials[4 * blk + k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (!last) return;
+    __threadfence();
+    const unsigned long long step = __hip_atomic_load(a.step_counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (a.forces) {
+        double tot[4] = {0.0, 0.0, 0.0, 0.0};
+        for (unsigned blk = threadIdx.x; blk < unsigned(a.nblocks); blk += WALL_THREADS)
+            for (int k = 0; k < 4; ++k)
+                tot[k] += __hip_atomic_load(&a.partials[4 * blk + k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) tot[k] = wave_sum(tot[k]);
+        __syncthreads();
+        if (lane == 0)
+            for (int k = 0; k < 4; ++k) red[k][wave] = tot[k];
+        __syncthreads();
+        if (threadIdx.x == 0) {
             const size_t slot = size_t(step % FORCE_RING_STEPS);
-            for (int k = 0; k < 4; ++k) a.ring[4 * slot + k] = tot[k];
+            for (int k = 0; k < 4; ++k) {
+                double t = 0.0;
+                for (int w = 0; w < NW; ++w) t += red[k][w];
+                a.ring[4 * slot + k] = t;
+            }
         }
+    }
+    if (threadIdx.x == 0) {
         *a.step_counter = step + 1;
         *a.ticket = 0u;  // ready for the next launch
     }
 }
 
+template <class R, class S>
+__global__ __launch_bounds__(WALL_THREADS) void wall_step_kernel(WallArgs<R, S> a) {
+    wall_body(a, int(blockIdx.x));
+}
+
+// Wall pass of a batch: block b runs block blocks[b].y of case blocks[b].x.
+template <class R, class S>
+__global__ __launch_bounds__(WALL_THREADS) void wall_batch_kernel(const WallArgs<R, S> *__restrict__ cases,
+                                                                  const int2 *__restrict__ blocks) {
+    const int2 bc = blocks[blockIdx.x];
+    const WallArgs<R, S> a = cases[bc.x];
+    wall_body(a, bc.y);
+}
+
 // Field-major double arrays (field 0 = rho - rho0, the stored deviation) <-> storage.
 template <class R, class S>
-__global__ void encode_kernel(const double *__restrict__ in, S *__restrict__ out, size_t stride, uint32_t n,
-                              StoreScale<R> sc) {
+__global__ void encode_kernel(const double *__restrict__ in, FieldSet<S> out, uint32_t n, StoreScale<R> sc) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     double v[10];
@@ -496,29 +648,28 @@ __global__ void encode_kernel(const double *__restrict__ in, S *__restrict__ out
     for (int f = 0; f < 10; ++f) v[f] = in[size_t(f) * n + i];
     if constexpr (!Conv<S>::scaled) {
 #pragma unroll
-        for (int f = 0; f < 10; ++f) out[f * stride + i] = Conv<S>::from(v[f]);
+        for (int f = 0; f < 10; ++f) st_field(out.f[f], i, Conv<S>::from(v[f]));
     } else {
         R M[10];
         M[0] = R(1) + R(v[0]);
 #pragma unroll
         for (int f = 1; f < 10; ++f) M[f] = R(v[f]);
-        store_node<R, S>(out, stride, i, sc, R(1), M);
-        out[i] = Conv<S>::from(R(v[0]) * sc.enc[0]);
+        store_node<R, S>(out, i, sc, R(1), M);
+        st_field(out.f[0], i, Conv<S>::from(R(v[0]) * sc.enc[0]));
     }
 }
 
 template <class R, class S>
-__global__ void decode_kernel(const S *__restrict__ in, size_t stride, double *__restrict__ out, uint32_t n,
-                              StoreScale<R> sc) {
+__global__ void decode_kernel(FieldSet<const S> in, double *__restrict__ out, uint32_t n, StoreScale<R> sc) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     if constexpr (!Conv<S>::scaled) {
 #pragma unroll
-        for (int f = 0; f < 10; ++f) out[size_t(f) * n + i] = double(Conv<S>::to(in[f * stride + i]));
+        for (int f = 0; f < 10; ++f) out[size_t(f) * n + i] = double(Conv<S>::to(ld_field(in.f[f], i)));
     } else {
         R M[10];
-        load_node<R, S>(in, stride, i, sc, R(0), M);
-        out[i] = double(R(Conv<S>::to(in[i])) * sc.dec[0]);
+        load_node<R, S>(in, i, sc, R(0), M);
+        out[i] = double(R(Conv<S>::to(ld_field(in.f[0], i))) * sc.dec[0]);
 #pragma unroll
         for (int f = 1; f < 10; ++f) out[size_t(f) * n + i] = double(M[f]);
     }
@@ -529,21 +680,28 @@ __global__ void decode_kernel(const S *__restrict__ in, size_t stride, double *_
 template <class R, class S>
 void launch_fused(const FusedArgs<R, S> &a, int bx, int by, int variant, hipStream_t stream) {
     const size_t lds = fused_lds_bytes(bx, by, sizeof(R), variant);
-    auto go = [&](auto kernel, int nt) {
-        static bool attr_set = false;
-        if (!attr_set) {
-            (void)hipFuncSetAttribute(reinterpret_cast<const void *>(kernel),
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
-            attr_set = true;
+    auto go = [&](void (*kernel)(FusedArgs<R, S>), int nt) {
+        // kernels needing more than the default dynamic LDS limit get the attribute once each
+        static std::mutex mu;
+        static std::vector<const void *> done;
+        if (lds > 65536) {
+            std::lock_guard<std::mutex> lock(mu);
+            const void *k = reinterpret_cast<const void *>(kernel);
+            if (std::find(done.begin(), done.end(), k) == done.end()) {
+                (void)hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
+                done.push_back(k);
+            }
         }
         hipLaunchKernelGGL(kernel, dim3(a.nblocks), dim3(nt), lds, stream, a);
     };
 #define MRLBM_FUSED_TILE(TX, TY)                                                     \
     if (bx == TX && by == TY) {                                                      \
-        if (variant == 1)                                                            \
-            go(fused_step_ring6_kernel<R, S, TX, TY>, TX * TY);                      \
+        if (variant == 3)                                                            \
+            go(fused_window_kernel<R, S, TX, TY, 1>, TX * TY);                       \
+        else if (variant == 4)                                                       \
+            go(fused_window_kernel<R, S, TX, TY, 2>, TX * TY);                       \
         else                                                                         \
-            go(fused_step_kernel<R, S, TX, TY>, fused_v2_threads<TX, TY>());         \
+            go(fused_step_ring6_kernel<R, S, TX, TY>, TX * TY);                      \
         return;                                                                      \
     }
     MRLBM_FUSED_TILE(32, 8)
@@ -555,21 +713,61 @@ void launch_fused(const FusedArgs<R, S> &a, int bx, int by, int variant, hipStre
 #undef MRLBM_FUSED_TILE
 }
 
+// Kernel of the batched fused pass for a tile and variant (null if not instantiated).
+template <class R, class S>
+auto fused_batch_kernel_for(int bx, int by, int variant) -> void (*)(const FusedArgs<R, S> *, const uint4 *, int, int) {
+#define MRLBM_BATCH_TILE(TX, TY)                                                     \
+    if (bx == TX && by == TY) {                                                      \
+        if (variant == 3) return fused_batch_kernel<R, S, TX, TY, 3>;                \
+        if (variant == 4) return fused_batch_kernel<R, S, TX, TY, 4>;                \
+        return fused_batch_kernel<R, S, TX, TY, 1>;                                  \
+    }
+    MRLBM_BATCH_TILE(32, 8)
+    MRLBM_BATCH_TILE(16, 8)
+    MRLBM_BATCH_TILE(32, 4)
+#undef MRLBM_BATCH_TILE
+    return nullptr;
+}
+
+template <class R, class S>
+int prepare_fused_batch(int bx, int by, int variant) {
+    auto kernel = fused_batch_kernel_for<R, S>(bx, by, variant);
+    if (!kernel) return -1;
+    const size_t lds = fused_lds_bytes(bx, by, sizeof(R), variant);
+    if (lds <= 65536) return 0;
+    return hipFuncSetAttribute(reinterpret_cast<const void *>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
+                               int(lds)) == hipSuccess
+               ? 0
+               : -1;
+}
+
+template <class R, class S>
+void launch_fused_batch(const FusedArgs<R, S> *cases, const uint4 *items, int nitems, int groups, int bx, int by,
+                        int variant, hipStream_t stream) {
+    const size_t lds = fused_lds_bytes(bx, by, sizeof(R), variant);
+    auto kernel = fused_batch_kernel_for<R, S>(bx, by, variant);
+    if (kernel) hipLaunchKernelGGL(kernel, dim3(nitems), dim3(bx * by), lds, stream, cases, items, nitems, groups);
+}
+
+template <class R, class S>
+void launch_wall_batch(const WallArgs<R, S> *cases, const int2 *blocks, int nblocks, hipStream_t stream) {
+    hipLaunchKernelGGL((wall_batch_kernel<R, S>), dim3(nblocks), dim3(WALL_THREADS), 0, stream, cases, blocks);
+}
+
 template <class R, class S>
 void launch_wall(const WallArgs<R, S> &a, hipStream_t stream) {
-    const int n = a.nwall + (a.forces ? a.nbcf : 0);
-    const int blocks = std::max(1, (n + WALL_THREADS - 1) / WALL_THREADS);
+    const int blocks = wall_blocks(a.nwall, a.forces ? a.nbcf : 0);
     hipLaunchKernelGGL((wall_step_kernel<R, S>), dim3(blocks), dim3(WALL_THREADS), 0, stream, a);
 }
 
 template <class R, class S>
-void launch_encode(const double *in, S *out, size_t stride, uint32_t n, const StoreScale<R> &sc, hipStream_t stream) {
-    hipLaunchKernelGGL((encode_kernel<R, S>), dim3((n + 255) / 256), dim3(256), 0, stream, in, out, stride, n, sc);
+void launch_encode(const double *in, const FieldSet<S> &out, uint32_t n, const StoreScale<R> &sc, hipStream_t stream) {
+    hipLaunchKernelGGL((encode_kernel<R, S>), dim3((n + 255) / 256), dim3(256), 0, stream, in, out, n, sc);
 }
 
 template <class R, class S>
-void launch_decode(const S *in, size_t stride, double *out, uint32_t n, const StoreScale<R> &sc, hipStream_t stream) {
-    hipLaunchKernelGGL((decode_kernel<R, S>), dim3((n + 255) / 256), dim3(256), 0, stream, in, stride, out, n, sc);
+void launch_decode(const FieldSet<const S> &in, double *out, uint32_t n, const StoreScale<R> &sc, hipStream_t stream) {
+    hipLaunchKernelGGL((decode_kernel<R, S>), dim3((n + 255) / 256), dim3(256), 0, stream, in, out, n, sc);
 }
 
 }  // namespace mrlbm
@@ -577,5 +775,11 @@ void launch_decode(const S *in, size_t stride, double *out, uint32_t n, const St
 #define MRLBM_INSTANTIATE(R, S)                                                                             \
     template void launch_fused<R, S>(const FusedArgs<R, S> &, int, int, int, hipStream_t);                  \
     template void launch_wall<R, S>(const WallArgs<R, S> &, hipStream_t);                                   \
-    template void launch_encode<R, S>(const double *, S *, size_t, uint32_t, const StoreScale<R> &, hipStream_t); \
-    template void launch_decode<R, S>(const S *, size_t, double *, uint32_t, const StoreScale<R> &, hipStream_t);
+    template void launch_fused_batch<R, S>(const FusedArgs<R, S> *, const uint4 *, int, int, int, int, int,     \
+                                           hipStream_t);                                                    \
+    template int prepare_fused_batch<R, S>(int, int, int);                                                  \
+    template void launch_wall_batch<R, S>(const WallArgs<R, S> *, const int2 *, int, hipStream_t);          \
+    template void launch_encode<R, S>(const double *, const FieldSet<S> &, uint32_t, const StoreScale<R> &,        \
+                                      hipStream_t);                                                         \
+    template void launch_decode<R, S>(const FieldSet<const S> &, double *, uint32_t, const StoreScale<R> &,  \
+                                      hipStream_t);

@@ -32,6 +32,10 @@ __device__ __forceinline__ R population(const R *M, int q) {
     return R(weight(q)) * M[0] * (R(1) + R(3) * cu + R(4.5) * hd + R(9) * ho);
 }
 
+// 1/x: the hardware reciprocal (about 1 ulp) in FP32, the IEEE quotient in FP64.
+__device__ __forceinline__ float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ double fast_rcp(double x) { return 1.0 / x; }
+
 // ---------------------------------------------------------------- mask helpers
 
 // Bit set of directions satisfying a predicate on (cx, cy, cz).
@@ -166,7 +170,7 @@ __device__ __forceinline__ void moments(const R *f, R *M) {
     const R pxy = (s[3] - s[6]) + (s[9] + s[10]) - (s[11] + s[12]);
     const R pxz = (s[4] - s[7]) + (s[9] + s[11]) - (s[10] + s[12]);
     const R pyz = (s[5] - s[8]) + (s[9] + s[12]) - (s[10] + s[11]);
-    const R inv = R(1) / rho;
+    const R inv = fast_rcp(rho);
     const R cs2 = R(1) / R(3);
     M[0] = rho;
     M[1] = jx * inv;
@@ -178,6 +182,99 @@ __device__ __forceinline__ void moments(const R *f, R *M) {
     M[7] = pxy * inv;
     M[8] = pxz * inv;
     M[9] = pyz * inv;
+}
+
+// ---------------------------------------------------------------- populations as held in LDS
+// The fused kernel streams populations through LDS. In FP64 it keeps the weighted populations
+// f_q, evaluated exactly as reconstruct() does. In FP32 it keeps g_q = f_q / w_q: the weight
+// products are dropped from the reconstruction and applied once per lattice class (rest, axis,
+// face diagonal, corner) when the moments are summed.
+
+template <class R>
+__device__ __forceinline__ void reconstruct_lds(const R *M, R *g) {
+    if constexpr (kExactOrder<R>) {
+        reconstruct(M, g);
+    } else {
+        const R rho = M[0];
+        const R P = rho * (R(1) - R(1.5) * (M[4] + M[5] + M[6]));
+        const R r3 = R(3) * rho, r45 = R(4.5) * rho, r9 = R(9) * rho;
+        const R Ux = r3 * M[1], Uy = r3 * M[2], Uz = r3 * M[3];
+        const R Dx = r45 * M[4], Dy = r45 * M[5], Dz = r45 * M[6];
+        const R Oxy = r9 * M[7], Oxz = r9 * M[8], Oyz = r9 * M[9];
+        g[0] = P;
+        auto pair = [&](int q, R S, R V) {
+            g[q] = S + V;
+            g[q + 1] = S - V;
+        };
+        const R Px = P + Dx;
+        pair(1, Px, Ux);
+        pair(3, P + Dy, Uy);
+        pair(5, P + Dz, Uz);
+        const R Pxy = Px + Dy, Pxz = Px + Dz, Pyz = (P + Dy) + Dz, Pxyz = Pxy + Dz;
+        const R Uxpy = Ux + Uy, Uxmy = Ux - Uy;
+        pair(7, Pxy + Oxy, Uxpy);
+        pair(9, Pxz + Oxz, Ux + Uz);
+        pair(11, Pyz + Oyz, Uy + Uz);
+        pair(13, Pxy - Oxy, Uxmy);
+        pair(15, Pxz - Oxz, Ux - Uz);
+        pair(17, Pyz - Oyz, Uy - Uz);
+        const R a = Pxyz + Oxy, b = Pxyz - Oxy, o1 = Oxz + Oyz, o2 = Oxz - Oyz;
+        pair(19, a + o1, Uxpy + Uz);
+        pair(21, a - o1, Uxpy - Uz);
+        pair(23, b + o2, Uxmy + Uz);
+        pair(25, b - o2, (Uy + Uz) - Ux);
+    }
+}
+
+// Moments of the populations held in LDS (eval_moments.cuh:5-22).
+template <class R>
+__device__ __forceinline__ void moments_lds(const R *g, R *M) {
+    if constexpr (kExactOrder<R>) {
+        moments(g, M);
+    } else {
+        R s[13], d[13];
+#pragma unroll
+        for (int k = 0; k < 13; ++k) {
+            s[k] = g[2 * k + 1] + g[2 * k + 2];
+            d[k] = g[2 * k + 1] - g[2 * k + 2];
+        }
+        // pair k: 0 x, 1 y, 2 z, 3 (1,1,0), 4 (1,0,1), 5 (0,1,1), 6 (1,-1,0), 7 (1,0,-1),
+        // 8 (0,1,-1), 9 (1,1,1), 10 (1,1,-1), 11 (1,-1,1), 12 (-1,1,1)
+        const R w0 = R(W_REST), w1 = R(W_AXIS), w2 = R(W_FACE), w3 = R(W_CORNER);
+        const R C = (s[9] + s[10]) + (s[11] + s[12]);
+        const R rho = w0 * g[0] + w1 * ((s[0] + s[1]) + s[2]) +
+                      w2 * (((s[3] + s[4]) + (s[5] + s[6])) + (s[7] + s[8])) + w3 * C;
+        const R jx = w1 * d[0] + w2 * ((d[3] + d[4]) + (d[6] + d[7])) + w3 * ((d[9] + d[10]) + (d[11] - d[12]));
+        const R jy = w1 * d[1] + w2 * ((d[3] + d[5]) + (d[8] - d[6])) + w3 * ((d[9] + d[10]) + (d[12] - d[11]));
+        const R jz = w1 * d[2] + w2 * ((d[4] + d[5]) - (d[7] + d[8])) + w3 * ((d[9] + d[11]) + (d[12] - d[10]));
+        const R pxx = w1 * s[0] + w2 * ((s[3] + s[4]) + (s[6] + s[7])) + w3 * C;
+        const R pyy = w1 * s[1] + w2 * ((s[3] + s[5]) + (s[6] + s[8])) + w3 * C;
+        const R pzz = w1 * s[2] + w2 * ((s[4] + s[5]) + (s[7] + s[8])) + w3 * C;
+        const R pxy = w2 * (s[3] - s[6]) + w3 * ((s[9] + s[10]) - (s[11] + s[12]));
+        const R pxz = w2 * (s[4] - s[7]) + w3 * ((s[9] + s[11]) - (s[10] + s[12]));
+        const R pyz = w2 * (s[5] - s[8]) + w3 * ((s[9] + s[12]) - (s[10] + s[11]));
+        const R inv = fast_rcp(rho);
+        const R cs2 = R(1) / R(3);
+        M[0] = rho;
+        M[1] = jx * inv;
+        M[2] = jy * inv;
+        M[3] = jz * inv;
+        M[4] = pxx * inv - cs2;
+        M[5] = pyy * inv - cs2;
+        M[6] = pzz * inv - cs2;
+        M[7] = pxy * inv;
+        M[8] = pxz * inv;
+        M[9] = pyz * inv;
+    }
+}
+
+// LDS form -> weighted populations f_q, in place (no-op in FP64).
+template <class R>
+__device__ __forceinline__ void lds_to_pop(R *g) {
+    if constexpr (!kExactOrder<R>) {
+#pragma unroll
+        for (int q = 0; q < Q; ++q) g[q] *= R(weight(q));
+    }
 }
 
 // BGK relaxation of the second moments towards u u (moment_collision, solver/mlbm.cu:3-16).

@@ -42,13 +42,15 @@ struct Conv<float> {
 // 16-bit forms round stochastically when given random bits: a uniform integer is added below
 // the kept mantissa bits before truncation, so the stored value is unbiased. Round-to-nearest
 // would drop every per-step increment smaller than half an ulp and stall slowly evolving flow.
+// `r` holds 8 random bits; the 8 mantissa bits below them are added as zero, which leaves a
+// bias below 2^-9 ulp.
 template <>
 struct Conv<__hip_bfloat16> {
     static constexpr bool scaled = true;
     __device__ static float to(__hip_bfloat16 v) { return __bfloat162float(v); }
     __device__ static __hip_bfloat16 from(float v) { return __float2bfloat16(v); }
-    __device__ static __hip_bfloat16 from_sr(float v, uint32_t r) {
-        const uint32_t b = __float_as_uint(v) + (r & 0xFFFFu);
+    __device__ static __hip_bfloat16 from_sr(float v, uint32_t r8) {
+        const uint32_t b = __float_as_uint(v) + (r8 << 8);
         return __ushort_as_bfloat16(static_cast<unsigned short>(b >> 16));
     }
 };
@@ -58,38 +60,79 @@ struct Conv<_Float16> {
     static constexpr bool scaled = true;
     __device__ static float to(_Float16 v) { return float(v); }
     __device__ static _Float16 from(float v) { return _Float16(v); }
-    __device__ static _Float16 from_sr(float v, uint32_t r) {
+    __device__ static _Float16 from_sr(float v, uint32_t r8) {
         // 13 float mantissa bits fall below the half mantissa (normal range)
-        const float t = __uint_as_float(__float_as_uint(v) + (r & 0x1FFFu));
+        const float t = __uint_as_float(__float_as_uint(v) + (r8 << 5));
         return _Float16(__float2half_rz(t));
     }
 };
 
-// Random bits for stochastic rounding: a 32-bit integer hash of (node, field pair, step seed).
-__device__ __forceinline__ uint32_t sr_hash(uint32_t idx, uint32_t k, uint32_t seed) {
-    uint32_t h = idx * 0x9E3779B1u ^ (seed + k * 0x85EBCA77u);
-    h ^= h >> 16;
-    h *= 0x7FEB352Du;
+// Random bits for stochastic rounding: one 32-bit hash of (node, step seed) per node; field f
+// takes the 8 bits starting at bit (8 f) mod 25 (overlapping windows inside the word: the
+// rounding of each field is unbiased, correlations between the fields of one node do not matter).
+__device__ __forceinline__ uint32_t sr_hash(uint32_t idx, uint32_t seed) {
+    uint32_t h = (idx ^ 0x61C88647u) * 0x9E3779B1u + seed;
     h ^= h >> 15;
-    h *= 0x846CA68Bu;
-    h ^= h >> 16;
+    h *= 0x2C1B3C6Du;
+    h ^= h >> 12;
     return h;
 }
+
+__device__ __forceinline__ uint32_t sr_bits(uint32_t h, int f) { return (h >> ((8 * f) % 25)) & 0xFFu; }
 
 // Field scale class: 0 for rho, 1 for u, 2 for the second moments.
 __device__ __host__ constexpr int field_class(int f) { return f == 0 ? 0 : (f < 4 ? 1 : 2); }
 
-// Decode the moments of node `idx` (M[0] absolute density).
+// The 10 field arrays of one moment buffer, one base pointer each. Kernels receive the pointers
+// as arguments, so every load and store is a uniform 64-bit base plus a 32-bit per-lane byte
+// offset (the library limits one field to 4 GiB) and needs no per-lane 64-bit address math.
+template <class S>
+struct FieldSet {
+    S *f[10];
+};
+
+template <class S>
+__host__ __device__ inline FieldSet<S> make_fields(S *base, size_t stride) {
+    FieldSet<S> fs;
+    for (int k = 0; k < 10; ++k) fs.f[k] = base + size_t(k) * stride;
+    return fs;
+}
+
+template <class S>
+__host__ __device__ inline FieldSet<const S> as_const(const FieldSet<S> &fs) {
+    FieldSet<const S> c;
+    for (int k = 0; k < 10; ++k) c.f[k] = fs.f[k];
+    return c;
+}
+
+template <class S>
+__device__ __forceinline__ S ld_field(const S *__restrict__ field, uint32_t idx) {
+    return *reinterpret_cast<const S *>(reinterpret_cast<const char *>(field) + idx * uint32_t(sizeof(S)));
+}
+
+template <class S>
+__device__ __forceinline__ void st_field(S *__restrict__ field, uint32_t idx, S v) {
+    *reinterpret_cast<S *>(reinterpret_cast<char *>(field) + idx * uint32_t(sizeof(S))) = v;
+}
+
+// The 10 stored values of node `idx`, untouched: decoding is left to the consumer so that a
+// prefetch does not wait for its data.
+template <class S>
+__device__ __forceinline__ void load_raw(const FieldSet<const S> &fs, uint32_t idx, S *raw) {
+#pragma unroll
+    for (int f = 0; f < 10; ++f) raw[f] = ld_field(fs.f[f], idx);
+}
+
+// Moments from stored values (M[0] absolute density).
 template <class R, class S>
-__device__ __forceinline__ void load_node(const S *__restrict__ base, size_t stride, uint32_t idx,
-                                          const StoreScale<R> &sc, R rho0, R *M) {
+__device__ __forceinline__ void decode_node(const S *raw, const StoreScale<R> &sc, R rho0, R *M) {
     if constexpr (!Conv<S>::scaled) {
 #pragma unroll
-        for (int f = 0; f < 10; ++f) M[f] = R(Conv<S>::to(base[f * stride + idx]));
+        for (int f = 0; f < 10; ++f) M[f] = R(Conv<S>::to(raw[f]));
         M[0] += rho0;
     } else {
 #pragma unroll
-        for (int f = 0; f < 10; ++f) M[f] = R(Conv<S>::to(base[f * stride + idx])) * sc.dec[field_class(f)];
+        for (int f = 0; f < 10; ++f) M[f] = R(Conv<S>::to(raw[f])) * sc.dec[field_class(f)];
         M[0] += rho0;
         M[4] += M[1] * M[1];
         M[5] += M[2] * M[2];
@@ -100,15 +143,24 @@ __device__ __forceinline__ void load_node(const S *__restrict__ base, size_t str
     }
 }
 
+// Decode the moments of node `idx` (M[0] absolute density).
+template <class R, class S>
+__device__ __forceinline__ void load_node(const FieldSet<const S> &fs, uint32_t idx, const StoreScale<R> &sc, R rho0,
+                                          R *M) {
+    S raw[10];
+    load_raw(fs, idx, raw);
+    decode_node<R, S>(raw, sc, rho0, M);
+}
+
 // Store the moments of node `idx`. For the 16-bit forms `seed` selects stochastic rounding
 // (seed != 0) or round-to-nearest (seed == 0).
 template <class R, class S>
-__device__ __forceinline__ void store_node(S *__restrict__ base, size_t stride, uint32_t idx, const StoreScale<R> &sc,
-                                           R rho0, const R *M, uint32_t seed = 0) {
+__device__ __forceinline__ void store_node(const FieldSet<S> &fs, uint32_t idx, const StoreScale<R> &sc, R rho0,
+                                           const R *M, uint32_t seed = 0) {
     if constexpr (!Conv<S>::scaled) {
-        base[idx] = Conv<S>::from(M[0] - rho0);
+        st_field(fs.f[0], idx, Conv<S>::from(M[0] - rho0));
 #pragma unroll
-        for (int f = 1; f < 10; ++f) base[f * stride + idx] = Conv<S>::from(M[f]);
+        for (int f = 1; f < 10; ++f) st_field(fs.f[f], idx, Conv<S>::from(M[f]));
     } else {
         R v[10];
         v[0] = M[0] - rho0;
@@ -123,15 +175,12 @@ __device__ __forceinline__ void store_node(S *__restrict__ base, size_t stride, 
         v[9] = M[9] - M[2] * M[3];
         if (seed == 0) {
 #pragma unroll
-            for (int f = 0; f < 10; ++f) base[f * stride + idx] = Conv<S>::from(v[f] * sc.enc[field_class(f)]);
+            for (int f = 0; f < 10; ++f) st_field(fs.f[f], idx, Conv<S>::from(v[f] * sc.enc[field_class(f)]));
         } else {
+            const uint32_t h = sr_hash(idx, seed);
 #pragma unroll
-            for (int k = 0; k < 5; ++k) {
-                const uint32_t h = sr_hash(idx, uint32_t(k), seed);
-                base[(2 * k) * stride + idx] = Conv<S>::from_sr(v[2 * k] * sc.enc[field_class(2 * k)], h);
-                base[(2 * k + 1) * stride + idx] =
-                    Conv<S>::from_sr(v[2 * k + 1] * sc.enc[field_class(2 * k + 1)], h >> 16 | h << 16);
-            }
+            for (int f = 0; f < 10; ++f)
+                st_field(fs.f[f], idx, Conv<S>::from_sr(v[f] * sc.enc[field_class(f)], sr_bits(h, f)));
         }
     }
 }

@@ -18,7 +18,7 @@ import os
 import numpy as np
 
 from . import _abi
-from ._abi import CaseDesc, Info, as_ptr
+from ._abi import CaseDesc, Info, TuneResult, as_ptr
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libmrlbm.so")
@@ -39,6 +39,7 @@ def lib():
         if not os.path.exists(LIB_PATH):
             raise MrlbmError(_abi.ERR_DEVICE, f"{LIB_PATH} is missing: run python -m mrlbm_3d_cuda_amd.build")
         _lib = _abi.bind(C.CDLL(LIB_PATH), _abi.PRODUCT_PROTOTYPES)
+        _abi.bind(_lib, _abi.BATCH_PROTOTYPES)
         _abi.bind(_lib, {"mrlbm_create_geometry_only": (C.c_int, [C.POINTER(CaseDesc), C.POINTER(C.c_void_p)])})
     return _lib
 
@@ -57,19 +58,24 @@ def default_desc(kind):
 class Simulation:
     """One case on the GPU, owned by the HIP library."""
 
-    def __init__(self, desc: CaseDesc, geometry_only=False):
+    def __init__(self, desc: CaseDesc, geometry_only=False, _handle=None):
         self.lib = lib()
         self.desc = desc.copy()
-        h = C.c_void_p()
-        fn = self.lib.mrlbm_create_geometry_only if geometry_only else self.lib.mrlbm_create
-        _check(fn(C.byref(self.desc), C.byref(h)))
+        self.owned = _handle is None
+        if _handle is None:
+            h = C.c_void_p()
+            fn = self.lib.mrlbm_create_geometry_only if geometry_only else self.lib.mrlbm_create
+            _check(fn(C.byref(self.desc), C.byref(h)))
+        else:
+            h = _handle
         self.h = h
         self.shape = (desc.nz, desc.ny, desc.nx)
         self.n = desc.num_nodes
 
     def close(self):
         if getattr(self, "h", None):
-            self.lib.mrlbm_destroy(self.h)
+            if self.owned:
+                self.lib.mrlbm_destroy(self.h)
             self.h = None
 
     def __del__(self):
@@ -160,6 +166,77 @@ class Simulation:
         _check(self.lib.mrlbm_get_bcfluid_table(self.h, group, as_ptr(node, C.c_int64), as_ptr(tag, C.c_uint32),
                                                 as_ptr(fin, C.c_uint32), as_ptr(fout, C.c_uint32)))
         return dict(node=node, tag=tag, in_mask=fin, out_mask=fout)
+
+
+class Batch:
+    """Many cases of one precision stepped together (one fused and one wall launch per step).
+
+    `cases` holds a Simulation view of every admitted case: fields, forces and tables read as for
+    a standalone case; the batch steps them."""
+
+    def __init__(self, descs, hbm_budget_bytes=0):
+        self.lib = lib()
+        arr = (CaseDesc * len(descs))(*[d.copy() for d in descs])
+        h, adm = C.c_void_p(), C.c_int32()
+        _check(self.lib.mrlbm_batch_create(arr, len(descs), hbm_budget_bytes, C.byref(h), C.byref(adm)))
+        self.h = h
+        self.admitted = adm.value
+        self.cases = []
+        for i in range(self.admitted):
+            ch = C.c_void_p()
+            _check(self.lib.mrlbm_batch_case(self.h, i, C.byref(ch)))
+            self.cases.append(Simulation(descs[i], _handle=ch))
+        self.num_nodes = sum(d.num_nodes for d in descs[:self.admitted])
+
+    @staticmethod
+    def plan(descs, hbm_budget_bytes):
+        """Host-only admission: (admitted cases, device bytes) within the budget."""
+        arr = (CaseDesc * len(descs))(*[d.copy() for d in descs])
+        adm, nbytes = C.c_int32(), C.c_uint64()
+        _check(lib().mrlbm_batch_plan(arr, len(descs), hbm_budget_bytes, C.byref(adm), C.byref(nbytes)))
+        return adm.value, nbytes.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            for c in self.cases:
+                c.h = None
+            self.lib.mrlbm_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def init(self):
+        _check(self.lib.mrlbm_batch_init(self.h))
+        return self
+
+    def step(self, n=1, forces=False):
+        _check(self.lib.mrlbm_batch_step(self.h, n, _abi.STEP_FORCES if forces else 0))
+        return self
+
+    def sync(self):
+        _check(self.lib.mrlbm_batch_sync(self.h))
+
+    def time_steps(self, n, forces=False):
+        ms = C.c_double()
+        _check(self.lib.mrlbm_batch_time_steps(self.h, n, _abi.STEP_FORCES if forces else 0, C.byref(ms)))
+        return ms.value
+
+    def autotune(self, trial_steps=10):
+        r = TuneResult()
+        _check(self.lib.mrlbm_batch_autotune(self.h, trial_steps, C.byref(r)))
+        return {k: getattr(r, k) for k, _ in r._fields_ if k != "reserved"}
+
+    def launch(self):
+        r = TuneResult()
+        _check(self.lib.mrlbm_batch_get_launch(self.h, C.byref(r)))
+        return {"bx": r.bx, "by": r.by, "variant": r.variant, "lz": r.lz}
 
 
 def algorithmic_bytes_per_update(precision):

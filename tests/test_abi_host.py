@@ -39,10 +39,12 @@ def test_struct_layout_matches_header(tmp_path):
     probe = tmp_path / "probe.c"
     fields = [n for n, _ in _abi.CaseDesc._fields_]
     ifields = [n for n, _ in _abi.Info._fields_]
+    tfields = [n for n, _ in _abi.TuneResult._fields_]
     lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', 'int main(void) {',
-             'printf("%zu %zu\\n", sizeof(mrlbm_case_desc), sizeof(mrlbm_info));']
+             'printf("%zu %zu %zu\\n", sizeof(mrlbm_case_desc), sizeof(mrlbm_info), sizeof(mrlbm_tune_result));']
     lines += [f'printf("d {f} %zu\\n", offsetof(mrlbm_case_desc, {f}));' for f in fields]
     lines += [f'printf("i {f} %zu\\n", offsetof(mrlbm_info, {f}));' for f in ifields]
+    lines += [f'printf("t {f} %zu\\n", offsetof(mrlbm_tune_result, {f}));' for f in tfields]
     lines += ['return 0; }']
     probe.write_text("\n".join(lines))
     exe = tmp_path / "probe"
@@ -51,11 +53,12 @@ def test_struct_layout_matches_header(tmp_path):
     sizes = out[0].split()
     assert int(sizes[0]) == C.sizeof(_abi.CaseDesc)
     assert int(sizes[1]) == C.sizeof(_abi.Info)
+    assert int(sizes[2]) == C.sizeof(_abi.TuneResult)
     for line in out[1:]:
         if not line:
             continue
         kind, name, off = line.split()
-        struct = _abi.CaseDesc if kind == "d" else _abi.Info
+        struct = {"d": _abi.CaseDesc, "i": _abi.Info, "t": _abi.TuneResult}[kind]
         assert getattr(struct, name).offset == int(off), name
 
 
@@ -123,3 +126,44 @@ def test_default_descriptors_match_configs():
         for name in ("nx", "ny", "nz", "xc", "yc", "r_inner", "r_outer", "tau", "u_max", "precision",
                      "periodic_x", "periodic_y", "periodic_z", "inlet_profile", "seed"):
             assert getattr(d, name) == pytest.approx(getattr(ref, name)), name
+
+
+def test_batch_admission_plan():
+    """Cases are admitted from the front while their device bytes stay within the budget; the
+    byte count is two moment buffers and the donor snapshot (10 padded fields each), 4 B of node
+    code per node and the 65536-step FP64 force ring."""
+    from mrlbm_3d_cuda_amd.solver import Batch
+    descs = configs.c5_cases(8)
+
+    def case_bytes(d):
+        n = d.num_nodes
+        stride = (n + 1 + 63) // 64 * 64
+        return 30 * stride * 2 + 4 * n + 65536 * 4 * 8
+
+    sizes = [case_bytes(d) for d in descs]
+    total = sum(sizes)
+    assert Batch.plan(descs, total) == (8, total)
+    assert Batch.plan(descs, total - 1) == (7, total - sizes[-1])
+    assert Batch.plan(descs, sizes[0] + sizes[1]) == (2, sizes[0] + sizes[1])
+    assert Batch.plan(descs, sizes[0] - 1) == (0, 0)
+    mixed = [configs.c1(), configs.c2()]
+    with pytest.raises(MrlbmError) as e:
+        Batch.plan(mixed, 1 << 40)
+    assert e.value.status == _abi.ERR_INVALID
+
+
+def test_sweep_generators_are_seeded_and_in_range():
+    c4, c5 = configs.c4_cases(), configs.c5_cases()
+    assert len(c4) == 64 and len(c5) == 256
+    assert [d.as_dict() for d in c4] == [d.as_dict() for d in configs.c4_cases()]
+    for d in c4:
+        assert 8 <= d.r_inner <= 20 and 10 <= d.r_outer - d.r_inner <= 30
+        assert 16 <= d.nz <= 64 and d.nx == d.ny and d.nx % 8 == 0 and d.nx >= 2 * d.r_outer + 8
+        assert 0.02 <= d.u_wall_inner[1] <= 0.08 and 0.55 <= d.tau <= 1.2
+    for d in c5:
+        assert 64 <= d.nx <= 192 and 64 <= d.ny <= 96 and 32 <= d.nz <= 96
+        assert d.nx % 8 == 0 and d.ny % 8 == 0 and d.nz % 8 == 0
+        assert 4 <= d.r_inner <= 12 and 0.03 <= d.u_max <= 0.06 and d.tau > 0.5
+        assert d.precision == _abi.FP32_BF16
+    assert 14e6 < sum(d.num_nodes for d in c4) < 22e6
+    assert 150e6 < sum(d.num_nodes for d in c5) < 190e6

@@ -1,0 +1,97 @@
+"""Batched launches on the GPU: many uneven cases stepped by one fused and one wall launch per
+step must give every case exactly the state and force records it has when run alone.
+
+Bar: bit-for-bit equality with the standalone runs (the per-node arithmetic is the same; only the
+scheduling differs), for FP64, FP32 and BF16 storage (whose stochastic rounding is seeded per case
+by the case's own step counter).
+"""
+import numpy as np
+import pytest
+
+from mrlbm_3d_cuda_amd import _abi, configs
+
+pytestmark = pytest.mark.gpu
+
+
+def mixed_cases(precision):
+    return [
+        configs.annulus(nx=40, ny=40, nz=8, xc=20.0, yc=20.0, r_in=7.0, r_out=17.0, u_inner=0.04,
+                        precision=precision),
+        configs.cylinder(nx=96, ny=48, nz=8, diameter=10.0, xc=24.0, yc=24.0, u_max=0.05, re=50.0,
+                         precision=precision),
+        configs.periodic_box(36, 20, 12, tau=0.7, precision=precision),
+        configs.cylinder(nx=72, ny=40, nz=40, diameter=8.0, xc=20.0, yc=20.0, u_max=0.04, re=80.0,
+                         precision=precision, seed=configs.SEED_BASE | 0x77),
+    ]
+
+
+def check_same(got, ref, what):
+    diff = float(np.max(np.abs(np.asarray(got) - np.asarray(ref))))
+    equal = bool(np.array_equal(got, ref))
+    assert equal, f"{what}: max |difference| {diff:.3e}"
+
+
+def standalone(desc, steps):
+    from mrlbm_3d_cuda_amd.solver import Simulation
+    s = Simulation(desc).init()
+    s.step(steps, forces=True)
+    F = s.fields()
+    f = s.forces(0, steps)
+    s.close()
+    return F, f
+
+
+@pytest.mark.parametrize("precision", [_abi.FP64, _abi.FP32, _abi.FP32_BF16])
+def test_batch_equals_standalone_runs(precision):
+    from mrlbm_3d_cuda_amd.solver import Batch
+    descs = mixed_cases(precision)
+    steps = 37  # two graph replays and an eager remainder
+    print(f"[gpu batch] {_abi.PRECISION_NAMES[precision]}: {len(descs)} cases, {steps} steps", flush=True)
+    with Batch(descs) as b:
+        assert b.admitted == len(descs)
+        b.init().step(steps, forces=True)
+        got = [(c.fields(), c.forces(0, steps)) for c in b.cases]
+        assert all(c.info().steps_done == steps for c in b.cases)
+    for i, d in enumerate(descs):
+        F, f = standalone(d, steps)
+        check_same(got[i][0], F, f"case {i} fields")
+        check_same(got[i][1], f, f"case {i} forces")
+
+
+def test_batch_autotune_keeps_results(monkeypatch):
+    """The tuner times every candidate launch configuration, re-initialises the cases and keeps
+    the fastest; results after tuning equal standalone runs with the same launch configuration
+    bit for bit. (In FP32 the compiler may contract multiply-adds differently in different kernel
+    instantiations, so only the same tile and march variant are compared bitwise; the FP64
+    kernels are compiled without contraction and are tile invariant.)"""
+    from mrlbm_3d_cuda_amd.solver import Batch
+    descs = mixed_cases(_abi.FP32)
+    with Batch(descs) as b:
+        r = b.autotune(trial_steps=4)
+        print(f"[gpu batch] tuned: {r}", flush=True)
+        assert r["candidates"] >= 12
+        assert (r["bx"], r["by"]) in [(32, 8), (32, 4), (16, 8)] and r["variant"] in (1, 3)
+        assert 0 < r["ms_per_step"] <= r["ms_per_step_default"] * 1.0001
+        assert b.launch() == {k: r[k] for k in ("bx", "by", "variant", "lz")}
+        assert all(c.info().steps_done == 0 for c in b.cases)
+        b.step(20, forces=True)
+        got = [(c.fields(), c.forces(0, 20)) for c in b.cases]
+    monkeypatch.setenv("MRLBM_TILE", f"{r['bx']}x{r['by']}")
+    monkeypatch.setenv("MRLBM_FUSED_VARIANT", str(r["variant"]))
+    monkeypatch.setenv("MRLBM_LZ", str(r["lz"]))
+    for i, d in enumerate(descs):
+        F, f = standalone(d, 20)
+        check_same(got[i][0], F, f"case {i} fields")
+        check_same(got[i][1], f, f"case {i} forces")
+
+
+def test_batch_case_handles_refuse_stepping():
+    from mrlbm_3d_cuda_amd.solver import Batch, MrlbmError
+    with Batch(mixed_cases(_abi.FP32)[:2]) as b:
+        b.init()
+        c = b.cases[0]
+        for call in (lambda: c.step(1), c.init, lambda: c.time_steps(2)):
+            with pytest.raises(MrlbmError) as e:
+                call()
+            assert e.value.status == _abi.ERR_INVALID
+        assert b.lib.mrlbm_destroy(c.h) == _abi.ERR_INVALID
