@@ -55,7 +55,14 @@ class Batch final : public BatchBase {
                 for (int b = 0; b < nb; ++b) blocks.push_back(make_int2(int(c), b));
             }
             nwblocks_[f] = int(blocks.size());
-            int rc = upload(d_wblocks_[f], blocks);
+            std::vector<int> cases_of;  // cases with a wall pass
+            for (size_t c = 0; c < nc; ++c)
+                if (sim(c).has_wall_launch(flags)) cases_of.push_back(int(c));
+            nwcases_[f] = int(cases_of.size());
+            if (int rc = upload(d_wcases_[f], cases_of)) return rc;
+            std::vector<int2> order(blocks.size());  // dispatch slot -> block (XCD-contiguous)
+            for (size_t b = 0; b < blocks.size(); ++b) order[b] = blocks[size_t(xcd_contiguous(int(b), int(blocks.size())))];
+            int rc = upload(d_wblocks_[f], order);
             if (rc) return rc;
         }
         return configure(sim(0).launch_config());
@@ -160,18 +167,19 @@ class Batch final : public BatchBase {
     }
 
     // Times trial_steps steps (with forces) of every candidate configuration -- tile 32x8, 32x4 or
-    // 16x8 (where its LDS fits), ring or register-window march, z chunks of 8, 16, 32 planes --
+    // 16x8 (where its LDS fits), ring or register-window march, z chunks of 4, 8, 16, 32 planes --
     // and keeps the fastest. The cases' states are scrambled by the trial
     // steps: the tuner re-initialises them at the end.
     int autotune(int trial_steps, mrlbm_tune_result *out) override {
         if (trial_steps <= 0) return set_error(MRLBM_ERR_INVALID, "trial steps must be positive");
-        const LaunchConfig def = sim(0).default_launch();
-        std::vector<LaunchConfig> cands;
+        const LaunchConfig def = launch;  // the configuration in use is timed first
+        std::vector<LaunchConfig> cands{def};
         const int tiles[3][2] = {{32, 8}, {32, 4}, {16, 8}};
         for (const auto &t : tiles)
             for (int variant : {1, 3})
-                for (int lz : {8, 16, 32}) {
+                for (int lz : {4, 8, 16, 32}) {
                     LaunchConfig lc = def;
+                    lc.xcd_remap = true;
                     lc.bx = t[0];
                     lc.by = t[1];
                     lc.variant = variant;
@@ -181,6 +189,9 @@ class Batch final : public BatchBase {
                     HIP_TRY(hipGetDevice(&dev));
                     HIP_TRY(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
                     if (fused_lds_bytes(lc.bx, lc.by, sizeof(R), lc.variant) > size_t(max_lds)) continue;
+                    if (lc.bx == def.bx && lc.by == def.by && lc.variant == def.variant && lc.lz == def.lz &&
+                        lc.lz_special == def.lz_special)
+                        continue;
                     cands.push_back(lc);
                 }
         if (int rc = init()) return rc;
@@ -203,7 +214,7 @@ class Batch final : public BatchBase {
             }
             ms /= trial_steps;
             ++tried;
-            if (lc.bx == def.bx && lc.by == def.by && lc.variant == def.variant && lc.lz == def.lz) at_default = ms;
+            if (tried == 1) at_default = ms;
             if (ms < best) {
                 best = ms;
                 best_lc = lc;
@@ -236,7 +247,8 @@ class Batch final : public BatchBase {
         launch_fused_batch<R, S>(d_fargs_[cur], d_items_, nitems_, groups_, launch.bx, launch.by, launch.variant,
                                  stream_);
         const int f = (flags & MRLBM_STEP_FORCES) ? 1 : 0;
-        if (nwblocks_[f] > 0) launch_wall_batch<R, S>(d_wargs_[cur][f], d_wblocks_[f], nwblocks_[f], stream_);
+        if (nwblocks_[f] > 0)
+            launch_wall_batch<R, S>(d_wargs_[cur][f], d_wblocks_[f], nwblocks_[f], d_wcases_[f], nwcases_[f], stream_);
     }
 
     int graph_for(uint32_t flags, hipGraphExec_t &exec) {
@@ -275,7 +287,7 @@ class Batch final : public BatchBase {
         if (stream_) (void)hipStreamSynchronize(stream_);
         drop_graphs();
         void *ptrs[] = {d_fargs_[0], d_fargs_[1], d_wargs_[0][0], d_wargs_[0][1], d_wargs_[1][0], d_wargs_[1][1],
-                        d_wblocks_[0], d_wblocks_[1], d_items_};
+                        d_wblocks_[0], d_wblocks_[1], d_wcases_[0], d_wcases_[1], d_items_};
         for (void *p : ptrs)
             if (p) (void)hipFree(p);
         for (auto &c : cases) c->impl->batch_stream = nullptr;
@@ -290,6 +302,8 @@ class Batch final : public BatchBase {
     WallArgs<R, S> *d_wargs_[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
     int2 *d_wblocks_[2] = {nullptr, nullptr};
     int nwblocks_[2] = {0, 0};
+    int *d_wcases_[2] = {nullptr, nullptr};
+    int nwcases_[2] = {0, 0};
     uint4 *d_items_ = nullptr;
     int nitems_ = 0, groups_ = 1;
     hipGraphExec_t graphs_[4] = {nullptr, nullptr, nullptr, nullptr};

@@ -17,7 +17,7 @@ template <class R, class S>
 struct FusedArgs {
     FieldSet<const S> src;  // moments at the start of the step
     FieldSet<S> dst;        // moments after the step
-    FieldSet<S> donor;      // pre-collision snapshot of donor nodes
+    S *__restrict__ donor;  // pre-collision snapshot of donor nodes, node-major
     uint32_t pad;           // index of a padding slot after the last node (discarded stores)
     int nx, ny, nz;
     int tiles_x;     // tiles along x
@@ -27,11 +27,12 @@ struct FusedArgs {
     // with boundary nodes, which need info[]) come first and march shorter z ranges
     const uint2 *__restrict__ items;
     const uint32_t *__restrict__ info;  // per-node NodeCode | flags | payload
-    const R *__restrict__ wall_nx;
+    const R *__restrict__ wall_nx;      // wall_count entries (at least 1)
     const R *__restrict__ wall_ny;
     const uint32_t *__restrict__ wall_in;
     const uint32_t *__restrict__ bcf_in;  // 256 per-tag incoming masks
-    const R *__restrict__ inlet;          // u_x per (y, z)
+    const R *__restrict__ inlet;          // u_x per (y, z); inlet_count entries (at least 1)
+    int wall_count, inlet_count;
     // steps completed on the device (advanced by wall_step); seeds stochastic rounding of the
     // 16-bit forms, null for round-to-nearest
     const unsigned long long *__restrict__ step_counter;
@@ -43,11 +44,12 @@ struct FusedArgs {
 template <class R, class S>
 struct WallArgs {
     FieldSet<S> mom;          // buffer the fused pass just wrote
-    FieldSet<const S> donor;  // its pre-collision snapshot
+    const S *__restrict__ donor;  // its pre-collision snapshot, node-major
     int nwall, nb_inner, nbcf;
     const uint32_t *__restrict__ node;
     const R *__restrict__ wnx;
     const R *__restrict__ wny;
+    // per-wall records, component-major (component j of wall i at j * nwall + i)
     const R *__restrict__ coef;        // 3 extrapolation coefficients per wall
     const R *__restrict__ wconst;      // WALL_CONSTS closure constants per wall
     const uint32_t *__restrict__ in_mask;
@@ -63,9 +65,8 @@ struct WallArgs {
     StoreScale<R> scale;
     R omega, rho0;
     int forces;
-    int nblocks;                    // blocks of this case in the launch (ticket target)
+    int nblocks;                    // blocks of this case in the wall pass
     double *__restrict__ partials;  // 4 per block
-    unsigned *__restrict__ ticket;
     double *__restrict__ ring;      // 4 per step, FORCE_RING_STEPS records
     unsigned long long *__restrict__ step_counter;  // steps completed on the device
 };
@@ -79,6 +80,14 @@ constexpr int WALL_THREADS = 64;
 
 inline int wall_blocks(int nwall, int nbcf) { return std::max(1, (nwall + nbcf + WALL_THREADS - 1) / WALL_THREADS); }
 
+// Blocks are dispatched round-robin over the 8 XCDs; the wall pass maps dispatch slot b to the
+// logical block that keeps consecutive wall nodes (which share stencil cache lines) on one XCD:
+// XCD j takes a contiguous range of n / 8 (+1 for the first n mod 8 XCDs) logical blocks.
+__host__ __device__ inline int xcd_contiguous(int b, int n) {
+    const int q = n / 8, r = n % 8, j = b % 8, k = b / 8;
+    return j * q + (j < r ? j : r) + k;
+}
+
 // variant 1: ring march; 3, 4: register-window march with one / two LDS planes
 template <class R, class S>
 void launch_fused(const FusedArgs<R, S> &a, int bx, int by, int variant, hipStream_t stream);
@@ -88,13 +97,15 @@ void launch_wall(const WallArgs<R, S> &a, hipStream_t stream);
 
 // Batched launches: one fused pass over the work items of many cases (item {case, tile |
 // special << 31, z0 | z1 << 16, 0}; tiles BX x BY in {32x8, 16x8, 32x4}), and one wall pass
-// whose block b runs block blocks[b].y of case blocks[b].x. Argument arrays live on the device.
+// whose block b runs block blocks[b].y of case blocks[b].x, followed by one block per case in
+// cases_of that files the case's per-step record. Argument arrays live on the device.
 template <class R, class S>
 void launch_fused_batch(const FusedArgs<R, S> *cases, const uint4 *items, int nitems, int groups, int bx, int by,
                         int variant, hipStream_t stream);
 
 template <class R, class S>
-void launch_wall_batch(const WallArgs<R, S> *cases, const int2 *blocks, int nblocks, hipStream_t stream);
+void launch_wall_batch(const WallArgs<R, S> *cases, const int2 *blocks, int nblocks, const int *cases_of,
+                       int ncases, hipStream_t stream);
 
 // Sets the dynamic-LDS limit of the batched fused kernel for this configuration (once, before any
 // launch or graph capture); 0 on success.
@@ -129,10 +140,11 @@ inline bool fused_tile_supported(int bx, int by) {
 // Bytes of dynamic LDS one fused block uses over the tile and its ring: three planes of the
 // c_z=+1 population group, two of the c_z=0 group and one of the c_z=-1 group (ring march,
 // variant 1), or one / two planes of all 27 pulled populations (register-window march,
-// variants 3 / 4).
+// variants 3 / 4); then the moments of the tile's west ring column for two planes.
 inline size_t fused_lds_bytes(int bx, int by, size_t real_bytes, int variant) {
     const size_t planes = variant == 3 ? 27 : 54;
-    return planes * size_t(bx + 2) * size_t(by + 2) * real_bytes;
+    // + the west ring column's moments of two planes (boundary tiles)
+    return (planes * size_t(bx + 2) * size_t(by + 2) + 20 * size_t(by)) * real_bytes;
 }
 
 }  // namespace mrlbm

@@ -40,46 +40,90 @@ __device__ __forceinline__ int wrap(int v, int n) {
     return v >= n ? v - n : v;
 }
 
-// Post-stream moments of one node from its 27 pulled populations f (in the LDS form of
-// reconstruct_lds), node boundary condition, donor snapshot and collision, written to dst (the
-// per-node part of streaming_and_evaluate_Mom and collision_halo_update,
-// solver/mlbm.cu:83-164, 186-242). `special` is uniform per block: blocks without boundary nodes
-// skip the node-type look-up.
-template <class R, class S>
-__device__ __forceinline__ void update_node(const FusedArgs<R, S> &a, R *f, int x, int y, int z, uint32_t plane,
-                                            bool special) {
-    const uint32_t idx = uint32_t(x) + uint32_t(a.nx) * uint32_t(y) + uint32_t(z) * plane;
-    uint32_t seed = 0;
-    if constexpr (Conv<S>::scaled) {
-        if (a.step_counter) seed = uint32_t(*a.step_counter) * 2u + 1u;
+// Frame of a tile: the BX x BY tile nodes and a one-node ring around them, as a (BX+2) x (BY+2)
+// grid of positions. Thread t owns tile node (t % BX, t / BX) as its first position; the ring
+// positions -- south row, north row, west column, east column -- are the second position of the
+// first NR threads.
+template <int BX, int BY>
+struct Frame {
+    static constexpr int W = BX + 2;
+    static constexpr int NH = W * (BY + 2);
+    static constexpr int NT = BX * BY;
+    static constexpr int NR = 2 * W + 2 * BY;
+    __device__ static void ring(int r, int &hx, int &hy) {
+        if (r < W) {
+            hx = r;
+            hy = 0;
+        } else if (r < 2 * W) {
+            hx = r - W;
+            hy = BY + 1;
+        } else if (r < 2 * W + BY) {
+            hx = 0;
+            hy = r - 2 * W + 1;
+        } else {
+            hx = W - 1;
+            hy = r - 2 * W - BY + 1;
+        }
     }
-    R M[10];
-    if (!special) {
+};
+
+// v mod n in [0, n)
+__device__ __forceinline__ int pmod(int v, int n) {
+    const int r = v % n;
+    return r < 0 ? r + n : r;
+}
+
+// Per-node table values of a boundary node, loaded one iteration before its update with
+// in-range indices whatever the node code (so every thread issues the same loads).
+template <class R>
+struct NodeTabs {
+    R wnx, wny, inlet;
+    uint32_t win, bcfin;
+};
+
+template <class R, class S>
+__device__ __forceinline__ void load_tabs(const FusedArgs<R, S> &a, uint32_t info, uint32_t inlet_idx, NodeTabs<R> &t) {
+    const uint32_t payload = info >> INFO_PAYLOAD_SHIFT;
+    const uint32_t wi = min(payload, uint32_t(a.wall_count - 1));
+    t.wnx = a.wall_nx[wi];
+    t.wny = a.wall_ny[wi];
+    t.win = a.wall_in[wi];
+    t.bcfin = a.bcf_in[payload & 0xFFu];
+    t.inlet = a.inlet[min(inlet_idx, uint32_t(a.inlet_count - 1))];
+}
+
+// Post-stream moments of one node of a boundary tile from its 27 pulled populations f (in the
+// LDS form of reconstruct_lds), its moments at the start of the step (old) and those of its x-1
+// neighbour (oxm), with the node boundary condition (the per-node part of
+// streaming_and_evaluate_Mom and collision_halo_update, solver/mlbm.cu:83-164, 186-242). Returns
+// in M the value to store (collided unless the wall pass finishes the node) and in Md the
+// pre-collision snapshot for donor nodes; store is false for frozen nodes.
+template <class R, class S>
+__device__ __forceinline__ void boundary_node(const FusedArgs<R, S> &a, R *f, const R *old, const R *oxm,
+                                              uint32_t info, const NodeTabs<R> &t, R *M, R *Md, bool &store,
+                                              bool &donor) {
+    const uint32_t code = info & INFO_CODE_MASK;
+    const uint32_t payload = info >> INFO_PAYLOAD_SHIFT;
+    bool do_collide = true;
+    store = true;
+    if (code == NC_BULK) {
         moments_lds(f, M);
-        collide(a.omega, M);
-        store_node<R, S>(a.dst, idx, a.scale, a.rho0, M, seed);
     } else {
         lds_to_pop(f);
-        const uint32_t info = a.info[idx];
-        const uint32_t code = info & INFO_CODE_MASK;
-        const uint32_t payload = info >> INFO_PAYLOAD_SHIFT;
-        bool do_collide = true;
-        if (code == NC_BULK) {
-            moments(f, M);
-        } else if (code == NC_KEEP || code == NC_FROZEN) {
-            load_node<R, S>(a.src, idx, a.scale, a.rho0, M);
-            do_collide = code == NC_KEEP;
+        if (code == NC_FROZEN || code == NC_KEEP) {
+#pragma unroll
+            for (int k = 0; k < 10; ++k) M[k] = old[k];
+            store = code == NC_KEEP;  // frozen: both buffers already hold its value
         } else if (code == NC_WALL) {
             // u keeps its old value; rho and m become the incoming rotated moments
-            load_node<R, S>(a.src, idx, a.scale, a.rho0, M);
-            incoming_rotated(a.wall_nx[payload], a.wall_ny[payload], a.wall_in[payload], f, M);
+#pragma unroll
+            for (int k = 0; k < 10; ++k) M[k] = old[k];
+            incoming_rotated(t.wnx, t.wny, t.win, f, M);
             do_collide = false;  // finished by wall_step
         } else if (code == NC_BCF) {
-            bc_fluid(payload, a.bcf_in[payload], f, M);
+            bc_fluid(payload, t.bcfin, f, M);
         } else if (code == NC_WEST) {
-            R old[10];
-            load_node<R, S>(a.src, idx, a.scale, a.rho0, old);
-            bc_west(f, old, a.inlet[uint32_t(y) + uint32_t(a.ny) * uint32_t(z)], M);
+            bc_west(f, old, t.inlet, M);
         } else if (code == NC_NORTH) {
             bc_ywall<face::K_N>(f, M);
         } else if (code == NC_SOUTH) {
@@ -88,75 +132,118 @@ __device__ __forceinline__ void update_node(const FusedArgs<R, S> &a, R *f, int 
             bc_west_edge<face::K_W & face::K_N, -1>(f, a.omega, M);
         } else if (code == NC_SOUTH_WEST) {
             bc_west_edge<face::K_W & face::K_S, +1>(f, a.omega, M);
-        } else {  // NC_EAST, NC_NORTH_EAST, NC_SOUTH_EAST
-            R old[10], oxm[10];
-            load_node<R, S>(a.src, idx, a.scale, a.rho0, old);
-            load_node<R, S>(a.src, idx - 1, a.scale, a.rho0, oxm);
-            if (code == NC_EAST)
-                bc_east(f, old, oxm, M);
-            else if (code == NC_NORTH_EAST)
-                bc_east_edge<face::K_E & face::K_N, +1>(f, old, oxm, M);
-            else
-                bc_east_edge<face::K_E & face::K_S, -1>(f, old, oxm, M);
+        } else if (code == NC_EAST) {
+            bc_east(f, old, oxm, M);
+        } else if (code == NC_NORTH_EAST) {
+            bc_east_edge<face::K_E & face::K_N, +1>(f, old, oxm, M);
+        } else {
+            bc_east_edge<face::K_E & face::K_S, -1>(f, old, oxm, M);
         }
-        if (info & INFO_DONOR) store_node<R, S>(a.donor, idx, a.scale, a.rho0, M, seed ^ 0x5A5A5A5Au);
-        if (do_collide) collide(a.omega, M);
-        store_node<R, S>(a.dst, idx, a.scale, a.rho0, M, seed);
     }
+    donor = store && (info & INFO_DONOR);
+#pragma unroll
+    for (int k = 0; k < 10; ++k) Md[k] = M[k];
+    if (do_collide) collide(a.omega, M);
 }
 
+// State and steps of the boundary-tile path shared by both marches. Per thread: the stored
+// values of its own node for the plane being updated and the next one, the node codes of those
+// planes, the table values of the plane being updated, and the previous plane's results (stored
+// at the start of the next iteration, like the bulk path's). The moments of the west ring column
+// go through LDS (wold) so that a node in the tile's first column can read its x-1 neighbour.
+template <class R, class S, int BX, int BY>
+struct BoundaryPath {
+    static constexpr int W = BX + 2;
+    S own_cur[10], own_nx[10];
+    uint32_t info_cur = 0, info_nx = 0;
+    NodeTabs<R> tab;
+    R Mp[10], Mdp[10];
+    uint32_t idxp, didxp;
+
+    __device__ void load_info(const FusedArgs<R, S> &a, uint32_t node, uint32_t &out) { out = a.info[node]; }
+    __device__ void copy_own(const S *raw, S *dst) {
+#pragma unroll
+        for (int k = 0; k < 10; ++k) dst[k] = raw[k];
+    }
+    __device__ void store_prev(const FusedArgs<R, S> &a, uint32_t seed) {
+        store_node<R, S>(a.dst, idxp, a.scale, a.rho0, Mp, seed);
+        store_node_nm<R, S>(a.donor, didxp, a.scale, a.rho0, Mdp, seed ^ 0x5A5A5A5Au);
+    }
+    // the x-1 neighbour's moments at the plane being updated: lane - 1 inside the tile row, the
+    // west ring column (LDS) for the first column
+    __device__ void neighbour_xm1(const R *old, int tx, int ty, const R *wold, R *oxm) {
+#pragma unroll
+        for (int k = 0; k < 10; ++k) {
+            const R v = __shfl(old[k], int(threadIdx.x) - 1, 64);
+            oxm[k] = tx > 0 ? v : wold[k * BY + ty];
+        }
+    }
+    __device__ void update(const FusedArgs<R, S> &a, R *f, int tx, int ty, bool inside, bool east_tile, uint32_t node,
+                           const R *wold) {
+        R old[10], oxm[10];
+        decode_node<R, S>(own_cur, a.scale, a.rho0, old);
+#pragma unroll
+        for (int k = 0; k < 10; ++k) oxm[k] = old[k];
+        if (east_tile) neighbour_xm1(old, tx, ty, wold, oxm);  // uniform per block
+        bool store, donor;
+        boundary_node(a, f, old, oxm, info_cur, tab, Mp, Mdp, store, donor);
+        idxp = (inside && store) ? node : a.pad;
+        didxp = (inside && donor) ? node : a.pad;
+    }
+};
+
 // z march of one work item. Populations of the last planes are kept in LDS grouped by c_z
-// (3 planes of the c_z=+1 group, 2 of c_z=0, 1 of c_z=-1); one thread per tile node, and the
-// (BX+2)(BY+2) - BX*BY ring positions spread over the first threads as a second position. The
-// stored values of the next plane are in flight in registers while a plane is updated.
+// (3 planes of the c_z=+1 group, 2 of c_z=0, 1 of c_z=-1). The stored values of the next plane
+// are in flight in registers while a plane is updated.
 //
-// Vector-memory loads and stores share one counter, and a wait for a load while stores are
-// outstanding waits for everything. Bulk tiles (SPECIAL false) therefore keep a plane's results
-// in registers and store them at the start of the next iteration, after the wait for the
-// prefetched plane and before the next prefetch is issued, so a prefetch is never queued behind
-// stores of the same iteration. In bulk tiles the prefetch past the item wraps around in z and
-// threads outside the grid store into the padding slot after the last node, so the stores are
-// not predicated.
+// Vector-memory loads and stores share one counter, counted in issue order. Every path of an
+// iteration issues the same loads and stores: results of a plane are kept in registers and
+// stored at the start of the next iteration (after the wait for the prefetched plane, before the
+// next prefetch), threads outside the grid, frozen nodes and non-donors store into the padding
+// slot after the last node, second-position loads of threads without one repeat the first, and
+// the prefetch past the item wraps around in z. A wait for a prefetched plane therefore never
+// waits for stores or loads issued after it.
 template <class R, class S, int BX, int BY, bool SPECIAL>
 __device__ __forceinline__ void fused_march(const FusedArgs<R, S> &a, R *lds, int tile, int z0, int z1) {
-    constexpr int NT = BX * BY;
-    constexpr int W = BX + 2;
-    constexpr int NH = W * (BY + 2);
-    // group planes: zp[3][9][NH], z0[2][9][NH], zm[9][NH]
+    using F = Frame<BX, BY>;
+    constexpr int W = F::W;
+    constexpr int NH = F::NH;
+    // group planes: zp[3][9][NH], z0[2][9][NH], zm[9][NH]; then the west-column moments of two planes
     R *const gp = lds;
     R *const g0 = lds + 27 * NH;
     R *const gm = lds + 45 * NH;
+    R *const wold = lds + 54 * NH;
 
     const int x0 = (tile % a.tiles_x) * BX;
     const int y0 = (tile / a.tiles_x) * BY;
     const int tid = threadIdx.x;
     const int tx = tid % BX, ty = tid / BX;
     const uint32_t plane = uint32_t(a.nx) * uint32_t(a.ny);
+    const int h = (ty + 1) * W + (tx + 1);  // first position: the thread's own node
 
-    // ring/tile positions loaded by this thread: p0 = tid, p1 = tid + NT (if inside the ring)
-    const int p1 = tid + NT;
-    const bool has1 = p1 < NH;
-    uint32_t off0, off1;
-    {
-        const int hx = tid % W, hy = tid / W;
-        off0 = uint32_t(wrap(x0 - 1 + hx, a.nx)) + uint32_t(a.nx) * uint32_t(wrap(y0 - 1 + hy, a.ny));
-        off1 = off0;
-        if (has1) {
-            const int hx1 = p1 % W, hy1 = p1 / W;
-            off1 = uint32_t(wrap(x0 - 1 + hx1, a.nx)) + uint32_t(a.nx) * uint32_t(wrap(y0 - 1 + hy1, a.ny));
-        }
-    }
+    const bool has1 = tid < F::NR;
+    int hx1 = 0, hy1 = 0;
+    if (has1) F::ring(tid, hx1, hy1);
+    const int p1 = hy1 * W + hx1;
+    const uint32_t off0 = uint32_t(pmod(x0 + tx, a.nx)) + uint32_t(a.nx) * uint32_t(pmod(y0 + ty, a.ny));
+    const uint32_t off1 = has1 ? uint32_t(pmod(x0 - 1 + hx1, a.nx)) + uint32_t(a.nx) * uint32_t(pmod(y0 - 1 + hy1, a.ny))
+                               : off0;
+    const bool west1 = has1 && hx1 == 0 && hy1 >= 1 && hy1 <= BY;  // west ring column
 
     S m0[10], m1[10];  // stored values of the plane in flight, per position
     auto load_plane = [&](int z) {
-        const uint32_t base = uint32_t(wrap(z, a.nz)) * plane;
+        const uint32_t base = uint32_t(pmod(z, a.nz)) * plane;
         load_raw(a.src, base + off0, m0);
-        if (has1) load_raw(a.src, base + off1, m1);
+        load_raw(a.src, base + off1, m1);
     };
-    // write the populations of a loaded plane into the requested group slots
-    auto put = [&](const S *raw, int p, int sp, int s0, bool want0, bool wantm) {
+    // write the populations of a loaded plane (index zp) into the requested group slots
+    auto put = [&](const S *raw, int p, int sp, int s0, bool want0, bool wantm, bool west, int zp) {
         R m[10], f[Q];
         decode_node<R, S>(raw, a.scale, a.rho0, m);
+        if (SPECIAL && west) {
+#pragma unroll
+            for (int k = 0; k < 10; ++k) wold[((zp & 1) * 10 + k) * BY + (hy1 - 1)] = m[k];
+        }
         reconstruct_lds(m, f);
 #pragma unroll
         for (int i = 0; i < 9; ++i) gp[(sp * 9 + i) * NH + p] = f[GROUP_ZP[i]];
@@ -169,28 +256,35 @@ __device__ __forceinline__ void fused_march(const FusedArgs<R, S> &a, R *lds, in
             for (int i = 0; i < 9; ++i) gm[i * NH + p] = f[GROUP_ZM[i]];
         }
     };
-    auto put_plane = [&](int sp, int s0, bool want0, bool wantm) {
-        put(m0, tid, sp, s0, want0, wantm);
-        if (has1) put(m1, p1, sp, s0, want0, wantm);
+    auto put_plane = [&](int sp, int s0, bool want0, bool wantm, int zp) {
+        put(m0, h, sp, s0, want0, wantm, false, zp);
+        if (has1) put(m1, p1, sp, s0, want0, wantm, west1, zp);
     };
-
-    // prologue: plane z0-1 (c_z=+1 group) and plane z0 (c_z=+1 and 0 groups), then z0+1 in flight
-    load_plane(z0 - 1);
-    put_plane(0, 0, false, false);
-    load_plane(z0);
-    put_plane(1, 0, true, false);
-    load_plane(z0 + 1);
 
     const int x = x0 + tx, y = y0 + ty;
     const bool inside = x < a.nx && y < a.ny;
-    const int h = (ty + 1) * W + (tx + 1);
-    int sp_prev = 0, sp_cur = 1, sp_next = 2;  // slots of planes z-1, z, z+1 in the +1 group
-    int s0_cur = 0;
+    const bool east_tile = x0 + BX >= a.nx;  // holds the last x column
     uint32_t seed = 0;  // stochastic rounding of the 16-bit forms
     if constexpr (Conv<S>::scaled) {
         if (a.step_counter) seed = uint32_t(*a.step_counter) * 2u + 1u;
     }
-    R Mp[10];           // bulk: results of the previous plane, stored one iteration late
+    BoundaryPath<R, S, BX, BY> bp;
+
+    // prologue: plane z0-1 (c_z=+1 group) and plane z0 (c_z=+1 and 0 groups), then z0+1 in flight
+    load_plane(z0 - 1);
+    put_plane(0, 0, false, false, z0 - 1);
+    load_plane(z0);
+    put_plane(1, 0, true, false, z0);
+    if constexpr (SPECIAL) {
+        bp.copy_own(m0, bp.own_cur);
+        bp.load_info(a, off0 + uint32_t(z0) * plane, bp.info_cur);
+        bp.idxp = bp.didxp = a.pad;
+    }
+    load_plane(z0 + 1);
+
+    int sp_prev = 0, sp_cur = 1, sp_next = 2;  // slots of planes z-1, z, z+1 in the +1 group
+    int s0_cur = 0;
+    R Mp[10];  // bulk: results of the previous plane, stored one iteration late
     uint32_t idxp = a.pad;
 
     auto pull = [&](R *f) {
@@ -218,37 +312,39 @@ __device__ __forceinline__ void fused_march(const FusedArgs<R, S> &a, R *lds, in
         s0_cur ^= 1;
     };
 
-    if constexpr (SPECIAL) {
-        for (int z = z0; z < z1; ++z) {
-            put_plane(sp_next, s0_cur ^ 1, true, true);  // plane z+1
-            if (z + 1 < z1) load_plane(z + 2);            // prefetch, consumed next iteration
-            lds_barrier();
-            if (inside) {
-                R f[Q];
-                pull(f);
-                update_node(a, f, x, y, z, plane, true);
-            }
-            lds_barrier();
-            rotate();
+    for (int z = z0; z < z1; ++z) {
+        if constexpr (SPECIAL) load_tabs(a, bp.info_cur, uint32_t(y) + uint32_t(a.ny) * uint32_t(z), bp.tab);
+        put_plane(sp_next, s0_cur ^ 1, true, true, z + 1);  // plane z+1
+        if constexpr (SPECIAL) {
+            bp.copy_own(m0, bp.own_nx);
+            bp.store_prev(a, seed);
+            bp.load_info(a, off0 + uint32_t(pmod(z + 1, a.nz)) * plane, bp.info_nx);
+        } else {
+            store_node<R, S>(a.dst, idxp, a.scale, a.rho0, Mp, seed);
         }
-    } else {
-        auto iteration = [&](int z, bool store_prev) {
-            put_plane(sp_next, s0_cur ^ 1, true, true);  // plane z+1
-            if (store_prev) store_node<R, S>(a.dst, idxp, a.scale, a.rho0, Mp, seed);
-            load_plane(z + 2);  // prefetch, consumed next iteration (wraps past the item)
-            lds_barrier();
-            R f[Q];
-            pull(f);
+        load_plane(z + 2);  // prefetch, consumed next iteration (wraps past the item)
+        lds_barrier();
+        R f[Q];
+        pull(f);
+        const uint32_t node = off0 + uint32_t(z) * plane;
+        if constexpr (SPECIAL) {
+            bp.update(a, f, tx, ty, inside, east_tile, node, wold + (z & 1) * 10 * BY);
+        } else {
             moments_lds(f, Mp);
             collide(a.omega, Mp);
-            idxp = inside ? uint32_t(x) + uint32_t(a.nx) * uint32_t(y) + uint32_t(z) * plane : a.pad;
-            lds_barrier();
-            rotate();
-        };
-        iteration(z0, false);
-        for (int z = z0 + 1; z < z1; ++z) iteration(z, true);
-        store_node<R, S>(a.dst, idxp, a.scale, a.rho0, Mp, seed);
+            idxp = inside ? node : a.pad;
+        }
+        lds_barrier();
+        rotate();
+        if constexpr (SPECIAL) {
+            bp.copy_own(bp.own_nx, bp.own_cur);
+            bp.info_cur = bp.info_nx;
+        }
     }
+    if constexpr (SPECIAL)
+        bp.store_prev(a, seed);
+    else
+        store_node<R, S>(a.dst, idxp, a.scale, a.rho0, Mp, seed);
 }
 
 // Register-window march: LDS holds NBUF planes of the 27 populations pulled in one iteration;
@@ -256,78 +352,88 @@ __device__ __forceinline__ void fused_march(const FusedArgs<R, S> &a, R *lds, in
 // populations of its last plane, and writes them into LDS in the iteration that pulls them.
 // NBUF = 1 needs two barriers per plane and a quarter of the LDS of the ring march (more blocks
 // per CU); NBUF = 2 alternates two LDS planes and needs one barrier per plane. The z loop is
-// unrolled by two so the two c_z=+1 windows swap roles without register moves. Stores of bulk
-// tiles are delayed by one iteration as in fused_march.
+// unrolled by two so the two c_z=+1 windows swap roles without register moves. Memory
+// operations are ordered and made unconditional as in fused_march.
 template <class R, class S, int BX, int BY, int NBUF, bool SPECIAL>
 __device__ __forceinline__ void window_march(const FusedArgs<R, S> &a, R *lds, int tile, int z0, int z1) {
-    constexpr int NT = BX * BY;
-    constexpr int W = BX + 2;
-    constexpr int NH = W * (BY + 2);
+    using F = Frame<BX, BY>;
+    constexpr int W = F::W;
+    constexpr int NH = F::NH;
+    R *const wold = lds + 27 * NBUF * NH;
 
     const int x0 = (tile % a.tiles_x) * BX;
     const int y0 = (tile / a.tiles_x) * BY;
     const int tid = threadIdx.x;
     const int tx = tid % BX, ty = tid / BX;
     const uint32_t plane = uint32_t(a.nx) * uint32_t(a.ny);
+    const int h = (ty + 1) * W + (tx + 1);
 
-    const int p1 = tid + NT;
-    const bool has1 = p1 < NH;
-    uint32_t off0, off1 = 0;
-    {
-        const int hx = tid % W, hy = tid / W;
-        off0 = uint32_t(wrap(x0 - 1 + hx, a.nx)) + uint32_t(a.nx) * uint32_t(wrap(y0 - 1 + hy, a.ny));
-        if (has1) {
-            const int hx1 = p1 % W, hy1 = p1 / W;
-            off1 = uint32_t(wrap(x0 - 1 + hx1, a.nx)) + uint32_t(a.nx) * uint32_t(wrap(y0 - 1 + hy1, a.ny));
-        }
-    }
+    const bool has1 = tid < F::NR;
+    int hx1 = 0, hy1 = 0;
+    if (has1) F::ring(tid, hx1, hy1);
+    const int p1 = hy1 * W + hx1;
+    const uint32_t off0 = uint32_t(pmod(x0 + tx, a.nx)) + uint32_t(a.nx) * uint32_t(pmod(y0 + ty, a.ny));
+    const uint32_t off1 = has1 ? uint32_t(pmod(x0 - 1 + hx1, a.nx)) + uint32_t(a.nx) * uint32_t(pmod(y0 - 1 + hy1, a.ny))
+                               : off0;
+    const bool west1 = has1 && hx1 == 0 && hy1 >= 1 && hy1 <= BY;
+
     S r0[10], r1[10];  // stored values of the plane in flight, per position
     auto load_plane = [&](int z) {
-        const uint32_t base = uint32_t(wrap(z, a.nz)) * plane;
+        const uint32_t base = uint32_t(pmod(z, a.nz)) * plane;
         load_raw(a.src, base + off0, r0);
-        if (has1) load_raw(a.src, base + off1, r1);
+        load_raw(a.src, base + off1, r1);
     };
     // windows, per position: c_z=+1 groups of two planes (A, B), c_z=0 group of one plane
     R pa0[9], pb0[9], q0[9], pa1[9], pb1[9], q1[9];
-    auto recon = [&](const S *raw, R *f) {
+    auto recon = [&](const S *raw, R *f, bool west, int zp) {
         R m[10];
         decode_node<R, S>(raw, a.scale, a.rho0, m);
+        if (SPECIAL && west) {
+#pragma unroll
+            for (int k = 0; k < 10; ++k) wold[((zp & 1) * 10 + k) * BY + (hy1 - 1)] = m[k];
+        }
         reconstruct_lds(m, f);
     };
 
+    BoundaryPath<R, S, BX, BY> bp;
     // prologue: +1 group of plane z0-1 into A, +1 and 0 groups of plane z0 into B and q
     {
         R f[Q];
         load_plane(z0 - 1);
-        recon(r0, f);
+        recon(r0, f, false, z0 - 1);
 #pragma unroll
         for (int i = 0; i < 9; ++i) pa0[i] = f[GROUP_ZP[i]];
         if (has1) {
-            recon(r1, f);
+            recon(r1, f, false, z0 - 1);
 #pragma unroll
             for (int i = 0; i < 9; ++i) pa1[i] = f[GROUP_ZP[i]];
         }
         load_plane(z0);
-        recon(r0, f);
+        recon(r0, f, false, z0);
 #pragma unroll
         for (int i = 0; i < 9; ++i) {
             pb0[i] = f[GROUP_ZP[i]];
             q0[i] = f[GROUP_Z0[i]];
         }
         if (has1) {
-            recon(r1, f);
+            recon(r1, f, west1, z0);
 #pragma unroll
             for (int i = 0; i < 9; ++i) {
                 pb1[i] = f[GROUP_ZP[i]];
                 q1[i] = f[GROUP_Z0[i]];
             }
         }
+        if constexpr (SPECIAL) {
+            bp.copy_own(r0, bp.own_cur);
+            bp.load_info(a, off0 + uint32_t(z0) * plane, bp.info_cur);
+            bp.idxp = bp.didxp = a.pad;
+        }
         load_plane(z0 + 1);
     }
 
     const int x = x0 + tx, y = y0 + ty;
     const bool inside = x < a.nx && y < a.ny;
-    const int h = (ty + 1) * W + (tx + 1);
+    const bool east_tile = x0 + BX >= a.nx;
     uint32_t seed = 0;
     if constexpr (Conv<S>::scaled) {
         if (a.step_counter) seed = uint32_t(*a.step_counter) * 2u + 1u;
@@ -336,9 +442,9 @@ __device__ __forceinline__ void window_march(const FusedArgs<R, S> &a, R *lds, i
     uint32_t idxp = a.pad;
 
     // one position: reconstruct plane z+1, write the pulled plane's groups, advance the windows
-    auto put = [&](const S *raw, int p, R *G, R *pold, R *q) {
+    auto put = [&](const S *raw, int p, R *G, R *pold, R *q, bool west, int zp) {
         R f[Q];
-        recon(raw, f);
+        recon(raw, f, west, zp);
 #pragma unroll
         for (int i = 0; i < 9; ++i) {
             G[GROUP_ZP[i] * NH + p] = pold[i];  // plane z-1
@@ -351,40 +457,49 @@ __device__ __forceinline__ void window_march(const FusedArgs<R, S> &a, R *lds, i
             q[i] = f[GROUP_Z0[i]];
         }
     };
-    auto iteration = [&](int z, R *G, R *w0, R *w1, bool store_prev) {
+    auto iteration = [&](int z, R *G, R *w0, R *w1) {
         if (NBUF == 1 && z > z0) lds_barrier();  // previous plane's pulls are done
-        put(r0, tid, G, w0, q0);
-        if (has1) put(r1, p1, G, w1, q1);
-        if constexpr (!SPECIAL) {
-            if (store_prev) store_node<R, S>(a.dst, idxp, a.scale, a.rho0, Mp, seed);
-            load_plane(z + 2);  // wraps past the item
+        if constexpr (SPECIAL) load_tabs(a, bp.info_cur, uint32_t(y) + uint32_t(a.ny) * uint32_t(z), bp.tab);
+        put(r0, h, G, w0, q0, false, z + 1);
+        if (has1) put(r1, p1, G, w1, q1, west1, z + 1);
+        if constexpr (SPECIAL) {
+            bp.copy_own(r0, bp.own_nx);
+            bp.store_prev(a, seed);
+            bp.load_info(a, off0 + uint32_t(pmod(z + 1, a.nz)) * plane, bp.info_nx);
         } else {
-            if (z + 1 < z1) load_plane(z + 2);
+            store_node<R, S>(a.dst, idxp, a.scale, a.rho0, Mp, seed);
         }
+        load_plane(z + 2);  // wraps past the item
         lds_barrier();
         R f[Q];
 #pragma unroll
         for (int q = 0; q < Q; ++q) f[q] = G[q * NH + h - LATTICE.cx[q] - LATTICE.cy[q] * W];
+        const uint32_t node = off0 + uint32_t(z) * plane;
         if constexpr (SPECIAL) {
-            if (inside) update_node(a, f, x, y, z, plane, true);
+            bp.update(a, f, tx, ty, inside, east_tile, node, wold + (z & 1) * 10 * BY);
+            bp.copy_own(bp.own_nx, bp.own_cur);
+            bp.info_cur = bp.info_nx;
         } else {
             moments_lds(f, Mp);
             collide(a.omega, Mp);
-            idxp = inside ? uint32_t(x) + uint32_t(a.nx) * uint32_t(y) + uint32_t(z) * plane : a.pad;
+            idxp = inside ? node : a.pad;
         }
     };
     R *const G0 = lds;
     R *const G1 = NBUF == 2 ? lds + 27 * NH : lds;
     int z = z0;
-    iteration(z, G0, pa0, pa1, false);
+    iteration(z, G0, pa0, pa1);
     ++z;
     while (z < z1) {
-        iteration(z, G1, pb0, pb1, true);
+        iteration(z, G1, pb0, pb1);
         if (++z >= z1) break;
-        iteration(z, G0, pa0, pa1, true);
+        iteration(z, G0, pa0, pa1);
         ++z;
     }
-    if constexpr (!SPECIAL) store_node<R, S>(a.dst, idxp, a.scale, a.rho0, Mp, seed);
+    if constexpr (SPECIAL)
+        bp.store_prev(a, seed);
+    else
+        store_node<R, S>(a.dst, idxp, a.scale, a.rho0, Mp, seed);
     if (NBUF == 1) lds_barrier();  // the block's LDS is free again for the next item
 }
 
@@ -457,49 +572,42 @@ __global__ __launch_bounds__(BX *BY, (VARIANT == 3 ? 3 : 2)) void fused_batch_ke
     }
 }
 
-template <class R, class S>
-__device__ __forceinline__ R bilinear(const S *__restrict__ field, uint32_t base, uint32_t pitch, R xd, R yd) {
-    const R q00 = R(Conv<S>::to(ld_field(field, base)));
-    const R q10 = R(Conv<S>::to(ld_field(field, base + 1)));
-    const R q01 = R(Conv<S>::to(ld_field(field, base + pitch)));
-    const R q11 = R(Conv<S>::to(ld_field(field, base + pitch + 1)));
+// Bilinear blend of four corner values (bilinear_interpolation, extrapolation_utils.cuh:5-47).
+template <class R>
+__device__ __forceinline__ R bilinear(R q00, R q10, R q01, R q11, R xd, R yd) {
     const R q0 = (R(1) - xd) * q00 + xd * q10;
     const R q1 = (R(1) - xd) * q01 + xd * q11;
     return (R(1) - yd) * q0 + yd * q1;
 }
 
 // Interpolate u and m at one stencil point and rotate into the wall frame
-// (initerpolate_and_rotate + rotate_velocity_moments, extrapolation_utils.cuh:63-107).
+// (initerpolate_and_rotate + rotate_velocity_moments, extrapolation_utils.cuh:63-107). The
+// donor snapshot is node-major: each corner's nine values come from one cache line or two.
 template <class R, class S>
 __device__ __forceinline__ void interp_rotate(const WallArgs<R, S> &a, uint32_t base, R xd, R yd, R c, R s, R *vm) {
-    R v[9];
-    if constexpr (!Conv<S>::scaled) {
+    const uint32_t corner[4] = {base, base + 1, base + uint32_t(a.nx), base + uint32_t(a.nx) + 1};
+    R q[4][9];
 #pragma unroll
-        for (int k = 0; k < 9; ++k)
-            v[k] = bilinear<R, S>(a.donor.f[k + 1], base, uint32_t(a.nx), xd, yd);
-    } else {
-        // stored second moments are non-equilibrium parts: m = (m - u u) + u u is restored at each
-        // corner before the corners are blended
-        const uint32_t corner[4] = {base, base + 1, base + uint32_t(a.nx), base + uint32_t(a.nx) + 1};
-        const R wgt[4] = {(R(1) - xd) * (R(1) - yd), xd * (R(1) - yd), (R(1) - xd) * yd, xd * yd};
+    for (int j = 0; j < 4; ++j) {
 #pragma unroll
-        for (int k = 0; k < 9; ++k) v[k] = R(0);
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            R q[9];
-#pragma unroll
-            for (int k = 0; k < 9; ++k)
-                q[k] = R(Conv<S>::to(ld_field(a.donor.f[k + 1], corner[j]))) * a.scale.dec[field_class(k + 1)];
-            q[3] += q[0] * q[0];
-            q[4] += q[1] * q[1];
-            q[5] += q[2] * q[2];
-            q[6] += q[0] * q[1];
-            q[7] += q[0] * q[2];
-            q[8] += q[1] * q[2];
-#pragma unroll
-            for (int k = 0; k < 9; ++k) v[k] += wgt[j] * q[k];
+        for (int k = 0; k < 9; ++k) {
+            q[j][k] = R(Conv<S>::to(a.donor[size_t(corner[j]) * 10 + 1 + k]));
+            if constexpr (Conv<S>::scaled) q[j][k] *= a.scale.dec[field_class(k + 1)];
+        }
+        if constexpr (Conv<S>::scaled) {
+            // stored second moments are non-equilibrium parts: m = (m - u u) + u u is restored at
+            // each corner before the corners are blended
+            q[j][3] += q[j][0] * q[j][0];
+            q[j][4] += q[j][1] * q[j][1];
+            q[j][5] += q[j][2] * q[j][2];
+            q[j][6] += q[j][0] * q[j][1];
+            q[j][7] += q[j][0] * q[j][2];
+            q[j][8] += q[j][1] * q[j][2];
         }
     }
+    R v[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) v[k] = bilinear(q[0][k], q[1][k], q[2][k], q[3][k], xd, yd);
     const R ux = v[0], uy = v[1], uz = v[2], mxx = v[3], myy = v[4], mzz = v[5], mxy = v[6], mxz = v[7], myz = v[8];
     const R s2 = R(2) * s * c, c2 = c * c - s * s;
     vm[0] = ux * c + uy * s;
@@ -514,6 +622,25 @@ __device__ __forceinline__ void interp_rotate(const WallArgs<R, S> &a, uint32_t 
 }
 
 
+// Sum of v over the block, the same fixed tree on every call (deterministic).
+__device__ __forceinline__ double wall_block_sum(double v) {
+    constexpr int NW = WALL_THREADS / 64;
+    __shared__ double red[NW];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if constexpr (NW == 1) {
+        return __shfl(v, 0, 64);
+    } else {
+        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+        __syncthreads();
+        if (lane == 0) red[wave] = v;
+        __syncthreads();
+        double t = 0.0;
+        for (int w = 0; w < NW; ++w) t += red[w];
+        return t;
+    }
+}
+
 // One block of the wall pass of one case: block `lb` of the case's a.nblocks blocks.
 template <class R, class S>
 __device__ __forceinline__ void wall_body(const WallArgs<R, S> &a, int lb) {
@@ -525,14 +652,15 @@ __device__ __forceinline__ void wall_body(const WallArgs<R, S> &a, int lb) {
         load_node<R, S>(as_const(a.mom), idx, a.scale, a.rho0, M);  // rho_I, u_old, m'_I
         const R c = a.wnx[i], s = a.wny[i];
         R vm1[9], vm2[9];
-        interp_rotate(a, a.st_base[2 * i], a.st_frac[4 * i], a.st_frac[4 * i + 1], c, s, vm1);
-        interp_rotate(a, a.st_base[2 * i + 1], a.st_frac[4 * i + 2], a.st_frac[4 * i + 3], c, s, vm2);
+        const int nw = a.nwall;  // per-wall records are component-major
+        interp_rotate(a, a.st_base[i], a.st_frac[i], a.st_frac[nw + i], c, s, vm1);
+        interp_rotate(a, a.st_base[nw + i], a.st_frac[2 * nw + i], a.st_frac[3 * nw + i], c, s, vm2);
         const int g = i < a.nb_inner ? 0 : 1;
         const R u0 = a.uw[g][0], u1 = a.uw[g][1], u2 = a.uw[g][2];
         // wall values u'_w and m'_w = u'_w u'_w (curvedLBM.cu:93-101)
         const R vw[7] = {u0, u1, u2, u0 * u0, u1 * u1, u2 * u2, u1 * u2};
         constexpr int sel[7] = {0, 1, 2, 3, 4, 5, 8};
-        const R c0 = a.coef[3 * i], c1 = a.coef[3 * i + 1], c2 = a.coef[3 * i + 2];
+        const R c0 = a.coef[i], c1 = a.coef[nw + i], c2 = a.coef[2 * nw + i];
         R p[7];
 #pragma unroll
         for (int j = 0; j < 7; ++j) p[j] = c0 * vw[j] + c1 * vm1[sel[j]] + c2 * vm2[sel[j]];
@@ -542,7 +670,7 @@ __device__ __forceinline__ void wall_body(const WallArgs<R, S> &a, int lb) {
         } else {
             R k[WALL_CONSTS];
 #pragma unroll
-            for (int j = 0; j < WALL_CONSTS; ++j) k[j] = a.wconst[size_t(i) * WALL_CONSTS + j];
+            for (int j = 0; j < WALL_CONSTS; ++j) k[j] = a.wconst[size_t(j) * nw + i];
             wall_closure(c, s, k, p, M[0], M[7], M[8], out);
         }
         if (a.forces) masked_force(out, a.in_mask[i], 1.0, F);
@@ -556,77 +684,65 @@ __device__ __forceinline__ void wall_body(const WallArgs<R, S> &a, int lb) {
         const uint32_t idx = a.bcf_node[k];
         const uint32_t t = a.bcf_tag[k];
         R M[10];
-        load_node<R, S>(a.donor, idx, a.scale, a.rho0, M);
+        load_node_nm<R, S>(a.donor, idx, a.scale, a.rho0, M);
         masked_force(M, a.bcf_in[t], 1.0, F);
         collide(a.omega, M);
         masked_force(M, a.bcf_out[t], -1.0, F);
     }
-    // block reduction in FP64: wave shuffles, then across the waves through LDS. The last block
-    // to finish (atomic ticket) sums the block partials: each thread adds the partials of blocks
-    // t, t + WALL_THREADS, ... in order, then a fixed shuffle tree combines the threads, so the
-    // per-step record is deterministic. It is filed under the device step counter, which the
-    // last block then advances.
-    constexpr int NW = WALL_THREADS / 64;
-    __shared__ double red[4][NW];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    auto wave_sum = [](double v) {
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-        return v;
-    };
+    // block partial of the force / mass records in FP64 (wave shuffles, then across the waves)
     if (a.forces) {
-#pragma unroll
-        for (int k = 0; k < 4; ++k) F[k] = wave_sum(F[k]);
-        if (lane == 0)
-            for (int k = 0; k < 4; ++k) red[k][wave] = F[k];
-    }
-    __syncthreads();
-    __shared__ bool last;
-    if (threadIdx.x == 0) {
-        if (a.forces) {
-            for (int k = 0; k < 4; ++k) {
-                double tot = 0.0;
-                for (int w = 0; w < NW; ++w) tot += red[k][w];
-                a.partials[4 * lb + k] = tot;
-            }
-        }
-        __threadfence();
-        const unsigned ticket = atomicAdd(a.ticket, 1u);
-        last = ticket == unsigned(a.nblocks) - 1;
-    }
-    __syncthreads();
-    if (!last) return;
-    __threadfence();
-    const unsigned long long step = __hip_atomic_load(a.step_counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (a.forces) {
-        double tot[4] = {0.0, 0.0, 0.0, 0.0};
-        for (unsigned blk = threadIdx.x; blk < unsigned(a.nblocks); blk += WALL_THREADS)
-            for (int k = 0; k < 4; ++k)
-                tot[k] += __hip_atomic_load(&a.partials[4 * blk + k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-#pragma unroll
-        for (int k = 0; k < 4; ++k) tot[k] = wave_sum(tot[k]);
-        __syncthreads();
-        if (lane == 0)
-            for (int k = 0; k < 4; ++k) red[k][wave] = tot[k];
-        __syncthreads();
+        const double p0 = wall_block_sum(F[0]), p1 = wall_block_sum(F[1]), p2 = wall_block_sum(F[2]),
+                     p3 = wall_block_sum(F[3]);
         if (threadIdx.x == 0) {
-            const size_t slot = size_t(step % FORCE_RING_STEPS);
-            for (int k = 0; k < 4; ++k) {
-                double t = 0.0;
-                for (int w = 0; w < NW; ++w) t += red[k][w];
-                a.ring[4 * slot + k] = t;
-            }
+            a.partials[4 * lb + 0] = p0;
+            a.partials[4 * lb + 1] = p1;
+            a.partials[4 * lb + 2] = p2;
+            a.partials[4 * lb + 3] = p3;
         }
-    }
-    if (threadIdx.x == 0) {
-        *a.step_counter = step + 1;
-        *a.ticket = 0u;  // ready for the next launch
     }
 }
 
+// Per-step record of one case: the block partials of the wall pass summed in a fixed order
+// (thread t adds blocks t, t + WALL_THREADS, ..., then the block tree), filed under the device
+// step counter, which is then advanced. Runs after the wall pass, so plain loads see its
+// partials.
+template <class R, class S>
+__device__ __forceinline__ void wall_finish_body(const WallArgs<R, S> &a) {
+    const unsigned long long step = *a.step_counter;
+    if (a.forces) {
+        double tot[4] = {0.0, 0.0, 0.0, 0.0};
+        for (int blk = int(threadIdx.x); blk < a.nblocks; blk += WALL_THREADS)
+            for (int k = 0; k < 4; ++k) tot[k] += a.partials[4 * blk + k];
+        const double s0 = wall_block_sum(tot[0]), s1 = wall_block_sum(tot[1]), s2 = wall_block_sum(tot[2]),
+                     s3 = wall_block_sum(tot[3]);
+        if (threadIdx.x == 0) {
+            const size_t slot = size_t(step % FORCE_RING_STEPS);
+            a.ring[4 * slot + 0] = s0;
+            a.ring[4 * slot + 1] = s1;
+            a.ring[4 * slot + 2] = s2;
+            a.ring[4 * slot + 3] = s3;
+        }
+    }
+    if (threadIdx.x == 0) *a.step_counter = step + 1;
+}
+
+
 template <class R, class S>
 __global__ __launch_bounds__(WALL_THREADS) void wall_step_kernel(WallArgs<R, S> a) {
-    wall_body(a, int(blockIdx.x));
+    wall_body(a, xcd_contiguous(int(blockIdx.x), int(gridDim.x)));
+}
+
+template <class R, class S>
+__global__ __launch_bounds__(WALL_THREADS) void wall_finish_kernel(WallArgs<R, S> a) {
+    wall_finish_body(a);
+}
+
+// Per-step records of a batch: block c finishes case cases_of[c].
+template <class R, class S>
+__global__ __launch_bounds__(WALL_THREADS) void wall_finish_batch_kernel(const WallArgs<R, S> *__restrict__ cases,
+                                                                         const int *__restrict__ cases_of) {
+    const WallArgs<R, S> a = cases[cases_of[blockIdx.x]];
+    wall_finish_body(a);
 }
 
 // Wall pass of a batch: block b runs block blocks[b].y of case blocks[b].x.
@@ -750,14 +866,18 @@ void launch_fused_batch(const FusedArgs<R, S> *cases, const uint4 *items, int ni
 }
 
 template <class R, class S>
-void launch_wall_batch(const WallArgs<R, S> *cases, const int2 *blocks, int nblocks, hipStream_t stream) {
+void launch_wall_batch(const WallArgs<R, S> *cases, const int2 *blocks, int nblocks, const int *cases_of,
+                       int ncases, hipStream_t stream) {
     hipLaunchKernelGGL((wall_batch_kernel<R, S>), dim3(nblocks), dim3(WALL_THREADS), 0, stream, cases, blocks);
+    hipLaunchKernelGGL((wall_finish_batch_kernel<R, S>), dim3(ncases), dim3(WALL_THREADS), 0, stream, cases,
+                       cases_of);
 }
 
 template <class R, class S>
 void launch_wall(const WallArgs<R, S> &a, hipStream_t stream) {
     const int blocks = wall_blocks(a.nwall, a.forces ? a.nbcf : 0);
     hipLaunchKernelGGL((wall_step_kernel<R, S>), dim3(blocks), dim3(WALL_THREADS), 0, stream, a);
+    hipLaunchKernelGGL((wall_finish_kernel<R, S>), dim3(1), dim3(WALL_THREADS), 0, stream, a);
 }
 
 template <class R, class S>
@@ -778,7 +898,8 @@ void launch_decode(const FieldSet<const S> &in, double *out, uint32_t n, const S
     template void launch_fused_batch<R, S>(const FusedArgs<R, S> *, const uint4 *, int, int, int, int, int,     \
                                            hipStream_t);                                                    \
     template int prepare_fused_batch<R, S>(int, int, int);                                                  \
-    template void launch_wall_batch<R, S>(const WallArgs<R, S> *, const int2 *, int, hipStream_t);          \
+    template void launch_wall_batch<R, S>(const WallArgs<R, S> *, const int2 *, int, const int *, int,        \
+                                          hipStream_t);                                                     \
     template void launch_encode<R, S>(const double *, const FieldSet<S> &, uint32_t, const StoreScale<R> &,        \
                                       hipStream_t);                                                         \
     template void launch_decode<R, S>(const FieldSet<const S> &, double *, uint32_t, const StoreScale<R> &,  \

@@ -170,13 +170,20 @@ class Sim final : public SimBase {
         nb_inner_ = int(g.wall[0].node.size());
         nbcf_ = int(bcf_node.size());
 
-        // columns (x, y) that hold a boundary or donor node at some z: they make a tile special
+        // columns (x, y) that hold a boundary or donor node at some z make a tile special;
+        // columns frozen at every z (inactive tiles of the reference) are never updated: both
+        // buffers hold their values, and tiles made only of them get no work items
         {
-            std::vector<uint8_t> col(size_t(g.nx) * g.ny, 0);
-            for (int64_t i = 0; i < g.n; ++i)
-                if (info[i] != NC_BULK) col[size_t(i % (int64_t(g.nx) * g.ny))] = 1;
+            const size_t ncol = size_t(g.nx) * g.ny;
+            std::vector<uint8_t> col(ncol, 0);
+            frozen_cols_.assign(ncol, 1);
+            for (int64_t i = 0; i < g.n; ++i) {
+                const size_t c = size_t(i % int64_t(ncol));
+                if (info[i] != NC_BULK) col[c] = 1;
+                if ((info[i] & INFO_CODE_MASK) != NC_FROZEN) frozen_cols_[c] = 0;
+            }
             special_cols_.clear();
-            for (size_t c = 0; c < col.size(); ++c)
+            for (size_t c = 0; c < ncol; ++c)
                 if (col[c]) special_cols_.push_back(uint32_t(c));
         }
 
@@ -189,6 +196,19 @@ class Sim final : public SimBase {
         }
         std::vector<R> inlet(std::max<size_t>(g.inlet.size(), 1), R(0));
         for (size_t k = 0; k < g.inlet.size(); ++k) inlet[k] = R(g.inlet[k]);
+
+        // per-wall records go to the device component-major (entry j of wall i at j * nwall + i) so
+        // that the wall pass reads them with coalesced loads
+        auto soa = [nwall](const auto &v, int per) {
+            std::decay_t<decltype(v)> out(v.size());
+            for (int i = 0; i < nwall; ++i)
+                for (int j = 0; j < per; ++j) out[size_t(j) * nwall + i] = v[size_t(i) * per + j];
+            return out;
+        };
+        w_const = soa(w_const, WALL_CONSTS);
+        w_coef = soa(w_coef, 3);
+        st_frac = soa(st_frac, 4);
+        st_base = soa(st_base, 2);
 
         int rc;
         if ((rc = upload(d_info_, info))) return rc;
@@ -209,8 +229,6 @@ class Sim final : public SimBase {
         if ((rc = upload(d_inlet_, inlet))) return rc;
         // the wall pass launches at most wall_blocks(nwall, nbcf) blocks (forces on)
         HIP_TRY(hipMalloc(&d_partials_, size_t(wall_blocks(nwall, nbcf_)) * 4 * sizeof(double)));
-        HIP_TRY(hipMalloc(&d_ticket_, sizeof(unsigned)));
-        HIP_TRY(hipMemsetAsync(d_ticket_, 0, sizeof(unsigned), stream_));
         HIP_TRY(hipMalloc(&d_ring_, size_t(FORCE_RING_STEPS) * 4 * sizeof(double)));
         HIP_TRY(hipMalloc(&d_step_counter_, sizeof(unsigned long long)));
         HIP_TRY(hipMemsetAsync(d_step_counter_, 0, sizeof(unsigned long long), stream_));
@@ -220,13 +238,16 @@ class Sim final : public SimBase {
     }
 
     // Fused-pass defaults: 16x8 tiles in FP64 and 32x8 in FP32 (LDS-bound), the ring march
-    // except for 16-bit storage (register windows), z chunks of up to 32 planes.
+    // except for 16-bit storage (register windows).
     LaunchConfig default_launch() const {
         LaunchConfig lc;
         lc.bx = sizeof(R) == 8 ? 16 : 32;
         lc.by = 8;
         lc.variant = Conv<S>::scaled ? 3 : 1;
-        lc.lz = 32;
+        // z chunks short enough for about 1024 work items (a block's plane-by-plane march is
+        // latency-bound, so small grids need many short chunks), at most 32 planes
+        const int64_t tiles = int64_t((g.nx + lc.bx - 1) / lc.bx) * ((g.ny + lc.by - 1) / lc.by);
+        lc.lz = int(std::max<int64_t>(1, std::min<int64_t>(32, tiles * g.nz / 1024)));
         lc.lz_special = 0;  // same as lz
         lc.xcd_remap = true;
         return lc;
@@ -247,11 +268,16 @@ class Sim final : public SimBase {
         const int tiles_x = (g.nx + lc.bx - 1) / lc.bx;
         const int tiles_y = (g.ny + lc.by - 1) / lc.by;
         const int ntiles = tiles_x * tiles_y;
-        std::vector<uint8_t> tile_special(ntiles, 0);
+        std::vector<uint8_t> tile_special(ntiles, 0), tile_live(ntiles, 0);
         for (uint32_t c : special_cols_) {
             const int x = int(c % uint32_t(g.nx)), y = int(c / uint32_t(g.nx));
             tile_special[x / lc.bx + tiles_x * (y / lc.by)] = 1;
         }
+        for (size_t c = 0; c < frozen_cols_.size(); ++c)
+            if (!frozen_cols_[c]) {
+                const int x = int(c % size_t(g.nx)), y = int(c / size_t(g.nx));
+                tile_live[x / lc.bx + tiles_x * (y / lc.by)] = 1;
+            }
         const int groups = (lc.xcd_remap && g.nz >= 32) ? 8 : 1;
         const int slab = (g.nz + groups - 1) / groups;
         const int lz = std::max(1, std::min(slab, lc.lz));
@@ -262,7 +288,7 @@ class Sim final : public SimBase {
             for (int pass = 0; pass < 2; ++pass)
                 for (int t = 0; t < ntiles; ++t) {
                     const bool sp = tile_special[t];
-                    if (sp != (pass == 0)) continue;
+                    if (sp != (pass == 0) || !tile_live[t]) continue;
                     const int step = sp ? lzs : lz;
                     for (int z = za; z < zb; z += step) {
                         const int z1 = std::min(zb, z + step);
@@ -313,7 +339,7 @@ class Sim final : public SimBase {
         FusedArgs<R, S> a{};
         a.src = as_const(make_fields(buf_[cur], stride_));
         a.dst = make_fields(buf_[cur ^ 1], stride_);
-        a.donor = make_fields(donor_, stride_);
+        a.donor = donor_;
         a.pad = uint32_t(g.n);
         a.nx = g.nx;
         a.ny = g.ny;
@@ -328,6 +354,8 @@ class Sim final : public SimBase {
         a.wall_in = d_win_;
         a.bcf_in = d_bcfin_;
         a.inlet = d_inlet_;
+        a.wall_count = std::max(nwall_, 1);
+        a.inlet_count = std::max<int>(int(g.inlet.size()), 1);
         a.step_counter = stochastic_ ? d_step_counter_ : nullptr;
         a.scale = scale_;
         a.omega = omega_;
@@ -339,7 +367,7 @@ class Sim final : public SimBase {
     WallArgs<R, S> wall_args(int cur, bool forces) const {
         WallArgs<R, S> w{};
         w.mom = make_fields(buf_[cur ^ 1], stride_);
-        w.donor = as_const(make_fields(donor_, stride_));
+        w.donor = donor_;
         w.nwall = nwall_;
         w.nb_inner = nb_inner_;
         w.nbcf = nbcf_;
@@ -365,7 +393,6 @@ class Sim final : public SimBase {
         w.forces = forces ? 1 : 0;
         w.nblocks = wall_blocks(nwall_, forces ? nbcf_ : 0);
         w.partials = d_partials_;
-        w.ticket = d_ticket_;
         w.ring = d_ring_;
         w.step_counter = d_step_counter_;
         return w;
@@ -454,8 +481,9 @@ class Sim final : public SimBase {
 
     int set_raw(const void *h) override {
         HIP_TRY(hipStreamSynchronize(stream_));
-        HIP_TRY(hipMemcpy2D(buf_[cur_], stride_ * sizeof(S), h, size_t(g.n) * sizeof(S), size_t(g.n) * sizeof(S), 10,
-                            hipMemcpyHostToDevice));
+        for (int b = 0; b < 2; ++b)  // both buffers (frozen nodes)
+            HIP_TRY(hipMemcpy2D(buf_[b], stride_ * sizeof(S), h, size_t(g.n) * sizeof(S), size_t(g.n) * sizeof(S), 10,
+                                hipMemcpyHostToDevice));
         return MRLBM_OK;
     }
 
@@ -575,7 +603,9 @@ class Sim final : public SimBase {
         double *tmp = nullptr;
         HIP_TRY(hipMalloc(&tmp, h.size() * sizeof(double)));
         HIP_TRY(hipMemcpy(tmp, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+        // both buffers: frozen nodes are never rewritten and must read the same from either
         launch_encode<R, S>(tmp, make_fields(buf_[cur_], stride_), uint32_t(g.n), scale_, stream_);
+        launch_encode<R, S>(tmp, make_fields(buf_[cur_ ^ 1], stride_), uint32_t(g.n), scale_, stream_);
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipStreamSynchronize(stream_));
         HIP_TRY(hipFree(tmp));
@@ -601,7 +631,7 @@ class Sim final : public SimBase {
             if (g) (void)hipGraphExecDestroy(g);
         void *ptrs[] = {buf_[0], buf_[1], donor_, d_info_, d_items_, d_wconst_, d_wnx_, d_wny_, d_win_, d_wout_, d_wnode_,
                         d_wcoef_, d_stbase_, d_stfrac_, d_bcfnode_, d_bcftag_, d_bcfin_, d_bcfout_, d_inlet_,
-                        d_partials_, d_ticket_, d_ring_, d_step_counter_};
+                        d_partials_, d_ring_, d_step_counter_};
         for (void *p : ptrs)
             if (p) (void)hipFree(p);
         if (stream_) (void)hipStreamDestroy(stream_);
@@ -622,6 +652,7 @@ class Sim final : public SimBase {
     hipGraphExec_t graphs_[4] = {nullptr, nullptr, nullptr, nullptr};
     int bx_ = 32, by_ = 8, lz_ = 16, lz_special_ = 16, tiles_x_ = 1, tiles_y_ = 1, ntiles_ = 1, nblocks_ = 1, xcd_ = 1;
     std::vector<uint32_t> special_cols_;  // x + nx y of columns holding non-bulk nodes
+    std::vector<uint8_t> frozen_cols_;    // per column: frozen at every z
     std::vector<std::vector<uint2>> lists_;
     LaunchConfig launch_;
     int nwall_ = 0, nb_inner_ = 0, nbcf_ = 0;
@@ -634,7 +665,6 @@ class Sim final : public SimBase {
     uint8_t *d_bcftag_ = nullptr;
     double *d_partials_ = nullptr, *d_ring_ = nullptr;
     unsigned long long *d_step_counter_ = nullptr;
-    unsigned *d_ticket_ = nullptr;
 };
 
 }  // namespace mrlbm

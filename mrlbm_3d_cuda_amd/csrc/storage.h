@@ -152,15 +152,15 @@ __device__ __forceinline__ void load_node(const FieldSet<const S> &fs, uint32_t 
     decode_node<R, S>(raw, sc, rho0, M);
 }
 
-// Store the moments of node `idx`. For the 16-bit forms `seed` selects stochastic rounding
-// (seed != 0) or round-to-nearest (seed == 0).
+// The 10 stored values of the moments M of node `idx`. For the 16-bit forms `seed` selects
+// stochastic rounding (seed != 0) or round-to-nearest (seed == 0).
 template <class R, class S>
-__device__ __forceinline__ void store_node(const FieldSet<S> &fs, uint32_t idx, const StoreScale<R> &sc, R rho0,
-                                           const R *M, uint32_t seed = 0) {
+__device__ __forceinline__ void encode_node(uint32_t idx, const StoreScale<R> &sc, R rho0, const R *M, uint32_t seed,
+                                            S *out) {
     if constexpr (!Conv<S>::scaled) {
-        st_field(fs.f[0], idx, Conv<S>::from(M[0] - rho0));
+        out[0] = Conv<S>::from(M[0] - rho0);
 #pragma unroll
-        for (int f = 1; f < 10; ++f) st_field(fs.f[f], idx, Conv<S>::from(M[f]));
+        for (int f = 1; f < 10; ++f) out[f] = Conv<S>::from(M[f]);
     } else {
         R v[10];
         v[0] = M[0] - rho0;
@@ -175,14 +175,43 @@ __device__ __forceinline__ void store_node(const FieldSet<S> &fs, uint32_t idx, 
         v[9] = M[9] - M[2] * M[3];
         if (seed == 0) {
 #pragma unroll
-            for (int f = 0; f < 10; ++f) st_field(fs.f[f], idx, Conv<S>::from(v[f] * sc.enc[field_class(f)]));
+            for (int f = 0; f < 10; ++f) out[f] = Conv<S>::from(v[f] * sc.enc[field_class(f)]);
         } else {
             const uint32_t h = sr_hash(idx, seed);
 #pragma unroll
-            for (int f = 0; f < 10; ++f)
-                st_field(fs.f[f], idx, Conv<S>::from_sr(v[f] * sc.enc[field_class(f)], sr_bits(h, f)));
+            for (int f = 0; f < 10; ++f) out[f] = Conv<S>::from_sr(v[f] * sc.enc[field_class(f)], sr_bits(h, f));
         }
     }
+}
+
+// Store the moments of node `idx` into a field-major buffer.
+template <class R, class S>
+__device__ __forceinline__ void store_node(const FieldSet<S> &fs, uint32_t idx, const StoreScale<R> &sc, R rho0,
+                                           const R *M, uint32_t seed = 0) {
+    S v[10];
+    encode_node<R, S>(idx, sc, rho0, M, seed, v);
+#pragma unroll
+    for (int f = 0; f < 10; ++f) st_field(fs.f[f], idx, v[f]);
+}
+
+// Node-major buffers (the 10 values of node i at base[10 i .. 10 i + 9]): the donor snapshot,
+// read at scattered stencil corners, where one node's values then share a cache line.
+template <class R, class S>
+__device__ __forceinline__ void store_node_nm(S *__restrict__ base, uint32_t idx, const StoreScale<R> &sc, R rho0,
+                                              const R *M, uint32_t seed = 0) {
+    S v[10];
+    encode_node<R, S>(idx, sc, rho0, M, seed, v);
+#pragma unroll
+    for (int f = 0; f < 10; ++f) base[size_t(idx) * 10 + f] = v[f];
+}
+
+template <class R, class S>
+__device__ __forceinline__ void load_node_nm(const S *__restrict__ base, uint32_t idx, const StoreScale<R> &sc, R rho0,
+                                             R *M) {
+    S raw[10];
+#pragma unroll
+    for (int f = 0; f < 10; ++f) raw[f] = base[size_t(idx) * 10 + f];
+    decode_node<R, S>(raw, sc, rho0, M);
 }
 
 }  // namespace mrlbm
