@@ -103,6 +103,7 @@ enum mrlbm_field {
 
 /* mrlbm_step flags */
 #define MRLBM_STEP_FORCES 0x1u /* accumulate per-step Fx, Fy, Fz, mass flux (K5 window) */
+#define MRLBM_STEP_STATS 0x2u  /* add this step's inner-wall pressure to the FP64 statistics */
 
 /*
  * Case descriptor. Lengths are lattice units; x is the fastest index of every host array
@@ -211,6 +212,19 @@ int mrlbm_checkpoint_write(mrlbm_sim *sim, const char *path);
 int mrlbm_checkpoint_read(mrlbm_sim *sim, const char *path);
 
 int mrlbm_destroy(mrlbm_sim *sim);
+
+/*
+ * Time-averaged wall pressure of the inner wall (the cylinder, or the annulus's inner wall) <-
+ * compute_surface_pressure + surface_pressure_extrapolation (K8,
+ * src/cases/cylinder/cylinder_kernels.cu:238-283, src/boundary/extrapolation_utils.cuh:109-154)
+ * and write_pressure (src/cases/cylinder/cylinder_kernels.cuh:251-285). Every step queued with
+ * MRLBM_STEP_STATS adds each inner-wall node's extrapolated wall pressure to an FP64 sum;
+ * mrlbm_get_surface_pressure returns the means (nb_inner values, wall-table order) and the
+ * number of steps averaged; mrlbm_stats_reset starts a new window. The statistics are not part
+ * of checkpoints.
+ */
+int mrlbm_stats_reset(mrlbm_sim *sim);
+int mrlbm_get_surface_pressure(mrlbm_sim *sim, double *ps_mean, int64_t *samples);
 
 /*
  * Batches: many cases of one precision stepped together -- one fused launch over the work items

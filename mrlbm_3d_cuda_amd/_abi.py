@@ -42,6 +42,7 @@ NUM_FIELDS = 10
 
 COMPAT_INIT_M_ZERO = 0x1
 STEP_FORCES = 0x1
+STEP_STATS = 0x2
 
 # reference node types (common/nodeTypeMap.cuh:38-80)
 NODE_TYPES = (
@@ -151,6 +152,8 @@ PRODUCT_PROTOTYPES = {
     "mrlbm_checkpoint_write": (C.c_int, [C.c_void_p, C.c_char_p]),
     "mrlbm_checkpoint_read": (C.c_int, [C.c_void_p, C.c_char_p]),
     "mrlbm_destroy": (C.c_int, [C.c_void_p]),
+    "mrlbm_stats_reset": (C.c_int, [C.c_void_p]),
+    "mrlbm_get_surface_pressure": (C.c_int, [C.c_void_p, _dp, P(C.c_int64)]),
     "mrlbm_last_error": (C.c_char_p, []),
     "mrlbm_abi_version": (C.c_int32, []),
 }

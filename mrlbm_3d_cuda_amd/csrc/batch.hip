@@ -132,13 +132,13 @@ class Batch final : public BatchBase {
                 hipGraphExec_t exec = nullptr;
                 if (int rc = graph_for(flags, exec)) return rc;
                 HIP_TRY(hipGraphLaunch(exec, stream_));
-                for (int s = 0; s < graph_steps_; ++s) advance();
+                for (int s = 0; s < graph_steps_; ++s) advance(flags);
                 done += graph_steps_;
             }
         }
         for (; done < nsteps; ++done) {
             queue_step(flags, cur_);
-            advance();
+            advance(flags);
         }
         HIP_TRY(hipGetLastError());
         return MRLBM_OK;
@@ -236,10 +236,10 @@ class Batch final : public BatchBase {
     }
 
   private:
-    void advance() {
+    void advance(uint32_t flags) {
         cur_ ^= 1;
         ++steps;
-        for (size_t c = 0; c < cases.size(); ++c) sim(c).advance();
+        for (size_t c = 0; c < cases.size(); ++c) sim(c).advance(flags);
     }
 
     // One step from buffer parity `cur` (the caller advances the parity).
@@ -249,10 +249,13 @@ class Batch final : public BatchBase {
         const int f = (flags & MRLBM_STEP_FORCES) ? 1 : 0;
         if (nwblocks_[f] > 0)
             launch_wall_batch<R, S>(d_wargs_[cur][f], d_wblocks_[f], nwblocks_[f], d_wcases_[f], nwcases_[f], stream_);
+        if (flags & MRLBM_STEP_STATS)  // wall-pressure statistics, one launch per case
+            for (size_t c = 0; c < cases.size(); ++c)
+                if (sim(c).stats_on(flags)) launch_pressure<R, S>(sim(c).pressure_args(cur), stream_);
     }
 
     int graph_for(uint32_t flags, hipGraphExec_t &exec) {
-        const int key = ((flags & MRLBM_STEP_FORCES) ? 1 : 0) * 2 + cur_;
+        const int key = int(flags & (MRLBM_STEP_FORCES | MRLBM_STEP_STATS)) * 2 + cur_;
         if (!graphs_[key]) {
             HIP_TRY(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
             for (int s = 0; s < graph_steps_; ++s) queue_step(flags, cur_ ^ (s & 1));
@@ -306,7 +309,7 @@ class Batch final : public BatchBase {
     int nwcases_[2] = {0, 0};
     uint4 *d_items_ = nullptr;
     int nitems_ = 0, groups_ = 1;
-    hipGraphExec_t graphs_[4] = {nullptr, nullptr, nullptr, nullptr};
+    hipGraphExec_t graphs_[8] = {};
 };
 
 // Cases admitted from the front of the list until their device bytes exceed the budget

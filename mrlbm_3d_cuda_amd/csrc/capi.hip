@@ -277,6 +277,17 @@ int mrlbm_checkpoint_read(mrlbm_sim *s, const char *path) {
     return s->impl->set_steps(h.steps);
 }
 
+int mrlbm_stats_reset(mrlbm_sim *s) {
+    DEV_CHECK(s);
+    return s->impl->stats_reset();
+}
+
+int mrlbm_get_surface_pressure(mrlbm_sim *s, double *ps_mean, int64_t *samples) {
+    DEV_CHECK(s);
+    if (!ps_mean) return set_error(MRLBM_ERR_INVALID, "null output");
+    return s->impl->get_surface_pressure(ps_mean, samples);
+}
+
 int mrlbm_destroy(mrlbm_sim *s) {
     if (s && s->in_batch) return set_error(MRLBM_ERR_INVALID, "case is owned by its batch");
     delete s;

@@ -71,6 +71,19 @@ struct WallArgs {
     unsigned long long *__restrict__ step_counter;  // steps completed on the device
 };
 
+// Arguments of the wall-pressure statistics pass (one thread per inner-wall node).
+template <class R, class S>
+struct PressureArgs {
+    const S *__restrict__ rho;        // stored density field of the state after the step
+    const uint32_t *__restrict__ node;  // inner-wall nodes
+    const R *__restrict__ wnx;
+    const R *__restrict__ wny;
+    int n, nx, ny;
+    double xc, yc, radius, rho0;
+    StoreScale<R> scale;
+    double *__restrict__ sum;  // per inner-wall node, FP64 running sum
+};
+
 // Steps of force history kept in the device ring.
 constexpr int FORCE_RING_STEPS = 1 << 16;
 
@@ -94,6 +107,9 @@ void launch_fused(const FusedArgs<R, S> &a, int bx, int by, int variant, hipStre
 
 template <class R, class S>
 void launch_wall(const WallArgs<R, S> &a, hipStream_t stream);
+
+template <class R, class S>
+void launch_pressure(const PressureArgs<R, S> &a, hipStream_t stream);
 
 // Batched launches: one fused pass over the work items of many cases (item {case, tile |
 // special << 31, z0 | z1 << 16, 0}; tiles BX x BY in {32x8, 16x8, 32x4}), and one wall pass

@@ -754,6 +754,58 @@ __global__ __launch_bounds__(WALL_THREADS) void wall_batch_kernel(const WallArgs
     wall_body(a, bc.y);
 }
 
+// Wall pressure of every inner-wall node (compute_surface_pressure,
+// cases/cylinder/cylinder_kernels.cu:238-283): density bilinearly interpolated at fluid points
+// 1, 2 and 3 spacings out along the normal, p = rho cs2, and the quadratic in the distance from
+// the centre through them evaluated at the wall radius (surface_pressure_extrapolation,
+// boundary/extrapolation_utils.cuh:109-154). Evaluated in FP64 and added to a per-node FP64 sum
+// (the reference keeps a running mean in the working precision).
+template <class R, class S>
+__global__ __launch_bounds__(256) void pressure_kernel(PressureArgs<R, S> a) {
+    const int i = int(blockIdx.x) * 256 + int(threadIdx.x);
+    if (i >= a.n) return;
+    const uint32_t idx = a.node[i];
+    const uint32_t plane = uint32_t(a.nx) * uint32_t(a.ny);
+    const uint32_t zoff = (idx / plane) * plane;
+    const double nx = double(a.wnx[i]), ny = double(a.wny[i]);
+    const double xw = a.xc + a.radius * nx, yw = a.yc + a.radius * ny;
+    double xs[3], ys[3], rho[3];
+#pragma unroll
+    for (int p = 0; p < 3; ++p) {
+        xs[p] = xw + double(p + 1) * 1.0 * nx;
+        ys[p] = yw + double(p + 1) * 1.0 * ny;
+        const int x0 = int(floor(xs[p])), y0 = int(floor(ys[p]));
+        const double xd = xs[p] - double(x0), yd = ys[p] - double(y0);
+        const uint32_t b = zoff + uint32_t(x0) + uint32_t(a.nx) * uint32_t(y0);
+        double q[4];
+        const uint32_t c[4] = {b, b + 1, b + uint32_t(a.nx), b + uint32_t(a.nx) + 1};
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            q[j] = double(Conv<S>::to(ld_field(a.rho, c[j])));
+            if constexpr (Conv<S>::scaled) q[j] *= double(a.scale.dec[0]);
+        }
+        const double q0 = (1.0 - xd) * q[0] + xd * q[1];
+        const double q1 = (1.0 - xd) * q[2] + xd * q[3];
+        rho[p] = a.rho0 + ((1.0 - yd) * q0 + yd * q1);
+    }
+    const double cs2 = 1.0 / 3.0;
+    const double xw_diff = xw - a.xc, yw_diff = yw - a.yc;
+    const double x1_diff = xs[0] - a.xc, y1_diff = ys[0] - a.yc;
+    const double x2_diff = xs[1] - a.xc, y2_diff = ys[1] - a.yc;
+    const double x3_diff = xs[2] - a.xc, y3_diff = ys[2] - a.yc;
+    const double rw = sqrt(xw_diff * xw_diff + yw_diff * yw_diff);
+    const double r1 = sqrt(x1_diff * x1_diff + y1_diff * y1_diff);
+    const double r2 = sqrt(x2_diff * x2_diff + y2_diff * y2_diff);
+    const double r3 = sqrt(x3_diff * x3_diff + y3_diff * y3_diff);
+    const double denom = (r1 - r2) * (r1 - r3) * (r2 - r3);
+    const double p1 = rho[0] * cs2, p2 = rho[1] * cs2, p3 = rho[2] * cs2;
+    const double a0 =
+        (r1 * r3 * p2 * (r3 - r1) + (r2 * r2) * (r3 * p1 - r1 * p3) + r2 * ((r1 * r1) * p3 - (r3 * r3) * p1)) / denom;
+    const double a1 = ((r3 * r3) * (p1 - p2) + (r1 * r1) * (p2 - p3) + (r2 * r2) * (p3 - p1)) / denom;
+    const double a2 = (r3 * (p2 - p1) + r2 * (p1 - p3) + r1 * (p3 - p2)) / denom;
+    a.sum[i] += a0 + a1 * rw + a2 * (rw * rw);
+}
+
 // Field-major double arrays (field 0 = rho - rho0, the stored deviation) <-> storage.
 template <class R, class S>
 __global__ void encode_kernel(const double *__restrict__ in, FieldSet<S> out, uint32_t n, StoreScale<R> sc) {
@@ -881,6 +933,11 @@ void launch_wall(const WallArgs<R, S> &a, hipStream_t stream) {
 }
 
 template <class R, class S>
+void launch_pressure(const PressureArgs<R, S> &a, hipStream_t stream) {
+    if (a.n > 0) hipLaunchKernelGGL((pressure_kernel<R, S>), dim3((a.n + 255) / 256), dim3(256), 0, stream, a);
+}
+
+template <class R, class S>
 void launch_encode(const double *in, const FieldSet<S> &out, uint32_t n, const StoreScale<R> &sc, hipStream_t stream) {
     hipLaunchKernelGGL((encode_kernel<R, S>), dim3((n + 255) / 256), dim3(256), 0, stream, in, out, n, sc);
 }
@@ -895,6 +952,7 @@ void launch_decode(const FieldSet<const S> &in, double *out, uint32_t n, const S
 #define MRLBM_INSTANTIATE(R, S)                                                                             \
     template void launch_fused<R, S>(const FusedArgs<R, S> &, int, int, int, hipStream_t);                  \
     template void launch_wall<R, S>(const WallArgs<R, S> &, hipStream_t);                                   \
+    template void launch_pressure<R, S>(const PressureArgs<R, S> &, hipStream_t);                           \
     template void launch_fused_batch<R, S>(const FusedArgs<R, S> *, const uint4 *, int, int, int, int, int,     \
                                            hipStream_t);                                                    \
     template int prepare_fused_batch<R, S>(int, int, int);                                                  \

@@ -54,6 +54,8 @@ class SimBase {
     virtual int time_steps(int n, uint32_t flags, double *ms_total, double *ms_k) = 0;
     virtual int sync() = 0;
     virtual int set_steps(int64_t n) = 0;
+    virtual int stats_reset() = 0;
+    virtual int get_surface_pressure(double *mean, int64_t *samples) = 0;
     mrlbm_case_desc d{};
     Geometry g;
     int64_t steps = 0;
@@ -230,6 +232,17 @@ class Sim final : public SimBase {
         // the wall pass launches at most wall_blocks(nwall, nbcf) blocks (forces on)
         HIP_TRY(hipMalloc(&d_partials_, size_t(wall_blocks(nwall, nbcf_)) * 4 * sizeof(double)));
         HIP_TRY(hipMalloc(&d_ring_, size_t(FORCE_RING_STEPS) * 4 * sizeof(double)));
+        // wall-pressure statistics of the inner wall: fluid points 1..3 spacings out must lie inside
+        stats_ok_ = g.ngroups > 0 && !g.wall[0].node.empty();
+        for (size_t k = 0; stats_ok_ && k < g.wall[0].node.size(); ++k) {
+            const double xw = d.xc + g.wall[0].radius * g.wall[0].nx[k], yw = d.yc + g.wall[0].radius * g.wall[0].ny[k];
+            for (int p = 1; p <= 3; ++p) {
+                const int x0 = int(std::floor(xw + p * g.wall[0].nx[k])), y0 = int(std::floor(yw + p * g.wall[0].ny[k]));
+                if (x0 < 0 || y0 < 0 || x0 + 1 >= g.nx || y0 + 1 >= g.ny) stats_ok_ = false;
+            }
+        }
+        HIP_TRY(hipMalloc(&d_ps_sum_, std::max<size_t>(g.wall[0].node.size(), 1) * sizeof(double)));
+        HIP_TRY(hipMemsetAsync(d_ps_sum_, 0, std::max<size_t>(g.wall[0].node.size(), 1) * sizeof(double), stream_));
         HIP_TRY(hipMalloc(&d_step_counter_, sizeof(unsigned long long)));
         HIP_TRY(hipMemsetAsync(d_step_counter_, 0, sizeof(unsigned long long), stream_));
         HIP_TRY(hipMemsetAsync(d_ring_, 0, size_t(FORCE_RING_STEPS) * 4 * sizeof(double), stream_));
@@ -326,6 +339,47 @@ class Sim final : public SimBase {
         return MRLBM_OK;
     }
 
+    // Wall-pressure statistics (MRLBM_STEP_STATS): FP64 sums per inner-wall node.
+    int stats_reset() override {
+        if (int rc = wait_batch()) return rc;
+        HIP_TRY(hipStreamSynchronize(stream_));
+        HIP_TRY(hipMemset(d_ps_sum_, 0, std::max<size_t>(g.wall[0].node.size(), 1) * sizeof(double)));
+        stats_samples_ = 0;
+        return MRLBM_OK;
+    }
+
+    int get_surface_pressure(double *mean, int64_t *samples) override {
+        if (!stats_ok_) return set_error(MRLBM_ERR_UNSUPPORTED, "no inner wall with a pressure stencil inside the grid");
+        if (int rc = wait_batch()) return rc;
+        HIP_TRY(hipStreamSynchronize(stream_));
+        const size_t nb = g.wall[0].node.size();
+        HIP_TRY(hipMemcpy(mean, d_ps_sum_, nb * sizeof(double), hipMemcpyDeviceToHost));
+        for (size_t k = 0; k < nb; ++k) mean[k] = stats_samples_ ? mean[k] / double(stats_samples_) : 0.0;
+        if (samples) *samples = stats_samples_;
+        return MRLBM_OK;
+    }
+
+    bool stats_on(uint32_t flags) const { return (flags & MRLBM_STEP_STATS) && stats_ok_; }
+
+    // Arguments of the wall-pressure pass on the state after a step from parity `cur`.
+    PressureArgs<R, S> pressure_args(int cur) const {
+        PressureArgs<R, S> p{};
+        p.rho = buf_[cur ^ 1];
+        p.node = d_wnode_;
+        p.wnx = d_wnx_;
+        p.wny = d_wny_;
+        p.n = int(g.wall[0].node.size());
+        p.nx = g.nx;
+        p.ny = g.ny;
+        p.xc = d.xc;
+        p.yc = d.yc;
+        p.radius = g.wall[0].radius;
+        p.rho0 = d.rho0;
+        p.scale = scale_;
+        p.sum = d_ps_sum_;
+        return p;
+    }
+
     const LaunchConfig &launch_config() const { return launch_; }
     // per-XCD work-item lists (each padded to the same length) of the current configuration
     const std::vector<std::vector<uint2>> &item_lists() const { return lists_; }
@@ -406,9 +460,10 @@ class Sim final : public SimBase {
     bool forces_on(uint32_t flags) const { return (flags & MRLBM_STEP_FORCES) && g.ngroups > 0; }
 
     // Buffers swap after every step (driven by a batch that launches the kernels itself).
-    void advance() {
+    void advance(uint32_t flags) {
         cur_ ^= 1;
         ++steps;
+        if (stats_on(flags)) ++stats_samples_;
         launched_ = true;
     }
 
@@ -442,6 +497,7 @@ class Sim final : public SimBase {
                 if (rc) return rc;
                 HIP_TRY(hipGraphLaunch(exec, stream_));
                 steps += graph_steps_;
+                if (stats_on(flags)) stats_samples_ += graph_steps_;
                 done += graph_steps_;
             }
         }
@@ -553,16 +609,17 @@ class Sim final : public SimBase {
 
     // Executable graph of graph_steps_ steps for these flags from the current buffer parity.
     int graph_for(uint32_t flags, hipGraphExec_t &exec) {
-        const int key = ((flags & MRLBM_STEP_FORCES) ? 1 : 0) * 2 + cur_;
+        const int key = int(flags & (MRLBM_STEP_FORCES | MRLBM_STEP_STATS)) * 2 + cur_;
         if (!graphs_[key]) {
             const int cur0 = cur_;
-            const int64_t steps0 = steps;
+            const int64_t steps0 = steps, samples0 = stats_samples_;
             HIP_TRY(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
             for (int s = 0; s < graph_steps_; ++s) queue_step(flags, nullptr);
             hipGraph_t graph = nullptr;
             HIP_TRY(hipStreamEndCapture(stream_, &graph));
             cur_ = cur0;
             steps = steps0;
+            stats_samples_ = samples0;
             hipGraphExec_t e = nullptr;
             const hipError_t err = hipGraphInstantiate(&e, graph, nullptr, nullptr, 0);
             (void)hipGraphDestroy(graph);
@@ -586,6 +643,10 @@ class Sim final : public SimBase {
         } else if (ev) {
             (void)hipEventRecord(ev[2], stream_);
             (void)hipEventRecord(ev[3], stream_);
+        }
+        if (stats_on(flags)) {
+            launch_pressure<R, S>(pressure_args(cur_), stream_);
+            ++stats_samples_;
         }
         cur_ ^= 1;
         ++steps;
@@ -631,7 +692,7 @@ class Sim final : public SimBase {
             if (g) (void)hipGraphExecDestroy(g);
         void *ptrs[] = {buf_[0], buf_[1], donor_, d_info_, d_items_, d_wconst_, d_wnx_, d_wny_, d_win_, d_wout_, d_wnode_,
                         d_wcoef_, d_stbase_, d_stfrac_, d_bcfnode_, d_bcftag_, d_bcfin_, d_bcfout_, d_inlet_,
-                        d_partials_, d_ring_, d_step_counter_};
+                        d_partials_, d_ring_, d_step_counter_, d_ps_sum_};
         for (void *p : ptrs)
             if (p) (void)hipFree(p);
         if (stream_) (void)hipStreamDestroy(stream_);
@@ -649,7 +710,7 @@ class Sim final : public SimBase {
     bool stochastic_ = Conv<S>::scaled;
     int graph_steps_ = 16;
     bool launched_ = false;
-    hipGraphExec_t graphs_[4] = {nullptr, nullptr, nullptr, nullptr};
+    hipGraphExec_t graphs_[8] = {};
     int bx_ = 32, by_ = 8, lz_ = 16, lz_special_ = 16, tiles_x_ = 1, tiles_y_ = 1, ntiles_ = 1, nblocks_ = 1, xcd_ = 1;
     std::vector<uint32_t> special_cols_;  // x + nx y of columns holding non-bulk nodes
     std::vector<uint8_t> frozen_cols_;    // per column: frozen at every z
@@ -663,7 +724,9 @@ class Sim final : public SimBase {
     uint32_t *d_win_ = nullptr, *d_wout_ = nullptr, *d_wnode_ = nullptr, *d_stbase_ = nullptr;
     uint32_t *d_bcfnode_ = nullptr, *d_bcfin_ = nullptr, *d_bcfout_ = nullptr;
     uint8_t *d_bcftag_ = nullptr;
-    double *d_partials_ = nullptr, *d_ring_ = nullptr;
+    double *d_partials_ = nullptr, *d_ring_ = nullptr, *d_ps_sum_ = nullptr;
+    bool stats_ok_ = false;
+    int64_t stats_samples_ = 0;
     unsigned long long *d_step_counter_ = nullptr;
 };
 

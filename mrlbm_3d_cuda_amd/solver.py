@@ -92,9 +92,20 @@ class Simulation:
         _check(self.lib.mrlbm_init(self.h))
         return self
 
-    def step(self, n=1, forces=False):
-        _check(self.lib.mrlbm_step(self.h, n, _abi.STEP_FORCES if forces else 0))
+    def step(self, n=1, forces=False, stats=False):
+        flags = (_abi.STEP_FORCES if forces else 0) | (_abi.STEP_STATS if stats else 0)
+        _check(self.lib.mrlbm_step(self.h, n, flags))
         return self
+
+    def stats_reset(self):
+        _check(self.lib.mrlbm_stats_reset(self.h))
+
+    def surface_pressure(self):
+        """(mean wall pressure per inner-wall node in wall-table order, steps averaged)."""
+        out = np.empty(self.info().nb_inner, np.float64)
+        n = C.c_int64()
+        _check(self.lib.mrlbm_get_surface_pressure(self.h, as_ptr(out, C.c_double), C.byref(n)))
+        return out, n.value
 
     def sync(self):
         _check(self.lib.mrlbm_sync(self.h))
@@ -216,8 +227,9 @@ class Batch:
         _check(self.lib.mrlbm_batch_init(self.h))
         return self
 
-    def step(self, n=1, forces=False):
-        _check(self.lib.mrlbm_batch_step(self.h, n, _abi.STEP_FORCES if forces else 0))
+    def step(self, n=1, forces=False, stats=False):
+        flags = (_abi.STEP_FORCES if forces else 0) | (_abi.STEP_STATS if stats else 0)
+        _check(self.lib.mrlbm_batch_step(self.h, n, flags))
         return self
 
     def sync(self):

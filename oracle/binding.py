@@ -39,6 +39,8 @@ _PROTOS = {
     "oracle_node_bc": (C.c_int, [C.c_uint32, C.c_uint32, C.c_uint32, C.c_double, C.c_double, _dp, _dp, _dp, _dp]),
     "oracle_incoming_rotated": (None, [C.c_double, C.c_double, C.c_uint32, _dp, _dp, _dp]),
     "oracle_extrapolation_coeffs": (None, [C.c_double, _dp]),
+    "oracle_surface_pressure": (C.c_int, [C.c_void_p, _dp]),
+    "oracle_surface_pressure_extrapolation": (C.c_double, [C.c_double, C.c_double, _dp, _dp]),
     "oracle_curved_solution": (None, [C.c_double, C.c_double, C.c_uint32, _dp, C.c_double, C.c_double, C.c_double, _dp]),
 }
 
@@ -133,6 +135,12 @@ class Oracle:
                "oracle_get_wall_table")
         return dict(node=node, nx=nx, ny=ny, delta=delta, in_mask=fin, out_mask=fout)
 
+    def surface_pressure(self):
+        """Instantaneous wall pressure of every inner-wall node (compute_surface_pressure)."""
+        out = np.empty(self.info().nb_inner, np.float64)
+        _check(self.lib.oracle_surface_pressure(self.h, as_ptr(out, C.c_double)), "surface_pressure")
+        return out
+
     def bcfluid_table(self, group):
         info = self.info()
         n = info.nb_fluid_inner if group == 0 else info.nb_fluid_outer
@@ -186,6 +194,12 @@ def incoming_rotated(nx, ny, in_mask, f27):
     m6 = np.empty(6, np.float64)
     lib().oracle_incoming_rotated(nx, ny, in_mask, as_ptr(f, C.c_double), C.byref(rho), as_ptr(m6, C.c_double))
     return rho.value, m6
+
+
+def surface_pressure_extrapolation(xc, yc, xy8, rho3):
+    a = np.ascontiguousarray(xy8, np.float64)
+    r = np.ascontiguousarray(rho3, np.float64)
+    return lib().oracle_surface_pressure_extrapolation(xc, yc, as_ptr(a, C.c_double), as_ptr(r, C.c_double))
 
 
 def extrapolation_coeffs(delta):

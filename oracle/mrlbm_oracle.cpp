@@ -444,6 +444,30 @@ double uniform01(uint64_t seed, uint64_t counter) {
     return double(splitmix64(seed ^ splitmix64(counter)) >> 11) * 0x1.0p-53;
 }
 
+// surface_pressure_extrapolation (boundary/extrapolation_utils.cuh:109-154): pressures
+// p_k = rho_k cs2 at three points along the wall normal, a quadratic in the distance from the
+// cylinder centre through them, evaluated at the wall radius.
+double surface_pressure_extrapolation(double xc, double yc, double xw, double yw, double x1, double y1, double x2,
+                                      double y2, double x3, double y3, double rho1, double rho2, double rho3) {
+    const double cs2 = 1.0 / 3.0;
+    const double xw_diff = xw - xc, yw_diff = yw - yc;
+    const double x1_diff = x1 - xc, y1_diff = y1 - yc;
+    const double x2_diff = x2 - xc, y2_diff = y2 - yc;
+    const double x3_diff = x3 - xc, y3_diff = y3 - yc;
+    const double rw2 = xw_diff * xw_diff + yw_diff * yw_diff;
+    const double r12 = x1_diff * x1_diff + y1_diff * y1_diff;
+    const double r22 = x2_diff * x2_diff + y2_diff * y2_diff;
+    const double r32 = x3_diff * x3_diff + y3_diff * y3_diff;
+    const double rw = std::sqrt(rw2), r1 = std::sqrt(r12), r2 = std::sqrt(r22), r3 = std::sqrt(r32);
+    const double denom = (r1 - r2) * (r1 - r3) * (r2 - r3);
+    const double p1 = rho1 * cs2, p2 = rho2 * cs2, p3 = rho3 * cs2;
+    const double a0 =
+        (r1 * r3 * p2 * (r3 - r1) + (r2 * r2) * (r3 * p1 - r1 * p3) + r2 * ((r1 * r1) * p3 - (r3 * r3) * p1)) / denom;
+    const double a1 = ((r3 * r3) * (p1 - p2) + (r1 * r1) * (p2 - p3) + (r2 * r2) * (p3 - p1)) / denom;
+    const double a2 = (r3 * (p2 - p1) + r2 * (p1 - p3) + r1 * (p3 - p2)) / denom;
+    return a0 + a1 * rw + a2 * (rw * rw);
+}
+
 // ---------------------------------------------------------------- simulation
 
 struct WallGroup {
@@ -477,6 +501,7 @@ struct Base {
     virtual int step(int n, uint32_t flags) = 0;
     virtual void get(int f, double *h) const = 0;
     virtual void set(int f, const double *h) = 0;
+    virtual int surface_pressure(double *out) const = 0;
     int setup();
 };
 
@@ -768,6 +793,30 @@ struct Sim : Base {
         return (R(1) - yd) * q0 + yd * q1;
     }
 
+    // compute_surface_pressure (cases/cylinder/cylinder_kernels.cu:238-283) without the running
+    // mean: the instantaneous wall pressure of every inner-wall node from the current moments,
+    // with fluid points 1, 2 and 3 spacings out along the normal.
+    int surface_pressure(double *out) const override {
+        if (ngroups < 1) return fail(MRLBM_ERR_INVALID, "oracle: no curved wall");
+        const WallGroup &g = grp[0];
+        for (size_t k = 0; k < g.node.size(); ++k) {
+            const int64_t i = g.node[k];
+            const int z = int(i / (int64_t(NX) * NY));
+            const double nx = g.nx[k], ny = g.ny[k];
+            const double xw = d.xc + g.radius * nx, yw = d.yc + g.radius * ny;
+            double xs[3], ys[3], rho[3];
+            for (int p = 0; p < 3; ++p) {
+                xs[p] = xw + double(p + 1) * 1.0 * nx;
+                ys[p] = yw + double(p + 1) * 1.0 * ny;
+                if (!in_bounds(R(xs[p]), R(ys[p]))) return fail(MRLBM_ERR_GEOMETRY, "oracle: pressure stencil leaves the grid");
+                rho[p] = d.rho0 + double(bilinear(M, 0, R(xs[p]), R(ys[p]), z));
+            }
+            out[k] = surface_pressure_extrapolation(d.xc, d.yc, xw, yw, xs[0], ys[0], xs[1], ys[1], xs[2], ys[2], rho[0],
+                                                    rho[1], rho[2]);
+        }
+        return MRLBM_OK;
+    }
+
     // initerpolate_and_rotate + rotate_velocity_moments (extrapolation_utils.cuh:63-107).
     void interp_rotate(const std::vector<R> *src, R nx, R ny, R x, R y, int z, R *vm) const {
         R v[9];
@@ -964,6 +1013,11 @@ int oracle_create(const mrlbm_case_desc *desc, int32_t real_bytes, oracle_sim **
 }
 
 int oracle_init(oracle_sim *s) { return s->impl->init(); }
+int oracle_surface_pressure(oracle_sim *s, double *out) { return s->impl->surface_pressure(out); }
+double oracle_surface_pressure_extrapolation(double xc, double yc, const double *xy8, const double *rho3) {
+    return surface_pressure_extrapolation(xc, yc, xy8[0], xy8[1], xy8[2], xy8[3], xy8[4], xy8[5], xy8[6], xy8[7],
+                                          rho3[0], rho3[1], rho3[2]);
+}
 int oracle_step(oracle_sim *s, int32_t n, uint32_t flags) { return s->impl->step(n, flags); }
 
 int oracle_get_moments(oracle_sim *s, int32_t f, double *h) {

@@ -48,6 +48,10 @@ int oracle_node_bc(uint32_t type, uint32_t tag, uint32_t in_mask, double omega, 
 void oracle_incoming_rotated(double nx, double ny, uint32_t in_mask, const double *f27,
                              double *rho, double *m6);
 void oracle_extrapolation_coeffs(double delta, double *c3);
+/* instantaneous wall pressure of every inner-wall node (nb_inner values) */
+int oracle_surface_pressure(oracle_sim *s, double *out);
+/* xy8 = (xw, yw, x1, y1, x2, y2, x3, y3); the quadratic wall-pressure extrapolation */
+double oracle_surface_pressure_extrapolation(double xc, double yc, const double *xy8, const double *rho3);
 void oracle_curved_solution(double nx, double ny, uint32_t in_mask, const double *prime7,
                             double rho_i, double mxy_i, double mxz_i, double *out10);
 

@@ -89,3 +89,35 @@ def test_checkpoint_round_trip_bitwise(tmp_path, precision):
     with pytest.raises(MrlbmError) as e:
         b.checkpoint_read(str(tmp_path / "missing.bin"))
     assert e.value.status == _abi.ERR_IO
+
+
+@pytest.mark.parametrize("precision", [_abi.FP64, _abi.FP32])
+def test_wall_pressure_statistics(precision):
+    """Time-averaged wall pressure (FP64 sums): the mean over the statistics window equals the
+    oracle's instantaneous wall pressures averaged over the same steps -- to 1e-12 relative for
+    FP64 storage, 1e-4 relative of the pressure level for FP32; steps without MRLBM_STEP_STATS and
+    steps before a reset do not count."""
+    from mrlbm_3d_cuda_amd.solver import Simulation
+    from oracle.binding import Oracle
+    d = configs.cylinder(nx=96, ny=48, nz=8, diameter=10.0, xc=24.0, yc=24.0, u_max=0.05, re=50.0,
+                         precision=precision)
+    s = Simulation(d).init()
+    s.step(20, stats=True)
+    s.stats_reset()
+    s.step(10)
+    s.step(37, forces=True, stats=True)
+    ps, n = s.surface_pressure()
+    s.close()
+    assert n == 37
+    d64 = d.copy()
+    d64.precision = _abi.FP64
+    o = Oracle(d64).init()
+    o.step(30)
+    acc = np.zeros_like(ps)
+    for _ in range(37):
+        o.step(1)
+        acc += o.surface_pressure()
+    ref = acc / 37
+    err = float(np.max(np.abs(ps - ref)) / np.max(np.abs(ref)))
+    print(f"[gpu stats] {_abi.PRECISION_NAMES[precision]}: wall-pressure mean rel. error {err:.3e}", flush=True)
+    assert err <= (1e-12 if precision == _abi.FP64 else 1e-4)

@@ -255,3 +255,35 @@ def test_curved_closure_reproduces_equilibrium(oracle_lib, theta):
     assert abs(out[0] - 1) < 1e-13
     assert np.allclose(out[1:4], u, atol=1e-15)
     assert np.allclose(out[4:], [uu[0, 0], uu[1, 1], uu[2, 2], uu[0, 1], uu[0, 2], uu[1, 2]], atol=1e-13)
+
+
+# ---- wall-pressure statistics (surface_pressure_extrapolation, extrapolation_utils.cuh:109-154)
+
+@pytest.mark.parametrize("theta,R", [(0.0, 10.0), (0.9, 7.3), (2.6, 12.5), (-2.0, 5.0)])
+def test_pressure_extrapolation_reproduces_quadratics(theta, R):
+    """Pressures from an exact quadratic p(r) = a + b r + c r^2 at the three fluid points give
+    that quadratic's value at the wall radius."""
+    from oracle.binding import surface_pressure_extrapolation
+    xc, yc = 31.5, 20.25
+    nx, ny = np.cos(theta), np.sin(theta)
+    a, b, c = 0.3312, 0.0021, -0.00037
+    pts = [(xc + (R + k) * nx, yc + (R + k) * ny) for k in range(4)]  # wall point, then 1, 2, 3 out
+    rho = [(a + b * (R + k) + c * (R + k) ** 2) * 3.0 for k in (1, 2, 3)]
+    xy8 = [v for p in pts for v in p]
+    ps = surface_pressure_extrapolation(xc, yc, xy8, rho)
+    assert abs(ps - (a + b * R + c * R * R)) < 1e-13
+
+
+def test_wall_pressure_of_uniform_and_linear_density():
+    from oracle.binding import Oracle
+    d = configs.cylinder(nx=64, ny=40, nz=3, diameter=12.0, xc=20.0, yc=19.5, precision=_abi.FP64, noise=0.0)
+    o = Oracle(d).init()
+    assert np.allclose(o.surface_pressure(), 1.0 / 3.0, atol=1e-15)
+    # bilinear interpolation is exact on a linear field, and p(r) along a normal is then linear
+    z, y, x = np.meshgrid(np.arange(d.nz), np.arange(d.ny), np.arange(d.nx), indexing="ij")
+    alpha, beta = 2e-4, -3e-4
+    o.set_field(0, alpha * x + beta * y)
+    t = o.wall_table(0)
+    R = d.r_inner
+    expect = (1.0 + alpha * (d.xc + R * t["nx"]) + beta * (d.yc + R * t["ny"])) / 3.0
+    assert np.allclose(o.surface_pressure(), expect, atol=1e-14)
