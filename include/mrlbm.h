@@ -227,6 +227,36 @@ int mrlbm_stats_reset(mrlbm_sim *sim);
 int mrlbm_get_surface_pressure(mrlbm_sim *sim, double *ps_mean, int64_t *samples);
 
 /*
+ * Visualisation output <- write_vti_3d / write_vti_3d_annulus / write_master_pvd
+ * (src/output/saveData.cu:3-230, called from src/cases/cylinder/cylinder_kernels.cu:133 and
+ * src/simulation.cu:8,133). The point arrays are computed on the device from the current moments:
+ *   MRLBM_VTI_CARTESIAN: rho (rho0 + stored deviation), ux, uy, uz
+ *   MRLBM_VTI_ANNULUS:   rho, uz, u_theta, u_radial -- the in-plane velocity projected on the
+ *                        polar frame about (xc, yc) and divided by vel_norm (0 at the axis)
+ * as float32, x fastest, with the velocity arrays NaN at SOLID nodes. vel_norm == 0 selects the
+ * case's VEL_NORM (annulus: u_wall_inner[1]; otherwise u_max).
+ * mrlbm_vti_points returns the four arrays (4 * num_nodes floats, field-major);
+ * mrlbm_write_vti writes <dir>/data<10000000 + timestep>.vti (VTK ImageData, appended raw data
+ * with UInt64 size headers); mrlbm_write_master_pvd writes <dir>/master.pvd listing the files of
+ * steps 0, save_every, ... <= max_iter. File errors return MRLBM_ERR_IO (the reference skips
+ * the file silently).
+ */
+enum mrlbm_vti_layout {
+    MRLBM_VTI_CARTESIAN = 0,
+    MRLBM_VTI_ANNULUS = 1
+};
+int mrlbm_vti_points(mrlbm_sim *sim, int32_t layout, double vel_norm, float *out4);
+int mrlbm_write_vti(mrlbm_sim *sim, const char *dir, int32_t timestep, int32_t layout, double vel_norm);
+int mrlbm_write_master_pvd(const char *dir, int32_t max_iter, int32_t save_every);
+
+/*
+ * Run summary text <- getSimInfoString / writeSimInfo (src/utils/file_utils.cuh:47-185): case,
+ * lattice parameters and the device layout. Writes at most cap - 1 characters and a NUL to buf
+ * (buf may be NULL when cap is 0) and the full length to *len.
+ */
+int mrlbm_sim_info(mrlbm_sim *sim, char *buf, int64_t cap, int64_t *len);
+
+/*
  * Batches: many cases of one precision stepped together -- one fused launch over the work items
  * of every case and one wall launch over the wall nodes of every case per step, replayed from
  * captured HIP graphs. No counterpart in the reference, which compiles one case per run

@@ -44,6 +44,10 @@ COMPAT_INIT_M_ZERO = 0x1
 STEP_FORCES = 0x1
 STEP_STATS = 0x2
 
+VTI_CARTESIAN = 0  # rho, ux, uy, uz
+VTI_ANNULUS = 1    # rho, uz, u_theta, u_radial
+VTI_FIELDS = {VTI_CARTESIAN: ("rho", "ux", "uy", "uz"), VTI_ANNULUS: ("rho", "uz", "u_theta", "u_radial")}
+
 # reference node types (common/nodeTypeMap.cuh:38-80)
 NODE_TYPES = (
     "BULK", "SOLID", "INNER", "OUTER", "BCFLUID_INNER", "BCFLUID_OUTER", "BCSOLID_INNER",
@@ -154,6 +158,10 @@ PRODUCT_PROTOTYPES = {
     "mrlbm_destroy": (C.c_int, [C.c_void_p]),
     "mrlbm_stats_reset": (C.c_int, [C.c_void_p]),
     "mrlbm_get_surface_pressure": (C.c_int, [C.c_void_p, _dp, P(C.c_int64)]),
+    "mrlbm_vti_points": (C.c_int, [C.c_void_p, C.c_int32, C.c_double, P(C.c_float)]),
+    "mrlbm_write_vti": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int32, C.c_int32, C.c_double]),
+    "mrlbm_write_master_pvd": (C.c_int, [C.c_char_p, C.c_int32, C.c_int32]),
+    "mrlbm_sim_info": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int64, P(C.c_int64)]),
     "mrlbm_last_error": (C.c_char_p, []),
     "mrlbm_abi_version": (C.c_int32, []),
 }

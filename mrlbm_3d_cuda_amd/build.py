@@ -14,13 +14,15 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libmrlbm.so")
 ARCH = os.environ.get("MRLBM_OFFLOAD_ARCH", "gfx950")
 
-# per-source extra flags: the FP64 kernels are compiled without FMA contraction
+# per-source extra flags: the FP64 kernels and the output arrays are compiled without FMA
+# contraction
 SOURCES = {
     "geometry.cpp": [],
     "kernels_f64.hip": ["-ffp-contract=off"],
     "kernels_f32.hip": [],
     "capi.hip": [],
     "batch.hip": [],
+    "output.hip": ["-ffp-contract=off"],
 }
 
 HIPCC_FLAGS = [

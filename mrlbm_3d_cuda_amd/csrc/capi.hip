@@ -288,6 +288,64 @@ int mrlbm_get_surface_pressure(mrlbm_sim *s, double *ps_mean, int64_t *samples) 
     return s->impl->get_surface_pressure(ps_mean, samples);
 }
 
+// Velocity divisor of the cylindrical components: the caller's, or the case's VEL_NORM
+// (src/cases/annulus/constants.cuh:75, src/cases/cylinder/constants.cuh:56).
+static int vti_check(mrlbm_sim *s, int32_t layout, double vel_norm, float *norm) {
+    if (layout != MRLBM_VTI_CARTESIAN && layout != MRLBM_VTI_ANNULUS) return set_error(MRLBM_ERR_INVALID, "bad VTI layout");
+    const mrlbm_case_desc &d = s->impl->d;
+    const double v = vel_norm != 0.0 ? vel_norm : (d.kind == MRLBM_CASE_ANNULUS ? d.u_wall_inner[1] : d.u_max);
+    if (layout == MRLBM_VTI_ANNULUS && !(std::isfinite(v) && v != 0.0))
+        return set_error(MRLBM_ERR_INVALID, "the annulus layout needs a finite nonzero velocity norm");
+    *norm = float(v);
+    return MRLBM_OK;
+}
+
+int mrlbm_vti_points(mrlbm_sim *s, int32_t layout, double vel_norm, float *out4) {
+    DEV_CHECK(s);
+    if (!out4) return set_error(MRLBM_ERR_INVALID, "null output");
+    float norm = 0.0f;
+    if (int rc = vti_check(s, layout, vel_norm, &norm)) return rc;
+    return s->impl->vti_points(layout, norm, out4);
+}
+
+int mrlbm_write_vti(mrlbm_sim *s, const char *dir, int32_t timestep, int32_t layout, double vel_norm) {
+    DEV_CHECK(s);
+    if (!dir || !*dir || timestep < 0) return set_error(MRLBM_ERR_INVALID, "bad directory or timestep");
+    float norm = 0.0f;
+    if (int rc = vti_check(s, layout, vel_norm, &norm)) return rc;
+    const mrlbm::Geometry &g = s->impl->g;
+    std::vector<float> pts;
+    try {
+        pts.resize(size_t(4) * size_t(g.n));
+    } catch (const std::bad_alloc &) {
+        return set_error(MRLBM_ERR_NOMEM, "host buffer of the VTI points");
+    }
+    if (int rc = s->impl->vti_points(layout, norm, pts.data())) return rc;
+    static const char *const cartesian[4] = {"rho", "ux", "uy", "uz"};
+    static const char *const annulus[4] = {"rho", "uz", "u_theta", "u_radial"};
+    return mrlbm::write_vti_file(std::string(dir) + "/" + mrlbm::vti_file_name(timestep), g.nx, g.ny, g.nz,
+                                 layout == MRLBM_VTI_CARTESIAN ? cartesian : annulus, pts.data());
+}
+
+int mrlbm_write_master_pvd(const char *dir, int32_t max_iter, int32_t save_every) {
+    if (!dir || !*dir || max_iter < 0 || save_every <= 0)
+        return set_error(MRLBM_ERR_INVALID, "bad directory, step count or save interval");
+    return mrlbm::write_pvd_file(std::string(dir) + "/master.pvd", max_iter, save_every);
+}
+
+int mrlbm_sim_info(mrlbm_sim *s, char *buf, int64_t cap, int64_t *len) {
+    SIM_CHECK(s);
+    if (cap < 0 || (cap > 0 && !buf)) return set_error(MRLBM_ERR_INVALID, "bad buffer");
+    const std::string t = mrlbm::sim_info_string(*s->impl);
+    if (len) *len = int64_t(t.size());
+    if (cap > 0) {
+        const size_t k = std::min(t.size(), size_t(cap - 1));
+        std::memcpy(buf, t.data(), k);
+        buf[k] = '\0';
+    }
+    return MRLBM_OK;
+}
+
 int mrlbm_destroy(mrlbm_sim *s) {
     if (s && s->in_batch) return set_error(MRLBM_ERR_INVALID, "case is owned by its batch");
     delete s;

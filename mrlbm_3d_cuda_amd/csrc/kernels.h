@@ -138,6 +138,21 @@ void launch_encode(const double *in, const FieldSet<S> &out, uint32_t n, const S
 template <class R, class S>
 void launch_decode(const FieldSet<const S> &in, double *out, uint32_t n, const StoreScale<R> &sc, hipStream_t stream);
 
+// Point arrays of a VTK output file (output.hip): four float32 arrays, field-major, x fastest.
+struct VtiArgs {
+    uint32_t n;          // nodes
+    int nx, ny;          // grid (x fastest)
+    int layout;          // MRLBM_VTI_CARTESIAN or MRLBM_VTI_ANNULUS
+    double xc, yc;       // axis of the cylindrical projection
+    float vel_norm;      // velocity divisor of the cylindrical components
+    const uint32_t *solid;  // one bit per node: SOLID points are written as NaN
+    float *out;             // 4 * n values
+};
+
+template <class R, class S>
+void launch_vti_points(const FieldSet<const S> &in, const StoreScale<R> &sc, R rho0, const VtiArgs &a,
+                       hipStream_t stream);
+
 // Launch configuration of the fused pass (chosen by defaults, the environment or the tuner).
 struct LaunchConfig {
     int bx = 32, by = 8;    // tile

@@ -56,6 +56,9 @@ class SimBase {
     virtual int set_steps(int64_t n) = 0;
     virtual int stats_reset() = 0;
     virtual int get_surface_pressure(double *mean, int64_t *samples) = 0;
+    // four float32 point arrays of a VTK file from the current state (MRLBM_VTI_* layout)
+    virtual int vti_points(int layout, float vel_norm, float *host) = 0;
+    virtual LaunchConfig current_launch() const = 0;
     mrlbm_case_desc d{};
     Geometry g;
     int64_t steps = 0;
@@ -381,6 +384,38 @@ class Sim final : public SimBase {
     }
 
     const LaunchConfig &launch_config() const { return launch_; }
+    LaunchConfig current_launch() const override { return launch_; }
+
+    int vti_points(int layout, float vel_norm, float *host) override {
+        if (int rc = wait_batch()) return rc;
+        const size_t n = size_t(g.n), words = (n + 31) / 32;
+        std::vector<uint32_t> bits(words, 0u);
+        for (size_t i = 0; i < n; ++i)
+            if (g.ref_type[i] == RT_SOLID) bits[i >> 5] |= 1u << (i & 31);
+        float *tmp = nullptr;
+        HIP_TRY(hipMalloc(&tmp, 4 * n * sizeof(float) + words * sizeof(uint32_t)));
+        const int rc = [&]() -> int {
+            VtiArgs a{};
+            a.n = uint32_t(n);
+            a.nx = g.nx;
+            a.ny = g.ny;
+            a.layout = layout;
+            a.xc = d.xc;
+            a.yc = d.yc;
+            a.vel_norm = vel_norm;
+            a.solid = reinterpret_cast<const uint32_t *>(tmp + 4 * n);
+            a.out = tmp;
+            HIP_TRY(hipStreamSynchronize(stream_));
+            HIP_TRY(hipMemcpy(tmp + 4 * n, bits.data(), words * sizeof(uint32_t), hipMemcpyHostToDevice));
+            launch_vti_points<R, S>(as_const(make_fields(buf_[cur_], stride_)), scale_, R(d.rho0), a, stream_);
+            HIP_TRY(hipGetLastError());
+            HIP_TRY(hipStreamSynchronize(stream_));
+            HIP_TRY(hipMemcpy(host, tmp, 4 * n * sizeof(float), hipMemcpyDeviceToHost));
+            return MRLBM_OK;
+        }();
+        (void)hipFree(tmp);
+        return rc;
+    }
     // per-XCD work-item lists (each padded to the same length) of the current configuration
     const std::vector<std::vector<uint2>> &item_lists() const { return lists_; }
     int nwall() const { return nwall_; }
@@ -736,6 +771,12 @@ namespace mrlbm {
 
 // A simulation of the descriptor's precision with its geometry built; with `device` also its
 // device buffers and tables.
+// output.hip: file formats of the visualisation output and the run summary
+std::string vti_file_name(int64_t timestep);
+int write_vti_file(const std::string &path, int nx, int ny, int nz, const char *const names[4], const float *points);
+int write_pvd_file(const std::string &path, int32_t max_iter, int32_t save_every);
+std::string sim_info_string(const SimBase &s);
+
 inline int make_sim(const mrlbm_case_desc &desc, bool device, std::unique_ptr<SimBase> &out) {
     std::unique_ptr<SimBase> impl;
     switch (desc.precision) {

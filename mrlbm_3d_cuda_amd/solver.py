@@ -147,6 +147,32 @@ class Simulation:
     def checkpoint_read(self, path):
         _check(self.lib.mrlbm_checkpoint_read(self.h, os.fsencode(path)))
 
+    # -- output
+    def default_vti_layout(self):
+        return _abi.VTI_ANNULUS if self.desc.kind == _abi.CASE_ANNULUS else _abi.VTI_CARTESIAN
+
+    def vti_points(self, layout=None, vel_norm=0.0):
+        """The four float32 point arrays of a VTK output file, shape (4, nz, ny, nx); names in
+        _abi.VTI_FIELDS[layout]. vel_norm 0 selects the case's velocity norm."""
+        layout = self.default_vti_layout() if layout is None else layout
+        out = np.empty(4 * self.n, dtype=np.float32)
+        _check(self.lib.mrlbm_vti_points(self.h, layout, vel_norm, as_ptr(out, C.c_float)))
+        return out.reshape((4,) + self.shape)
+
+    def write_vti(self, directory, timestep=None, layout=None, vel_norm=0.0):
+        """Write directory/data<10000000 + timestep>.vti (timestep: steps done) and return its path."""
+        layout = self.default_vti_layout() if layout is None else layout
+        timestep = self.info().steps_done if timestep is None else timestep
+        _check(self.lib.mrlbm_write_vti(self.h, os.fsencode(directory), timestep, layout, vel_norm))
+        return os.path.join(directory, f"data{10000000 + timestep}.vti")
+
+    def info_text(self):
+        n = C.c_int64()
+        _check(self.lib.mrlbm_sim_info(self.h, None, 0, C.byref(n)))
+        buf = C.create_string_buffer(n.value + 1)
+        _check(self.lib.mrlbm_sim_info(self.h, buf, n.value + 1, C.byref(n)))
+        return buf.value.decode()
+
     # -- geometry
     def info(self):
         i = Info()
@@ -177,6 +203,12 @@ class Simulation:
         _check(self.lib.mrlbm_get_bcfluid_table(self.h, group, as_ptr(node, C.c_int64), as_ptr(tag, C.c_uint32),
                                                 as_ptr(fin, C.c_uint32), as_ptr(fout, C.c_uint32)))
         return dict(node=node, tag=tag, in_mask=fin, out_mask=fout)
+
+
+def write_master_pvd(directory, max_iter, save_every):
+    """Write directory/master.pvd listing data<10000000 + it>.vti for it = 0, save_every, ... <= max_iter."""
+    _check(lib().mrlbm_write_master_pvd(os.fsencode(directory), max_iter, save_every))
+    return os.path.join(directory, "master.pvd")
 
 
 class Batch:

@@ -41,6 +41,10 @@ _PROTOS = {
     "oracle_extrapolation_coeffs": (None, [C.c_double, _dp]),
     "oracle_surface_pressure": (C.c_int, [C.c_void_p, _dp]),
     "oracle_surface_pressure_extrapolation": (C.c_double, [C.c_double, C.c_double, _dp, _dp]),
+    "oracle_vti_points": (None, [C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_double,
+                                 C.c_double, _dp, _dp, _dp, _dp, _P(C.c_uint8), _P(C.c_float)]),
+    "oracle_write_vti": (C.c_int, [C.c_char_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32, _P(C.c_float)]),
+    "oracle_write_master_pvd": (C.c_int, [C.c_char_p, C.c_int32, C.c_int32]),
     "oracle_curved_solution": (None, [C.c_double, C.c_double, C.c_uint32, _dp, C.c_double, C.c_double, C.c_double, _dp]),
 }
 
@@ -213,3 +217,27 @@ def curved_solution(nx, ny, in_mask, prime7, rho_i, mxy_i, mxz_i):
     out = np.empty(10, np.float64)
     lib().oracle_curved_solution(nx, ny, in_mask, as_ptr(p, C.c_double), rho_i, mxy_i, mxz_i, as_ptr(out, C.c_double))
     return out
+
+
+# ---- VTK output
+
+def vti_points(layout, shape, rho0, xc, yc, vel_norm, rho_dev, ux, uy, uz, solid):
+    """Point arrays (4, nz, ny, nx) float32 of the reference writer from FP64 moments."""
+    nz, ny, nx = shape
+    arrs = [np.ascontiguousarray(a, np.float64).reshape(-1) for a in (rho_dev, ux, uy, uz)]
+    sol = np.ascontiguousarray(solid, np.uint8).reshape(-1)
+    out = np.empty(4 * nx * ny * nz, np.float32)
+    lib().oracle_vti_points(layout, nx, ny, nz, rho0, xc, yc, vel_norm, *(as_ptr(a, C.c_double) for a in arrs),
+                            as_ptr(sol, C.c_uint8), as_ptr(out, C.c_float))
+    return out.reshape((4,) + tuple(shape))
+
+
+def write_vti(path, layout, points):
+    """Write points (4, nz, ny, nx) as the reference writer's .vti file."""
+    p = np.ascontiguousarray(points, np.float32)
+    _, nz, ny, nx = p.shape
+    _check(lib().oracle_write_vti(os.fsencode(path), layout, nx, ny, nz, as_ptr(p, C.c_float)), "oracle_write_vti")
+
+
+def write_master_pvd(path, max_iter, save_every):
+    _check(lib().oracle_write_master_pvd(os.fsencode(path), max_iter, save_every), "oracle_write_master_pvd")

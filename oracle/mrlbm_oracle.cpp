@@ -16,6 +16,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <fstream>
 #include <memory>
 #include <string>
 #include <vector>
@@ -1095,6 +1096,90 @@ int oracle_get_info(oracle_sim *s, mrlbm_info *info) {
 int oracle_destroy(oracle_sim *s) {
     delete s;
     return MRLBM_OK;
+}
+
+// ---- VTK output, src/output/saveData.cu. `real` is double in the FP64 build, so RHO_0 + rho
+// is a double sum rounded to float (:105-106, :211-212), velocities are rounded to float and
+// NaN at SOLID nodes (:107-112, :220-221), and get_cylindrical (:180-208) runs in float with
+// dx = (float)x - XC evaluated in double (XC is a double constant) and rounded on assignment.
+void oracle_vti_points(int32_t layout, int32_t nx, int32_t ny, int32_t nz, double rho0, double xc, double yc,
+                       double vel_norm, const double *rho_dev, const double *ux, const double *uy,
+                       const double *uz, const uint8_t *solid, float *out4) {
+    const size_t n = size_t(nx) * size_t(ny) * size_t(nz);
+    float *rho_out = out4, *a1 = out4 + n, *a2 = out4 + 2 * n, *a3 = out4 + 3 * n;
+    auto cylindrical = [&](size_t idx, int x, int y, bool get_theta) -> float {
+        if (solid[idx]) return NAN;
+        const float vx = float(ux[idx]);
+        const float vy = float(uy[idx]);
+        const float dx = float(double(float(x)) - xc);
+        const float dy = float(double(float(y)) - yc);
+        const float r2 = dx * dx + dy * dy;
+        if (r2 < 1e-12f) return 0.0f;
+        const float inv_r = 1.0f / std::sqrt(r2);
+        const float cos_t = dx * inv_r;
+        const float sin_t = dy * inv_r;
+        if (get_theta) return (-vx * sin_t + vy * cos_t) / float(vel_norm);
+        return (vx * cos_t + vy * sin_t) / float(vel_norm);
+    };
+    for (int z = 0; z < nz; ++z)
+        for (int y = 0; y < ny; ++y)
+            for (int x = 0; x < nx; ++x) {
+                const size_t i = size_t(x) + size_t(nx) * (size_t(y) + size_t(ny) * size_t(z));
+                rho_out[i] = float(rho0 + rho_dev[i]);
+                if (layout == 0) {
+                    a1[i] = solid[i] ? NAN : float(ux[i]);
+                    a2[i] = solid[i] ? NAN : float(uy[i]);
+                    a3[i] = solid[i] ? NAN : float(uz[i]);
+                } else {
+                    a1[i] = solid[i] ? NAN : float(uz[i]);
+                    a2[i] = cylindrical(i, x, y, true);
+                    a3[i] = cylindrical(i, x, y, false);
+                }
+            }
+}
+
+// write_vti_3d / write_vti_3d_annulus file layout (saveData.cu:42-116, 118-230): XML header with
+// four appended Float32 arrays at offsets k * (8 + 4n), '_', then per array its UInt64 byte
+// count and the values in x-fastest order, then the closing tags.
+int oracle_write_vti(const char *path, int32_t layout, int32_t nx, int32_t ny, int32_t nz, const float *pts4) {
+    static const char *names[2][4] = {{"rho", "ux", "uy", "uz"}, {"rho", "uz", "u_theta", "u_radial"}};
+    std::ofstream f(path, std::ios::binary);
+    if (!f) return fail(MRLBM_ERR_IO, "oracle: cannot open vti file");
+    const uint64_t points = uint64_t(nx) * ny * nz;
+    const uint64_t bytes = points * sizeof(float);
+    f << "<?xml version=\"1.0\"?>\n"
+      << "<VTKFile type=\"ImageData\" version=\"1.0\" byte_order=\"LittleEndian\" header_type=\"UInt64\">\n"
+      << "  <ImageData WholeExtent=\"0 " << nx - 1 << " 0 " << ny - 1 << " 0 " << nz - 1
+      << "\" Origin=\"0 0 0\" Spacing=\"1 1 1\">\n"
+      << "    <Piece Extent=\"0 " << nx - 1 << " 0 " << ny - 1 << " 0 " << nz - 1 << "\">\n"
+      << "      <PointData>\n";
+    uint64_t offset = 0;
+    for (int k = 0; k < 4; ++k) {
+        f << "        <DataArray type=\"Float32\" Name=\"" << names[layout ? 1 : 0][k]
+          << "\" format=\"appended\" offset=\"" << offset << "\"/>\n";
+        offset += 8 + bytes;
+    }
+    f << "      </PointData>\n    </Piece>\n  </ImageData>\n  <AppendedData encoding=\"raw\">\n_";
+    for (int k = 0; k < 4; ++k) {
+        f.write(reinterpret_cast<const char *>(&bytes), 8);
+        f.write(reinterpret_cast<const char *>(pts4 + size_t(k) * points), std::streamsize(bytes));
+    }
+    f << "\n  </AppendedData>\n</VTKFile>\n";
+    return f ? MRLBM_OK : fail(MRLBM_ERR_IO, "oracle: vti write failed");
+}
+
+// write_master_pvd (saveData.cu:3-40): one DataSet per saved step, timestep = iter / MACR_SAVE.
+int oracle_write_master_pvd(const char *path, int32_t max_iter, int32_t save_every) {
+    if (save_every <= 0) return fail(MRLBM_ERR_INVALID, "oracle: save interval must be positive");
+    std::ofstream out(path);
+    if (!out) return fail(MRLBM_ERR_IO, "oracle: cannot open pvd file");
+    out << "<VTKFile type=\"Collection\" version=\"0.1\">\n\t<Collection>\n";
+    for (int iter = 0; iter <= max_iter; iter++)
+        if (iter % save_every == 0)
+            out << "\t\t<DataSet timestep=\"" << iter / save_every << "\" file=\"data" << (10000000 + iter)
+                << ".vti\" />\n";
+    out << "\t</Collection>\n</VTKFile>\n";
+    return out ? MRLBM_OK : fail(MRLBM_ERR_IO, "oracle: pvd write failed");
 }
 
 int32_t oracle_num_threads(void) { return omp_get_max_threads(); }

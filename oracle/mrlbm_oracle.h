@@ -52,6 +52,15 @@ void oracle_extrapolation_coeffs(double delta, double *c3);
 int oracle_surface_pressure(oracle_sim *s, double *out);
 /* xy8 = (xw, yw, x1, y1, x2, y2, x3, y3); the quadratic wall-pressure extrapolation */
 double oracle_surface_pressure_extrapolation(double xc, double yc, const double *xy8, const double *rho3);
+/* VTK output (src/output/saveData.cu) as the reference's FP64 build evaluates it: the point arrays
+ * of write_vti_3d (layout 0: rho, ux, uy, uz) or write_vti_3d_annulus (layout 1: rho, uz,
+ * u_theta, u_radial) from FP64 moments (rho_dev = rho - rho0; solid[i] != 0 marks SOLID nodes),
+ * the files themselves, and master.pvd. */
+void oracle_vti_points(int32_t layout, int32_t nx, int32_t ny, int32_t nz, double rho0, double xc, double yc,
+                       double vel_norm, const double *rho_dev, const double *ux, const double *uy,
+                       const double *uz, const uint8_t *solid, float *out4);
+int oracle_write_vti(const char *path, int32_t layout, int32_t nx, int32_t ny, int32_t nz, const float *pts4);
+int oracle_write_master_pvd(const char *path, int32_t max_iter, int32_t save_every);
 void oracle_curved_solution(double nx, double ny, uint32_t in_mask, const double *prime7,
                             double rho_i, double mxy_i, double mxz_i, double *out10);
 
