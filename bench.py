@@ -68,12 +68,18 @@ def copy_calibration(nbytes, reps=5):
     return 0
 
 
-def cpu_baseline(desc, steps=20, warmup=2):
+def cpu_baseline(desc, steps=0, warmup=2, target_s=12.0):
+    """The oracle timed on the host cores. steps = 0 sizes the sample to about target_s seconds
+    from the warm-up steps' duration."""
     from oracle import binding
 
     d = desc.copy()
     o = binding.Oracle(d, real_bytes=4).init()
-    o.step(warmup)
+    t0 = time.perf_counter()
+    o.step(warmup, _abi.STEP_FORCES)
+    per_step = (time.perf_counter() - t0) / warmup
+    if steps <= 0:
+        steps = int(min(2000, max(3, round(target_s / per_step))))
     t0 = time.perf_counter()
     o.step(steps, _abi.STEP_FORCES)
     dt = time.perf_counter() - t0
@@ -160,11 +166,12 @@ def bench_batch(args):
 
 
 def c5_cd_errors(descs, steps):
-    """Drag error of BF16 storage against FP64 on a subset of the C5 cases, both run as batches
-    for `steps` steps; mean Cd over the second half."""
+    """Drag error of BF16 storage against FP64 on a subset of the C5 cases, all run as batches
+    for `steps` steps; mean Cd over the second half. The errors of FP16 storage and of FP32 on the
+    same cases are reported beside it."""
     from mrlbm_3d_cuda_amd.solver import Batch
     cds = {}
-    for prec in (_abi.FP64, _abi.FP32_BF16):
+    for prec in (_abi.FP64, _abi.FP32_BF16, _abi.FP32_FP16, _abi.FP32):
         ds = []
         for d in descs:
             d = d.copy()
@@ -174,7 +181,13 @@ def c5_cd_errors(descs, steps):
             b.init().step(steps, forces=True)
             cds[prec] = [mean_cd(c.forces(0, steps), ds[i]) for i, c in enumerate(b.cases)]
     err = [abs(a - r) / abs(r) for a, r in zip(cds[_abi.FP32_BF16], cds[_abi.FP64])]
+    err32 = [abs(a - r) / abs(r) for a, r in zip(cds[_abi.FP32], cds[_abi.FP64])]
+    err16 = [abs(a - r) / abs(r) for a, r in zip(cds[_abi.FP32_FP16], cds[_abi.FP64])]
     return {"cases": len(descs), "steps": steps, "max_rel": float(max(err)), "mean_rel": float(np.mean(err)),
+            "per_case_rel": [round(float(e), 5) for e in err],
+            "fp16_storage_max_rel": float(max(err16)), "fp16_storage_mean_rel": float(np.mean(err16)),
+            "fp32_max_rel": float(max(err32)), "fp32_mean_rel": float(np.mean(err32)),
+            "fp32_per_case_rel": [round(float(e), 5) for e in err32],
             "cd_fp64_mean": float(np.mean(cds[_abi.FP64]))}
 
 
@@ -185,7 +198,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--precision", default="f32", choices=["f64", "f32", "bf16", "fp16"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=20)
+    ap.add_argument("--cpu-steps", type=int, default=0, help="CPU baseline steps (0: about 12 s of work)")
     ap.add_argument("--cd-steps", type=int, default=2000, help="c5: steps of the BF16-vs-FP64 drag check")
     ap.add_argument("--workload", default="c2", choices=["c2", "c3", "box", "c4", "c5"],
                     help="c2: confined cylinder 256x128x64 (default); c3: wake 512x256x256; "
