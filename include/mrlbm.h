@@ -100,6 +100,11 @@ enum mrlbm_field {
 /* compat_flags bits */
 #define MRLBM_COMPAT_INIT_M_ZERO 0x1u /* initial m = 0, as the reference's K9 computes (it reads
                                           the constant tables before they are uploaded) */
+#define MRLBM_COMPAT_MASS_STRONG 0x2u /* curved walls close with MassBC::Strong,
+                                          numerical_solution_strong_rotated (numerical_solution.cu:124-279);
+                                          default MassBC::Equilibrium (constants.cuh:17) */
+#define MRLBM_COMPAT_BCF_MASS_STRONG 0x4u /* BCFLUID density from the Strong quadratic
+                                              (BCF_MASS_CONSERV, boundaries.cuh:894-904) */
 
 /* mrlbm_step flags */
 #define MRLBM_STEP_FORCES 0x1u /* accumulate per-step Fx, Fy, Fz, mass flux (K5 window) */
@@ -227,6 +232,19 @@ int mrlbm_stats_reset(mrlbm_sim *sim);
 int mrlbm_get_surface_pressure(mrlbm_sim *sim, double *ps_mean, int64_t *samples);
 
 /*
+ * Text statistics files in the reference's formats:
+ *   mrlbm_write_forces   <- write_forces / write_mass_flux (src/statistics/stat_header.cuh:16-51):
+ *                           <dir>/forces.dat "iter Fx Fy Fz" and <dir>/mass_flux.dat "iter m",
+ *                           16 fixed decimals, for the force records first .. first + count - 1
+ *   mrlbm_write_pressure <- write_pressure (src/cases/cylinder/cylinder_kernels.cuh:251-285):
+ *                           <dir>/pressure.dat "theta z Cp" header, then per inner-wall node its
+ *                           angle in degrees [0, 360), z and the time-averaged wall pressure
+ * Files are overwritten; errors return MRLBM_ERR_IO.
+ */
+int mrlbm_write_forces(mrlbm_sim *sim, const char *dir, int64_t first, int32_t count);
+int mrlbm_write_pressure(mrlbm_sim *sim, const char *dir);
+
+/*
  * Visualisation output <- write_vti_3d / write_vti_3d_annulus / write_master_pvd
  * (src/output/saveData.cu:3-230, called from src/cases/cylinder/cylinder_kernels.cu:133 and
  * src/simulation.cu:8,133). The point arrays are computed on the device from the current moments:
@@ -274,7 +292,7 @@ typedef struct mrlbm_batch mrlbm_batch;
 
 typedef struct mrlbm_tune_result {
     int32_t bx, by;         /* fused tile */
-    int32_t variant;        /* 1 ring march, 3 register-window march */
+    int32_t variant;        /* 1 ring march, 3 / 4 register-window march with 1 / 2 LDS planes */
     int32_t lz;             /* z chunk */
     int32_t candidates;     /* configurations timed */
     int32_t reserved;

@@ -41,6 +41,8 @@ FIELDS = ("rho", "ux", "uy", "uz", "mxx", "myy", "mzz", "mxy", "mxz", "myz")
 NUM_FIELDS = 10
 
 COMPAT_INIT_M_ZERO = 0x1
+COMPAT_MASS_STRONG = 0x2
+COMPAT_BCF_MASS_STRONG = 0x4
 STEP_FORCES = 0x1
 STEP_STATS = 0x2
 
@@ -162,6 +164,8 @@ PRODUCT_PROTOTYPES = {
     "mrlbm_write_vti": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int32, C.c_int32, C.c_double]),
     "mrlbm_write_master_pvd": (C.c_int, [C.c_char_p, C.c_int32, C.c_int32]),
     "mrlbm_sim_info": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int64, P(C.c_int64)]),
+    "mrlbm_write_forces": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int64, C.c_int32]),
+    "mrlbm_write_pressure": (C.c_int, [C.c_void_p, C.c_char_p]),
     "mrlbm_last_error": (C.c_char_p, []),
     "mrlbm_abi_version": (C.c_int32, []),
 }

@@ -176,7 +176,7 @@ class Batch final : public BatchBase {
         std::vector<LaunchConfig> cands{def};
         const int tiles[3][2] = {{32, 8}, {32, 4}, {16, 8}};
         for (const auto &t : tiles)
-            for (int variant : {1, 3})
+            for (int variant : {1, 3, 4})
                 for (int lz : {4, 8, 16, 32}) {
                     LaunchConfig lc = def;
                     lc.xcd_remap = true;

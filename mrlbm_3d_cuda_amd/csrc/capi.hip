@@ -333,6 +333,64 @@ int mrlbm_write_master_pvd(const char *dir, int32_t max_iter, int32_t save_every
     return mrlbm::write_pvd_file(std::string(dir) + "/master.pvd", max_iter, save_every);
 }
 
+// Text statistics files: "<iter> <Fx> <Fy> <Fz>" to forces.dat and "<iter> <mass>" to
+// mass_flux.dat, 16 fixed decimals, one line per step record (write_forces / write_mass_flux,
+// src/statistics/stat_header.cuh:16-51).
+int mrlbm_write_forces(mrlbm_sim *s, const char *dir, int64_t first, int32_t count) {
+    DEV_CHECK(s);
+    if (!dir || !*dir || count < 0) return set_error(MRLBM_ERR_INVALID, "bad directory or count");
+    std::vector<double> f(size_t(4) * size_t(count));
+    if (count > 0)
+        if (int rc = s->impl->get_forces(first, count, f.data())) return rc;
+    const std::string base(dir);
+    FILE *ff = std::fopen((base + "/forces.dat").c_str(), "w");
+    if (!ff) return set_error(MRLBM_ERR_IO, "cannot write " + base + "/forces.dat");
+    FILE *fm = std::fopen((base + "/mass_flux.dat").c_str(), "w");
+    if (!fm) {
+        std::fclose(ff);
+        return set_error(MRLBM_ERR_IO, "cannot write " + base + "/mass_flux.dat");
+    }
+    bool ok = true;
+    for (int32_t k = 0; k < count && ok; ++k) {
+        const double *r = f.data() + 4 * size_t(k);
+        const long long it = static_cast<long long>(first + k);
+        ok = std::fprintf(ff, "%lld %.16f %.16f %.16f\n", it, r[0], r[1], r[2]) > 0 &&
+             std::fprintf(fm, "%lld %.16f\n", it, r[3]) > 0;
+    }
+    ok = (std::fclose(ff) == 0) && ok;
+    ok = (std::fclose(fm) == 0) && ok;
+    return ok ? MRLBM_OK : set_error(MRLBM_ERR_IO, "write failed in " + base);
+}
+
+// pressure.dat: header "theta z Cp", then per inner-wall node (wall-table order) its angle about
+// the centre in degrees [0, 360), its z and the time-averaged wall pressure, 12 significant
+// digits (write_pressure, src/cases/cylinder/cylinder_kernels.cuh:251-285).
+int mrlbm_write_pressure(mrlbm_sim *s, const char *dir) {
+    DEV_CHECK(s);
+    if (!dir || !*dir) return set_error(MRLBM_ERR_INVALID, "bad directory");
+    const mrlbm::Geometry &g = s->impl->g;
+    const mrlbm::WallTable &w = g.wall[0];
+    std::vector<double> ps(std::max<size_t>(w.node.size(), 1));
+    int64_t samples = 0;
+    if (int rc = s->impl->get_surface_pressure(ps.data(), &samples)) return rc;
+    const std::string path = std::string(dir) + "/pressure.dat";
+    FILE *fp = std::fopen(path.c_str(), "w");
+    if (!fp) return set_error(MRLBM_ERR_IO, "cannot write " + path);
+    const double pi = 3.14159265358979323846;
+    const mrlbm_case_desc &d = s->impl->d;
+    bool ok = std::fputs("theta z Cp\n", fp) >= 0;
+    for (size_t k = 0; k < w.node.size() && ok; ++k) {
+        const int64_t i = w.node[k];
+        const unsigned x = unsigned(i % g.nx), y = unsigned((i / g.nx) % g.ny), z = unsigned(i / (int64_t(g.nx) * g.ny));
+        const double dx = double(x) - d.xc, dy = double(y) - d.yc;
+        double theta = std::atan2(dy, dx) * 180.0 / pi;
+        if (theta < 0) theta += 360.0;
+        ok = std::fprintf(fp, "%.12g %u %.12g\n", theta, z, ps[k]) > 0;
+    }
+    ok = (std::fclose(fp) == 0) && ok;
+    return ok ? MRLBM_OK : set_error(MRLBM_ERR_IO, "write failed: " + path);
+}
+
 int mrlbm_sim_info(mrlbm_sim *s, char *buf, int64_t cap, int64_t *len) {
     SIM_CHECK(s);
     if (cap < 0 || (cap > 0 && !buf)) return set_error(MRLBM_ERR_INVALID, "bad buffer");

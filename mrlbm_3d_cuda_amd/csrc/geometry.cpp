@@ -63,19 +63,23 @@ uint8_t domain_type(const mrlbm_case_desc &d, int x, int y, int z) {
     return RT_BULK;
 }
 
-// Node-type grid with a one-node SOLID frame in x and y and periodic wrap in z, so that the
-// neighbour look-ups of the classification (load_neighbors, nodeClass.cuh:27-60) need no
-// bounds tests.
+// Node-type grid with a one-node SOLID frame in x and y, so that the neighbour look-ups of the
+// classification (load_neighbors, nodeClass.cuh:27-60) need no x/y bounds tests; z wraps when
+// periodic and reads SOLID beyond the end planes otherwise.
 struct TypeGrid {
     int nx, ny, nz, px, py;
+    bool pz;
     std::vector<uint8_t> t;
-    TypeGrid(int nx_, int ny_, int nz_) : nx(nx_), ny(ny_), nz(nz_), px(nx_ + 2), py(ny_ + 2) {
+    TypeGrid(int nx_, int ny_, int nz_, bool pz_) : nx(nx_), ny(ny_), nz(nz_), px(nx_ + 2), py(ny_ + 2), pz(pz_) {
         t.assign(size_t(px) * py * nz, RT_SOLID);
     }
     uint8_t &at(int x, int y, int z) { return t[size_t(x + 1) + size_t(px) * (size_t(y + 1) + size_t(py) * z)]; }
     uint8_t nb(int x, int y, int z, int q) const {
         int zn = z + LATTICE.cz[q];
-        zn = zn < 0 ? zn + nz : (zn >= nz ? zn - nz : zn);
+        if (zn < 0 || zn >= nz) {
+            if (!pz) return RT_SOLID;
+            zn = zn < 0 ? zn + nz : zn - nz;
+        }
         return t[size_t(x + 1 + LATTICE.cx[q]) + size_t(px) * (size_t(y + 1 + LATTICE.cy[q]) + size_t(py) * zn)];
     }
     // bitmask over octants that contain a neighbour of type `want`
@@ -130,10 +134,6 @@ int build_geometry(const mrlbm_case_desc &d, Geometry &g, std::string &err) {
     }
     const bool walls = d.kind != MRLBM_CASE_PERIODIC_BOX;
     const bool ann = d.kind == MRLBM_CASE_ANNULUS;
-    if (walls && !d.periodic_z) {
-        err = "curved-wall cases need a periodic z axis";
-        return MRLBM_ERR_UNSUPPORTED;
-    }
     if (walls && (!(d.r_inner > 0.0) || (ann && !(d.r_outer > d.r_inner)))) {
         err = "bad wall radii";
         return MRLBM_ERR_GEOMETRY;
@@ -144,7 +144,7 @@ int build_geometry(const mrlbm_case_desc &d, Geometry &g, std::string &err) {
     g.n = int64_t(d.nx) * d.ny * d.nz;
     const int NX = d.nx, NY = d.ny, NZ = d.nz;
 
-    TypeGrid tg(NX, NY, NZ);
+    TypeGrid tg(NX, NY, NZ, d.periodic_z != 0);
     for (int z = 0; z < NZ; ++z)
         for (int y = 0; y < NY; ++y)
             for (int x = 0; x < NX; ++x) tg.at(x, y, z) = walls ? domain_type(d, x, y, z) : RT_BULK;
@@ -172,9 +172,12 @@ int build_geometry(const mrlbm_case_desc &d, Geometry &g, std::string &err) {
             ya = std::max(0, int(std::ceil(d.yc - r_in)) - 4);
             yb = std::min(NY, int(std::floor(d.yc + r_in)) + 5);
         }
+        // without z periodicity the wall and BCFLUID scans skip the end planes
+        // (nodeClass.cuh:171-172, 220-221)
+        const int za = d.periodic_z ? 0 : 1, zb = d.periodic_z ? NZ : NZ - 1;
         // wall nodes: SOLID with BULK neighbours in more than two octants. Converting a node does
         // not change any other node's test (only SOLID centres and BULK neighbours count).
-        for (int z = 0; z < NZ; ++z)
+        for (int z = za; z < zb; ++z)
             for (int y = ya; y < yb; ++y)
                 for (int x = xa; x < xb; ++x) {
                     if (tg.at(x, y, z) != RT_SOLID) continue;
@@ -183,13 +186,28 @@ int build_geometry(const mrlbm_case_desc &d, Geometry &g, std::string &err) {
                 }
         // BCFLUID nodes: BULK with a SOLID neighbour, tagged by the octants holding SOLID
         std::vector<uint32_t> tag(g.n, 0);
-        for (int z = 0; z < NZ; ++z)
+        for (int z = za; z < zb; ++z)
             for (int y = ya; y < yb; ++y)
                 for (int x = xa; x < xb; ++x) {
                     if (tg.at(x, y, z) != RT_BULK || !tg.any_nb(x, y, z, RT_SOLID)) continue;
                     tag[g.idx(x, y, z)] = tg.octants_with(x, y, z, RT_SOLID);
                     tg.at(x, y, z) = (ann && radius(x, y) >= r_mid) ? RT_BCFLUID_OUTER : RT_BCFLUID_INNER;
                 }
+        // BCSOLID nodes (end planes without z periodicity): BACK/FRONT nodes with BULK neighbours
+        // in fewer than four octants, tagged by those octants (classify_bcsolid_nodes_triangular,
+        // cylinder nodeClass.cuh:254-287, annulus nodeClass.cuh:309-357). BACK/FRONT are not BULK,
+        // so converting one leaves the other tests unchanged.
+        if (!d.periodic_z)
+            for (int z = 0; z < NZ; z += std::max(1, NZ - 1))
+                for (int y = ya; y < yb; ++y)
+                    for (int x = xa; x < xb; ++x) {
+                        const uint8_t t = tg.at(x, y, z);
+                        if (t != RT_BACK && t != RT_FRONT) continue;
+                        const uint32_t bits = tg.octants_with(x, y, z, RT_BULK);
+                        if (popcount8(bits) >= 4) continue;
+                        tag[g.idx(x, y, z)] = bits;
+                        tg.at(x, y, z) = (ann && radius(x, y) >= r_mid) ? RT_BCSOLID_OUTER : RT_BCSOLID_INNER;
+                    }
         g.wall[0].radius = r_in;
         g.wall[1].radius = r_out;
         g.wall[1].fluid_dir = -1.0;
@@ -212,6 +230,8 @@ int build_geometry(const mrlbm_case_desc &d, Geometry &g, std::string &err) {
                         g.ref_index[i] = tag[i];
                         w.bcf_node.push_back(i);
                         w.bcf_tag.push_back(tag[i]);
+                    } else if (t == RT_BCSOLID_INNER || t == RT_BCSOLID_OUTER) {
+                        g.ref_index[i] = tag[i];
                     }
                 }
         for (int gi = 0; gi < g.ngroups; ++gi) {
@@ -319,6 +339,16 @@ int build_geometry(const mrlbm_case_desc &d, Geometry &g, std::string &err) {
                 case RT_BULK: code = NC_BULK; break;
                 case RT_INNER: code = NC_WALL; payload = g.ref_index[i]; break;
                 case RT_OUTER: code = NC_WALL; payload = uint32_t(g.wall[0].node.size()) + g.ref_index[i]; break;
+                case RT_BCSOLID_INNER:
+                case RT_BCSOLID_OUTER: {
+                    const uint32_t tg8 = g.ref_index[i];
+                    if (tg8 == 7 || tg8 == 11 || tg8 == 13 || tg8 == 14 || tg8 == 112 || tg8 == 176 || tg8 == 208 ||
+                        tg8 == 224) {
+                        code = NC_BCS;
+                        payload = tg8;
+                    }
+                    break;
+                }
                 case RT_BCFLUID_INNER:
                 case RT_BCFLUID_OUTER: {
                     const uint32_t tg8 = g.ref_index[i];
@@ -341,6 +371,24 @@ int build_geometry(const mrlbm_case_desc &d, Geometry &g, std::string &err) {
                         case RT_SOUTH_WEST: code = NC_SOUTH_WEST; break;
                         case RT_NORTH_EAST: code = NC_NORTH_EAST; break;
                         case RT_SOUTH_EAST: code = NC_SOUTH_EAST; break;
+                        case RT_FRONT: code = NC_FRONT; break;
+                        case RT_BACK: code = NC_BACK; break;
+                        case RT_NORTH_FRONT: code = NC_NORTH_FRONT; break;
+                        case RT_NORTH_BACK: code = NC_NORTH_BACK; break;
+                        case RT_SOUTH_FRONT: code = NC_SOUTH_FRONT; break;
+                        case RT_SOUTH_BACK: code = NC_SOUTH_BACK; break;
+                        case RT_WEST_FRONT: code = NC_WEST_FRONT; break;
+                        case RT_WEST_BACK: code = NC_WEST_BACK; break;
+                        case RT_EAST_FRONT: code = NC_EAST_FRONT; break;
+                        case RT_EAST_BACK: code = NC_EAST_BACK; break;
+                        case RT_NWF: code = NC_WEST_CORNER; payload = 0; break;
+                        case RT_NWB: code = NC_WEST_CORNER; payload = 1; break;
+                        case RT_SWF: code = NC_WEST_CORNER; payload = 2; break;
+                        case RT_SWB: code = NC_WEST_CORNER; payload = 3; break;
+                        case RT_NEF:
+                        case RT_NEB:
+                        case RT_SEF:
+                        case RT_SEB: code = NC_EAST_CORNER; break;
                         default: break;
                         }
                     }

@@ -39,6 +39,20 @@ enum NodeCode : uint32_t {
     NC_SOUTH_WEST = 10,
     NC_NORTH_EAST = 11,
     NC_SOUTH_EAST = 12,
+    // end planes without z periodicity (cylinder faces; BCSOLID in both curved-wall cases)
+    NC_FRONT = 13,
+    NC_BACK = 14,
+    NC_NORTH_FRONT = 15,
+    NC_NORTH_BACK = 16,
+    NC_SOUTH_FRONT = 17,
+    NC_SOUTH_BACK = 18,
+    NC_WEST_FRONT = 19,
+    NC_WEST_BACK = 20,
+    NC_EAST_FRONT = 21,
+    NC_EAST_BACK = 22,
+    NC_WEST_CORNER = 23,  // payload: 0 NWF, 1 NWB, 2 SWF, 3 SWB
+    NC_EAST_CORNER = 24,
+    NC_BCS = 25,  // BCSOLID closed form, payload: tag (7, 11, 13, 14, 112, 176, 208, 224)
 };
 constexpr uint32_t INFO_CODE_MASK = 0x1Fu;
 constexpr uint32_t INFO_DONOR = 0x20u;  // node's pre-collision moments are read by a wall stencil

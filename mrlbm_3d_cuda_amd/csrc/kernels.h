@@ -38,6 +38,7 @@ struct FusedArgs {
     const unsigned long long *__restrict__ step_counter;
     StoreScale<R> scale;
     R omega, rho0;
+    int bcf_strong;  // BCFLUID density from the Strong quadratic
 };
 
 // Arguments of the curved-wall + force pass (one thread per wall node, then one per BCFLUID).
@@ -65,6 +66,7 @@ struct WallArgs {
     StoreScale<R> scale;
     R omega, rho0;
     int forces;
+    int strong;                     // MassBC::Strong closure instead of Equilibrium
     int nblocks;                    // blocks of this case in the wall pass
     double *__restrict__ partials;  // 4 per block
     double *__restrict__ ring;      // 4 per step, FORCE_RING_STEPS records

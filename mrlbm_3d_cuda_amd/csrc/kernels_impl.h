@@ -121,7 +121,7 @@ __device__ __forceinline__ void boundary_node(const FusedArgs<R, S> &a, R *f, co
             incoming_rotated(t.wnx, t.wny, t.win, f, M);
             do_collide = false;  // finished by wall_step
         } else if (code == NC_BCF) {
-            bc_fluid(payload, t.bcfin, f, M);
+            bc_fluid(payload, t.bcfin, a.omega, a.bcf_strong != 0, f, M);
         } else if (code == NC_WEST) {
             bc_west(f, old, t.inlet, M);
         } else if (code == NC_NORTH) {
@@ -136,8 +136,34 @@ __device__ __forceinline__ void boundary_node(const FusedArgs<R, S> &a, R *f, co
             bc_east(f, old, oxm, M);
         } else if (code == NC_NORTH_EAST) {
             bc_east_edge<face::K_E & face::K_N, +1>(f, old, oxm, M);
-        } else {
+        } else if (code == NC_SOUTH_EAST) {
             bc_east_edge<face::K_E & face::K_S, -1>(f, old, oxm, M);
+        } else if (code == NC_FRONT) {
+            bc_zwall<face::K_F>(f, M);
+        } else if (code == NC_BACK) {
+            bc_zwall<face::K_B>(f, M);
+        } else if (code == NC_NORTH_FRONT) {
+            bc_yz_edge<face::K_N & face::K_F, +1>(f, a.omega, M);
+        } else if (code == NC_NORTH_BACK) {
+            bc_yz_edge<face::K_N & face::K_B, -1>(f, a.omega, M);
+        } else if (code == NC_SOUTH_FRONT) {
+            bc_yz_edge<face::K_S & face::K_F, -1>(f, a.omega, M);
+        } else if (code == NC_SOUTH_BACK) {
+            bc_yz_edge<face::K_S & face::K_B, +1>(f, a.omega, M);
+        } else if (code == NC_WEST_FRONT) {
+            bc_west_edge<face::K_W & face::K_F, -1, 8>(f, a.omega, M);
+        } else if (code == NC_WEST_BACK) {
+            bc_west_edge<face::K_W & face::K_B, +1, 8>(f, a.omega, M);
+        } else if (code == NC_EAST_FRONT) {
+            bc_east_edge<face::K_E & face::K_F, +1, 8>(f, old, oxm, M);
+        } else if (code == NC_EAST_BACK) {
+            bc_east_edge<face::K_E & face::K_B, -1, 8>(f, old, oxm, M);
+        } else if (code == NC_WEST_CORNER) {
+            bc_west_corner(payload, f, M);
+        } else if (code == NC_EAST_CORNER) {
+            bc_east_corner(old, oxm, M);
+        } else {
+            bc_solid(payload, a.omega, f, M);  // NC_BCS
         }
     }
     donor = store && (info & INFO_DONOR);
@@ -665,7 +691,9 @@ __device__ __forceinline__ void wall_body(const WallArgs<R, S> &a, int lb) {
 #pragma unroll
         for (int j = 0; j < 7; ++j) p[j] = c0 * vw[j] + c1 * vm1[sel[j]] + c2 * vm2[sel[j]];
         R out[10];
-        if constexpr (kExactOrder<R>) {
+        if (a.strong) {
+            wall_closure_strong(c, s, a.in_mask[i], a.out_mask[i], a.omega, p, M[0], M[7], M[8], out);
+        } else if constexpr (kExactOrder<R>) {
             wall_closure_loop(c, s, a.in_mask[i], p, M[0], M[7], M[8], out);
         } else {
             R k[WALL_CONSTS];

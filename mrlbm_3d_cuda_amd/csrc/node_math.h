@@ -300,6 +300,8 @@ constexpr uint32_t K_W = dir_mask([](int cx, int, int) { return cx <= 0; });
 constexpr uint32_t K_E = dir_mask([](int cx, int, int) { return cx >= 0; });
 constexpr uint32_t K_N = dir_mask([](int, int cy, int) { return cy >= 0; });
 constexpr uint32_t K_S = dir_mask([](int, int cy, int) { return cy <= 0; });
+constexpr uint32_t K_F = dir_mask([](int, int, int cz) { return cz >= 0; });
+constexpr uint32_t K_B = dir_mask([](int, int, int cz) { return cz <= 0; });
 constexpr uint32_t XY_P = dir_mask([](int cx, int cy, int) { return cx * cy > 0; });
 constexpr uint32_t XY_N = dir_mask([](int cx, int cy, int) { return cx * cy < 0; });
 constexpr uint32_t XZ_P = dir_mask([](int cx, int, int cz) { return cx * cz > 0; });
@@ -385,43 +387,166 @@ __device__ __forceinline__ void bc_ywall(const R *f, R *M) {
     M[9] = R(2) * myzI * rI / rho;
 }
 
-// inlet corners with the y walls (NORTH_WEST sgn = -1 ... SOUTH_WEST sgn = +1,
-// boundaries.cuh:360-403): sgn is the sign of c_x c_y of the known diagonal pair.
-template <uint32_t K, int SGN, class R>
-__device__ __forceinline__ void bc_west_edge(const R *f, R omega, R *M) {
+// no-slip z walls (BACK/FRONT, boundaries.cuh:316-359), without z periodicity
+template <uint32_t K, class R>
+__device__ __forceinline__ void bc_zwall(const R *f, R *M) {
     using namespace face;
     const R rI = msum<K>(f);
     const R inv = R(1) / rI;
-    const R mxyI = (SGN > 0 ? msum<K & XY_P>(f) : -msum<K & XY_N>(f)) * inv;
-    const R myzI = mseq<K, YZ_P, YZ_N>(f) * inv;
-    R rho;
-    if (SGN > 0)
-        rho = R(36) * rI * (R(1) - mxyI + mxyI * omega) / (R(24) + omega);
-    else
-        rho = R(36) * rI * (R(1) + mxyI - mxyI * omega) / (R(24) + omega);
+    const R mxzI = mdiff<K, XZ_P, XZ_N>(f) * inv;
+    const R myzI = mdiff<K, YZ_P, YZ_N>(f) * inv;
+    const R rho = R(6) * rI / R(5);
     M[0] = rho;
     M[1] = M[2] = M[3] = R(0);
     M[4] = M[5] = M[6] = R(0);
-    M[7] = (R(36) * mxyI * rI - R(SGN) * rho) / (R(9) * rho);
-    M[8] = R(0);
+    M[7] = R(0);
+    M[8] = R(2) * mxzI * rI / rho;
+    M[9] = R(2) * myzI * rI / rho;
+}
+
+// Inlet edges. With the y walls (NORTH_WEST sgn = -1, SOUTH_WEST sgn = +1,
+// boundaries.cuh:360-403) the edge shear is m_xy (SHEAR = 7); with the z walls (WEST_FRONT
+// sgn = -1, WEST_BACK sgn = +1, :404-447) it is m_xz (SHEAR = 8). sgn is the sign of the
+// product of the two face-normal velocity components of the known diagonal pair.
+template <uint32_t K, int SGN, int SHEAR = 7, class R>
+__device__ __forceinline__ void bc_west_edge(const R *f, R omega, R *M) {
+    using namespace face;
+    constexpr uint32_t P = SHEAR == 7 ? XY_P : XZ_P, N = SHEAR == 7 ? XY_N : XZ_N;
+    const R rI = msum<K>(f);
+    const R inv = R(1) / rI;
+    const R msI = (SGN > 0 ? msum<K & P>(f) : -msum<K & N>(f)) * inv;
+    const R myzI = mseq<K, YZ_P, YZ_N>(f) * inv;
+    R rho;
+    if (SGN > 0)
+        rho = R(36) * rI * (R(1) - msI + msI * omega) / (R(24) + omega);
+    else
+        rho = R(36) * rI * (R(1) + msI - msI * omega) / (R(24) + omega);
+    M[0] = rho;
+    M[1] = M[2] = M[3] = R(0);
+    M[4] = M[5] = M[6] = R(0);
+    M[7] = M[8] = R(0);
+    M[SHEAR] = (R(36) * msI * rI - R(SGN) * rho) / (R(9) * rho);
     M[9] = R(12) * myzI * rI / (R(5) * rho);
 }
 
-// outlet corners with the y walls (NORTH_EAST sgn = +1, SOUTH_EAST sgn = -1,
-// boundaries.cuh:448-513)
-template <uint32_t K, int SGN, class R>
+// Outlet edges: with the y walls (NORTH_EAST sgn = +1, SOUTH_EAST sgn = -1,
+// boundaries.cuh:448-511, SHEAR = 7) and the z walls (EAST_FRONT sgn = +1, EAST_BACK sgn = -1,
+// :512-577, SHEAR = 8); rho and u from the outlet blend.
+template <uint32_t K, int SGN, int SHEAR = 7, class R>
 __device__ __forceinline__ void bc_east_edge(const R *f, const R *old, const R *old_xm1, R *M) {
     using namespace face;
+    constexpr uint32_t P = SHEAR == 7 ? XY_P : XZ_P, N = SHEAR == 7 ? XY_N : XZ_N;
     const R rI = msum<K>(f);
     const R inv = R(1) / rI;
-    const R mxyI = (SGN > 0 ? msum<K & XY_P>(f) : -msum<K & XY_N>(f)) * inv;
+    const R msI = (SGN > 0 ? msum<K & P>(f) : -msum<K & N>(f)) * inv;
     const R myzI = mseq<K, YZ_P, YZ_N>(f) * inv;
     outlet_blend(old, old_xm1, M);
     const R rho = M[0];
     M[4] = M[5] = M[6] = R(0);
-    M[7] = (R(36) * mxyI * rI - R(SGN) * rho) / (R(9) * rho);
-    M[8] = R(0);
+    M[7] = M[8] = R(0);
+    M[SHEAR] = (R(36) * msI * rI - R(SGN) * rho) / (R(9) * rho);
     M[9] = (R(12) * myzI * rI) / (R(5) * rho);
+}
+
+// y-wall / z-wall edges (NORTH_FRONT and SOUTH_BACK sgn = +1, NORTH_BACK and SOUTH_FRONT
+// sgn = -1, boundaries.cuh:578-661): only m_yz survives.
+template <uint32_t K, int SGN, class R>
+__device__ __forceinline__ void bc_yz_edge(const R *f, R omega, R *M) {
+    using namespace face;
+    const R rI = msum<K>(f);
+    const R myzI = (SGN > 0 ? msum<K & YZ_P>(f) : -msum<K & YZ_N>(f)) * (R(1) / rI);
+    R rho;
+    if (SGN > 0)
+        rho = R(36) * rI * (R(1) - myzI + myzI * omega) / (R(24) + omega);
+    else
+        rho = R(36) * rI * (R(1) + myzI - myzI * omega) / (R(24) + omega);
+    M[0] = rho;
+    M[1] = M[2] = M[3] = R(0);
+    M[4] = M[5] = M[6] = R(0);
+    M[7] = M[8] = R(0);
+    M[9] = (R(36) * myzI * rI - R(SGN) * rho) / (R(9) * rho);
+}
+
+// Inlet corners (boundaries.cuh:662-733): rho = 216 rho_I / 125 at rest; corner 0..3 = NWF,
+// NWB, SWF, SWB.
+template <class R>
+__device__ __forceinline__ void bc_west_corner(uint32_t corner, const R *f, R *M) {
+    using namespace face;
+    R rI;
+    if (corner == 0)
+        rI = msum<K_W & K_N & K_F>(f);
+    else if (corner == 1)
+        rI = msum<K_W & K_N & K_B>(f);
+    else if (corner == 2)
+        rI = msum<K_W & K_S & K_F>(f);
+    else
+        rI = msum<K_W & K_S & K_B>(f);
+    M[0] = R(216) * rI / R(125);
+#pragma unroll
+    for (int k = 1; k < 10; ++k) M[k] = R(0);
+}
+
+// Outlet corners (boundaries.cuh:734-841): outlet blend, second moments zero.
+template <class R>
+__device__ __forceinline__ void bc_east_corner(const R *old, const R *old_xm1, R *M) {
+    outlet_blend(old, old_xm1, M);
+#pragma unroll
+    for (int k = 4; k < 10; ++k) M[k] = R(0);
+}
+
+template <int... Qs>
+constexpr uint32_t qbits() {
+    return ((1u << Qs) | ...);
+}
+
+// BCSOLID end-wall nodes next to a curved wall (bcsolid_boundary_condition,
+// cases/cylinder/boundaries.cuh:1039-1214): known set K per tag, m_xz and m_yz from signed
+// sums over K, then one of four closed-form families (tags 7/224, 11/208, 13/176, 14/112).
+template <uint32_t K, int FAM, class R>
+__device__ __forceinline__ void bc_solid_tag(R omega, const R *f, R *M) {
+    using namespace face;
+    const R rI = msum<K>(f);
+    const R inv = R(1.0) / rI;
+    const R mxzI = mseq<K, XZ_P, XZ_N>(f) * inv;
+    const R myzI = mseq<K, YZ_P, YZ_N>(f) * inv;
+    const R om = omega;
+    R rho, mxz, myz;
+    if constexpr (FAM == 0) {
+        rho = (R(108) * (R(10) + mxzI * (R(-1) + om) + myzI * (R(-1) + om)) * rI) / (R(874) + om);
+        mxz = (R(198) * mxzI * rI + R(18) * myzI * rI - rho) / (R(90.) * rho);
+        myz = (R(18) * mxzI * rI + R(198) * myzI * rI - rho) / (R(90.) * rho);
+    } else if constexpr (FAM == 1) {
+        rho = (R(108) * (R(10) + mxzI + myzI * (R(-1) + om) - mxzI * om) * rI) / (R(874) + om);
+        mxz = (R(198) * mxzI * rI - R(18) * myzI * rI + rho) / (R(90.) * rho);
+        myz = -(R(18) * mxzI * rI - R(198) * myzI * rI + rho) / (R(90.) * rho);
+    } else if constexpr (FAM == 2) {
+        rho = (R(108) * (R(10) + myzI + mxzI * (R(-1) + om) - myzI * om) * rI) / (R(874) + om);
+        mxz = -(R(-198) * mxzI * rI + R(18) * myzI * rI + rho) / (R(90.) * rho);
+        myz = (R(-18) * mxzI * rI + R(198) * myzI * rI + rho) / (R(90.) * rho);
+    } else {
+        rho = (R(-108) * (R(-10) + mxzI * (R(-1) + om) + myzI * (R(-1) + om)) * rI) / (R(874) + om);
+        mxz = (R(198) * mxzI * rI + R(18) * myzI * rI + rho) / (R(90.) * rho);
+        myz = (R(18) * mxzI * rI + R(198) * myzI * rI + rho) / (R(90.) * rho);
+    }
+    M[0] = rho;
+#pragma unroll
+    for (int k = 1; k < 8; ++k) M[k] = R(0);
+    M[8] = mxz;
+    M[9] = myz;
+}
+
+template <class R>
+__device__ __forceinline__ void bc_solid(uint32_t tag, R omega, const R *f, R *M) {
+    switch (tag) {
+    case 7: bc_solid_tag<qbits<0, 1, 2, 3, 4, 5, 7, 9, 11, 13, 14, 16, 18, 19, 23, 25>(), 0>(omega, f, M); break;
+    case 11: bc_solid_tag<qbits<0, 1, 2, 3, 4, 5, 7, 8, 9, 11, 14, 16, 18, 19, 22, 25>(), 1>(omega, f, M); break;
+    case 13: bc_solid_tag<qbits<0, 1, 2, 3, 4, 5, 7, 8, 9, 11, 13, 16, 18, 19, 22, 23>(), 2>(omega, f, M); break;
+    case 14: bc_solid_tag<qbits<0, 1, 2, 3, 4, 5, 8, 9, 11, 13, 14, 16, 18, 22, 23, 25>(), 3>(omega, f, M); break;
+    case 112: bc_solid_tag<qbits<0, 1, 2, 3, 4, 6, 7, 10, 12, 13, 14, 15, 17, 21, 24, 26>(), 3>(omega, f, M); break;
+    case 176: bc_solid_tag<qbits<0, 1, 2, 3, 4, 6, 7, 8, 10, 12, 14, 15, 17, 20, 21, 24>(), 2>(omega, f, M); break;
+    case 208: bc_solid_tag<qbits<0, 1, 2, 3, 4, 6, 7, 8, 10, 12, 13, 15, 17, 20, 21, 26>(), 1>(omega, f, M); break;
+    default: bc_solid_tag<qbits<0, 1, 2, 3, 4, 6, 8, 10, 12, 13, 14, 15, 17, 20, 24, 26>(), 0>(omega, f, M); break;
+    }
 }
 
 // ---------------------------------------------------------------- masked lattice-frame sums
@@ -460,7 +585,7 @@ __device__ __forceinline__ MaskedSums<R> masked_sums(uint32_t mask, const R *f) 
 // and (-,-) for 136, each tag's formulas are tag 17's applied to (sx ux, sy uy, sx sy mxy,
 // sx mxz, sy myz), with the outputs mapped back the same way.
 template <class R>
-__device__ __forceinline__ void bc_fluid(uint32_t tag, uint32_t in_mask, const R *f, R *M) {
+__device__ __forceinline__ void bc_fluid(uint32_t tag, uint32_t in_mask, R omega, bool strong, const R *f, R *M) {
     const R third = R(1) / R(3);
     const R sx = (tag == 34u || tag == 136u) ? R(-1) : R(1);
     const R sy = (tag == 68u || tag == 136u) ? R(-1) : R(1);
@@ -508,14 +633,27 @@ __device__ __forceinline__ void bc_fluid(uint32_t tag, uint32_t in_mask, const R
         mxz = sx * (S.pxz * inv);
         myz = sy * (S.pyz * inv);
     }
-    // density: root of a rho^2 + b rho + c = 0 closest to rho_I (boundaries.cuh:905-915)
-    const R a = R(-9998);
-    const R b = R(6) * rI * (R(1734) + R(66) * mxx + R(198) * mxy + R(66) * myy + R(253) * ux + R(253) * uy);
-    const R c = R(3) * (rI * rI) *
-                (R(45) * (mxx * mxx) + R(405) * (mxy * mxy) + R(45) * (myy * myy) + R(345) * myy * ux +
-                 R(589) * (ux * ux) + R(345) * myy * uy + R(1467) * ux * uy + R(589) * (uy * uy) +
-                 R(45) * mxy * (R(6) * myy + R(23) * (ux + uy)) +
-                 R(15) * mxx * (R(18) * mxy + R(6) * myy + R(23) * (ux + uy)));
+    // density: root of a rho^2 + b rho + c = 0 closest to rho_I, Equilibrium form
+    // (boundaries.cuh:905-915) or Strong form (:894-904; written times -1 for tags 34 and 68,
+    // which swaps the labels of the roots, so a, b, c carry that sign)
+    const R poly = R(45) * (mxx * mxx) + R(405) * (mxy * mxy) + R(45) * (myy * myy) + R(345) * myy * ux +
+                   R(589) * (ux * ux) + R(345) * myy * uy + R(1467) * ux * uy + R(589) * (uy * uy) +
+                   R(45) * mxy * (R(6) * myy + R(23) * (ux + uy)) +
+                   R(15) * mxx * (R(18) * mxy + R(6) * myy + R(23) * (ux + uy));
+    R a, b, c;
+    if (strong) {
+        const R sg = (tag == 34u || tag == 68u) ? R(-1) : R(1);
+        a = sg * (R(-9996) - R(206) * omega);
+        b = sg * (R(-6) * rI *
+                  (R(12) * mxx * (R(-17) + R(20) * omega) -
+                   R(17) * (R(102) + R(36) * mxy + R(12) * myy - R(5) * ux - R(5) * uy) +
+                   omega * (R(720) * mxy + R(240) * myy + R(53) * (ux + uy))));
+        c = sg * (R(3) * omega * (rI * rI) * poly);
+    } else {
+        a = R(-9998);
+        b = R(6) * rI * (R(1734) + R(66) * mxx + R(198) * mxy + R(66) * myy + R(253) * ux + R(253) * uy);
+        c = R(3) * (rI * rI) * poly;
+    }
     const R disc = fmax(b * b - R(4) * a * c, R(0));
     const R sq = sqrt(disc);
     const R rho1 = (-b + sq) / (R(2) * a);
@@ -720,6 +858,94 @@ __device__ __forceinline__ void wall_closure_loop(R nx, R ny, uint32_t in_mask, 
     M[8] = mxzp * c - myz * s;
     M[9] = mxzp * s + myz * c;
     M[0] = rhoI / E;
+}
+
+// MassBC::Strong form of the closure (numerical_solution_strong_rotated,
+// boundary/numerical_solution.cu:124-279): A', E' and B' over the outgoing directions, D' and F'
+// over the incoming ones, F' corrected by (1 - omega) m'_I B', G' = (1 - omega) A' + omega E',
+// and rho = rho_I / (G' + (1 - omega) m' : B'), so that the collided populations leaving along
+// the outgoing directions carry rho_I. Per-direction loop in the FP64 evaluation order.
+template <class R>
+__device__ __forceinline__ void wall_closure_strong(R nx, R ny, uint32_t in_mask, uint32_t out_mask, R omega,
+                                                    const R *p, R rhoI, R mxyI, R mxzI, R *M) {
+    const R c = nx, s = ny, third = R(1) / R(3);
+    const R s2 = R(2) * s * c, c2 = c * c - s * s;
+    const R ux = p[0], uy = p[1], uz = p[2], mxx = p[3], myy = p[4], mzz = p[5], myz = p[6];
+    R A = 0, E = 0, B11 = 0, B22 = 0, B33 = 0, B12 = 0, B13 = 0, B23 = 0, Dxy = 0, Dxz = 0;
+    R F11a = 0, F22a = 0, F33a = 0, F12a = 0, F13a = 0, F23a = 0;
+    R F11b = 0, F22b = 0, F33b = 0, F12b = 0, F13b = 0, F23b = 0;
+    for (int q = 0; q < Q; ++q) {
+        const bool o = out_mask & (1u << q), in = in_mask & (1u << q);
+        if (!o && !in) continue;
+        const R cx = R(LATTICE.cx[q]), cy = R(LATTICE.cy[q]), cz = R(LATTICE.cz[q]);
+        const R xp = cx * c + cy * s, yp = -cx * s + cy * c, zp = cz;
+        const R Hxx = xp * xp - third, Hyy = yp * yp - third, Hzz = zp * zp - third;
+        const R Hxy = xp * yp, Hxz = xp * zp, Hyz = yp * zp;
+        const R wq = R(weight(q));
+        const R cf = R(4.5) * wq;
+        const R Ai = wq * (R(1) + R(3) * (ux * xp + uy * yp + uz * zp));
+        if (o) {
+            A += Ai;
+            E += Ai + cf * (ux * ux * Hxx + uy * uy * Hyy + uz * uz * Hzz + R(2) * ux * uy * Hxy +
+                            R(2) * ux * uz * Hxz + R(2) * uy * uz * Hyz);
+            B11 += cf * Hxx;
+            B22 += cf * Hyy;
+            B33 += cf * Hzz;
+            B12 += cf * Hxy;
+            B13 += cf * Hxz;
+            B23 += cf * Hyz;
+        }
+        if (in) {
+            Dxy += Ai * Hxy;
+            Dxz += Ai * Hxz;
+            F11a += (cf * Hxx) * Hxy;
+            F22a += (cf * Hyy) * Hxy;
+            F33a += (cf * Hzz) * Hxy;
+            F12a += (cf * Hxy) * Hxy;
+            F13a += (cf * Hxz) * Hxy;
+            F23a += (cf * Hyz) * Hxy;
+            F11b += (cf * Hxx) * Hxz;
+            F22b += (cf * Hyy) * Hxz;
+            F33b += (cf * Hzz) * Hxz;
+            F12b += (cf * Hxy) * Hxz;
+            F13b += (cf * Hxz) * Hxz;
+            F23b += (cf * Hyz) * Hxz;
+        }
+    }
+    const R om1 = R(1) - omega;
+    const R G = om1 * A + omega * E;
+    const R oxy = om1 * mxyI, oxz = om1 * mxzI;
+    F11a -= oxy * B11;
+    F22a -= oxy * B22;
+    F33a -= oxy * B33;
+    F12a -= oxy * B12;
+    F13a -= oxy * B13;
+    F23a -= oxy * B23;
+    F11b -= oxz * B11;
+    F22b -= oxz * B22;
+    F33b -= oxz * B33;
+    F12b -= oxz * B12;
+    F13b -= oxz * B13;
+    F23b -= oxz * B23;
+    const R a1 = R(2) * F12a, b1 = R(2) * F13a;
+    const R c1 = (G * mxyI - Dxy) - mxx * F11a - myy * F22a - mzz * F33a - R(2) * myz * F23a;
+    const R a2 = R(2) * F12b, b2 = R(2) * F13b;
+    const R cc = (G * mxzI - Dxz) - mxx * F11b - myy * F22b - mzz * F33b - R(2) * myz * F23b;
+    const R inv = R(1) / (a2 * b1 - a1 * b2);
+    const R mxyp = (b1 * cc - b2 * c1) * inv;
+    const R mxzp = (a2 * c1 - a1 * cc) * inv;
+    M[1] = ux * c - uy * s;
+    M[2] = ux * s + uy * c;
+    M[3] = uz;
+    M[4] = mxx * c * c + myy * s * s - mxyp * s2;
+    M[5] = mxx * s * s + myy * c * c + mxyp * s2;
+    M[6] = mzz;
+    M[7] = (mxx - myy) * R(0.5) * s2 + mxyp * c2;
+    M[8] = mxzp * c - myz * s;
+    M[9] = mxzp * s + myz * c;
+    const R den = G + om1 * (mxx * B11 + myy * B22 + mzz * B33 + R(2) * mxyp * B12 + R(2) * mxzp * B13 +
+                             R(2) * myz * B23);
+    M[0] = rhoI / den;
 }
 
 // Momentum and mass carried by the populations of M selected by mask (compute_forces_mass,

@@ -259,7 +259,9 @@ class Sim final : public SimBase {
         LaunchConfig lc;
         lc.bx = sizeof(R) == 8 ? 16 : 32;
         lc.by = 8;
-        lc.variant = Conv<S>::scaled ? 3 : 1;
+        // register-window march with two LDS planes for FP32 arithmetic: one barrier per plane
+        // and no register cap (the one-plane form spills at its three-waves-per-SIMD bound)
+        lc.variant = sizeof(R) == 8 ? 1 : 4;
         // z chunks short enough for about 1024 work items (a block's plane-by-plane march is
         // latency-bound, so small grids need many short chunks), at most 32 planes
         const int64_t tiles = int64_t((g.nx + lc.bx - 1) / lc.bx) * ((g.ny + lc.by - 1) / lc.by);
@@ -448,6 +450,7 @@ class Sim final : public SimBase {
         a.step_counter = stochastic_ ? d_step_counter_ : nullptr;
         a.scale = scale_;
         a.omega = omega_;
+        a.bcf_strong = (d.compat_flags & MRLBM_COMPAT_BCF_MASS_STRONG) ? 1 : 0;
         a.rho0 = rho0_;
         return a;
     }
@@ -478,6 +481,7 @@ class Sim final : public SimBase {
             for (int k = 0; k < 3; ++k) w.uw[gi][k] = R(g.wall[gi].uw[k]);
         w.scale = scale_;
         w.omega = omega_;
+        w.strong = (d.compat_flags & MRLBM_COMPAT_MASS_STRONG) ? 1 : 0;
         w.rho0 = rho0_;
         w.forces = forces ? 1 : 0;
         w.nblocks = wall_blocks(nwall_, forces ? nbcf_ : 0);

@@ -166,6 +166,14 @@ class Simulation:
         _check(self.lib.mrlbm_write_vti(self.h, os.fsencode(directory), timestep, layout, vel_norm))
         return os.path.join(directory, f"data{10000000 + timestep}.vti")
 
+    def write_forces(self, directory, first, count):
+        """forces.dat and mass_flux.dat of the records first .. first + count - 1."""
+        _check(self.lib.mrlbm_write_forces(self.h, os.fsencode(directory), first, count))
+
+    def write_pressure(self, directory):
+        """pressure.dat: angle, z and mean wall pressure per inner-wall node."""
+        _check(self.lib.mrlbm_write_pressure(self.h, os.fsencode(directory)))
+
     def info_text(self):
         n = C.c_int64()
         _check(self.lib.mrlbm_sim_info(self.h, None, 0, C.byref(n)))

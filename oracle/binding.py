@@ -45,6 +45,9 @@ _PROTOS = {
                                  C.c_double, _dp, _dp, _dp, _dp, _P(C.c_uint8), _P(C.c_float)]),
     "oracle_write_vti": (C.c_int, [C.c_char_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32, _P(C.c_float)]),
     "oracle_write_master_pvd": (C.c_int, [C.c_char_p, C.c_int32, C.c_int32]),
+    "oracle_curved_solution_strong": (None, [C.c_double, C.c_double, C.c_uint32, C.c_uint32, C.c_double, _dp,
+                                             C.c_double, C.c_double, C.c_double, _dp]),
+    "oracle_bcfluid": (C.c_int, [C.c_uint32, C.c_uint32, C.c_double, C.c_int32, _dp, _dp]),
     "oracle_curved_solution": (None, [C.c_double, C.c_double, C.c_uint32, _dp, C.c_double, C.c_double, C.c_double, _dp]),
 }
 
@@ -210,6 +213,21 @@ def extrapolation_coeffs(delta):
     c = np.empty(3, np.float64)
     lib().oracle_extrapolation_coeffs(delta, as_ptr(c, C.c_double))
     return c
+
+
+def curved_solution_strong(nx, ny, in_mask, out_mask, omega, prime7, rho_i, mxy_i, mxz_i):
+    p = np.ascontiguousarray(prime7, np.float64)
+    out = np.empty(10, np.float64)
+    lib().oracle_curved_solution_strong(nx, ny, in_mask, out_mask, omega, as_ptr(p, C.c_double), rho_i, mxy_i, mxz_i,
+                                        as_ptr(out, C.c_double))
+    return out
+
+
+def bcfluid(tag, in_mask, omega, strong, f27):
+    f = np.ascontiguousarray(f27, np.float64)
+    out = np.empty(10, np.float64)
+    ok = lib().oracle_bcfluid(tag, in_mask, omega, int(strong), as_ptr(f, C.c_double), as_ptr(out, C.c_double))
+    return out if ok else None
 
 
 def curved_solution(nx, ny, in_mask, prime7, rho_i, mxy_i, mxz_i):

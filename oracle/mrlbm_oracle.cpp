@@ -267,16 +267,229 @@ bool face_bc(uint32_t type, R omega, R u_in, const R *f, const R *old, const R *
         myz = (R(12) * myzI * rI) / (R(5) * rho);
         return true;
     }
+    case T_BACK: {  // boundaries.cuh:316-337, the z = 0 no-slip wall
+        const R rI = sum_pops(f, {0, 1, 2, 3, 4, 6, 7, 8, 10, 12, 13, 14, 15, 17, 20, 21, 24, 26});
+        const R inv = R(1) / rI;
+        const R mxzI = ((f[10] + f[20] + f[24]) - (f[15] + f[21] + f[26])) * inv;
+        const R myzI = ((f[12] + f[20] + f[26]) - (f[17] + f[21] + f[24])) * inv;
+        rho = R(6) * rI / R(5);
+        ux = uy = uz = 0;
+        mxx = myy = mzz = 0;
+        mxy = 0;
+        mxz = R(2) * mxzI * rI / rho;
+        myz = R(2) * myzI * rI / rho;
+        return true;
+    }
+    case T_FRONT: {  // boundaries.cuh:338-359, the z = NZ-1 no-slip wall
+        const R rI = sum_pops(f, {0, 1, 2, 3, 4, 5, 7, 8, 9, 11, 13, 14, 16, 18, 19, 22, 23, 25});
+        const R inv = R(1) / rI;
+        const R mxzI = ((f[9] + f[19] + f[23]) - (f[16] + f[22] + f[25])) * inv;
+        const R myzI = ((f[11] + f[19] + f[25]) - (f[18] + f[22] + f[23])) * inv;
+        rho = R(6) * rI / R(5);
+        ux = uy = uz = 0;
+        mxx = myy = mzz = 0;
+        mxy = 0;
+        mxz = R(2) * mxzI * rI / rho;
+        myz = R(2) * myzI * rI / rho;
+        return true;
+    }
+    case T_WEST_FRONT: {  // boundaries.cuh:404-425
+        const R rI = sum_pops(f, {0, 2, 3, 4, 5, 8, 11, 14, 16, 18, 22, 25});
+        const R inv = R(1) / rI;
+        const R mxzI = -(f[16] + f[22] + f[25]) * inv;
+        const R myzI = (f[11] - f[18] - f[22] + f[25]) * inv;
+        rho = R(36) * rI * (R(1) + mxzI - mxzI * omega) / (R(24) + omega);
+        ux = uy = uz = 0;
+        mxx = myy = mzz = 0;
+        mxy = 0;
+        mxz = (R(36) * mxzI * rI + rho) / (R(9) * rho);
+        myz = R(12) * myzI * rI / (R(5) * rho);
+        return true;
+    }
+    case T_WEST_BACK: {  // boundaries.cuh:426-447
+        const R rI = sum_pops(f, {0, 2, 3, 4, 6, 8, 10, 12, 14, 17, 20, 24});
+        const R inv = R(1) / rI;
+        const R mxzI = (f[10] + f[20] + f[24]) * inv;
+        const R myzI = (f[12] - f[17] + f[20] - f[24]) * inv;
+        rho = R(36) * rI * (R(1) - mxzI + mxzI * omega) / (R(24) + omega);
+        ux = uy = uz = 0;
+        mxx = myy = mzz = 0;
+        mxy = 0;
+        mxz = (R(36) * mxzI * rI - rho) / (R(9) * rho);
+        myz = R(12) * myzI * rI / (R(5) * rho);
+        return true;
+    }
+    case T_EAST_FRONT: {  // boundaries.cuh:512-544 (outlet blend for rho and u)
+        const R rI = sum_pops(f, {0, 1, 3, 4, 5, 7, 9, 11, 13, 18, 19, 23});
+        const R inv = R(1) / rI;
+        const R mxzI = (f[9] + f[19] + f[23]) * inv;
+        const R myzI = (f[11] - f[18] + f[19] - f[23]) * inv;
+        neumann(rho, ux, uy, uz);
+        mxx = myy = mzz = 0;
+        mxy = 0;
+        mxz = (R(36) * mxzI * rI - rho) / (R(9) * rho);
+        myz = (R(12) * myzI * rI) / (R(5) * rho);
+        return true;
+    }
+    case T_EAST_BACK: {  // boundaries.cuh:545-577
+        const R rI = sum_pops(f, {0, 1, 3, 4, 6, 7, 12, 13, 15, 17, 21, 26});
+        const R inv = R(1) / rI;
+        const R mxzI = -(f[15] + f[21] + f[26]) * inv;
+        const R myzI = (f[12] - f[17] - f[21] + f[26]) * inv;
+        neumann(rho, ux, uy, uz);
+        mxx = myy = mzz = 0;
+        mxy = 0;
+        mxz = (R(36) * mxzI * rI + rho) / (R(9) * rho);
+        myz = (R(12) * myzI * rI) / (R(5) * rho);
+        return true;
+    }
+    case T_NORTH_FRONT:    // boundaries.cuh:578-598
+    case T_NORTH_BACK:     // :599-619
+    case T_SOUTH_FRONT:    // :620-640
+    case T_SOUTH_BACK: {   // :641-661
+        R rI, myzI;
+        if (type == T_NORTH_FRONT) {
+            rI = sum_pops(f, {0, 1, 2, 3, 5, 7, 9, 11, 14, 16, 19, 25});
+            myzI = (f[11] + f[19] + f[25]) * (R(1) / rI);
+        } else if (type == T_NORTH_BACK) {
+            rI = sum_pops(f, {0, 1, 2, 3, 6, 7, 10, 14, 15, 17, 21, 24});
+            myzI = -(f[17] + f[21] + f[24]) * (R(1) / rI);
+        } else if (type == T_SOUTH_FRONT) {
+            rI = sum_pops(f, {0, 1, 2, 4, 5, 8, 9, 13, 16, 18, 22, 23});
+            myzI = -(f[18] + f[22] + f[23]) * (R(1) / rI);
+        } else {
+            rI = sum_pops(f, {0, 1, 2, 4, 6, 8, 10, 12, 13, 15, 20, 26});
+            myzI = (f[12] + f[20] + f[26]) * (R(1) / rI);
+        }
+        // the known diagonal pair has c_y c_z > 0 (NORTH_FRONT, SOUTH_BACK) or < 0
+        const bool pos = type == T_NORTH_FRONT || type == T_SOUTH_BACK;
+        if (pos)
+            rho = R(36) * rI * (R(1) - myzI + myzI * omega) / (R(24) + omega);
+        else
+            rho = R(36) * rI * (R(1) + myzI - myzI * omega) / (R(24) + omega);
+        ux = uy = uz = 0;
+        mxx = myy = mzz = 0;
+        mxy = 0;
+        mxz = 0;
+        myz = pos ? (R(36) * myzI * rI - rho) / (R(9) * rho) : (R(36) * myzI * rI + rho) / (R(9) * rho);
+        return true;
+    }
+    case T_NWF: case T_NWB: case T_SWF: case T_SWB: {  // boundaries.cuh:662-733: resting corners
+        R rI;
+        if (type == T_NWF) rI = sum_pops(f, {0, 2, 3, 5, 11, 14, 16, 25});
+        else if (type == T_NWB) rI = sum_pops(f, {0, 2, 3, 6, 10, 14, 17, 24});
+        else if (type == T_SWF) rI = sum_pops(f, {0, 2, 4, 5, 8, 16, 18, 22});
+        else rI = sum_pops(f, {0, 2, 4, 6, 8, 10, 12, 20});
+        rho = R(216) * rI / R(125);
+        ux = uy = uz = 0;
+        mxx = myy = mzz = 0;
+        mxy = mxz = myz = 0;
+        return true;
+    }
+    case T_NEF: case T_NEB: case T_SEF: case T_SEB: {  // boundaries.cuh:734-841: outlet corners
+        neumann(rho, ux, uy, uz);
+        mxx = myy = mzz = 0;
+        mxy = mxz = myz = 0;
+        return true;
+    }
     default:
         return false;
     }
+}
+
+// bcsolid_boundary_condition, cases/cylinder/boundaries.cuh:1039-1214 (identical in
+// cases/annulus/boundaries.cuh:290-465): end-wall nodes next to a curved wall, tags
+// TAG_BCS_1..8 (common/nodeTypeMap.cuh:29-36). Four closed-form families: tags 7 and 224,
+// 11 and 208, 13 and 176, 14 and 112. Other tags leave the node unchanged (no default).
+template <class R>
+bool bcsolid_bc(uint32_t tag, R omega, const R *f, R *m) {
+    R rI, mxzI, myzI;
+    int fam;
+    switch (tag) {
+    case 7:
+        rI = sum_pops(f, {0, 1, 2, 3, 4, 5, 7, 9, 11, 13, 14, 16, 18, 19, 23, 25});
+        mxzI = (f[9] - f[16] + f[19] + f[23] - f[25]) * (R(1.0) / rI);
+        myzI = (f[11] - f[18] + f[19] - f[23] + f[25]) * (R(1.0) / rI);
+        fam = 0;
+        break;
+    case 11:
+        rI = sum_pops(f, {0, 1, 2, 3, 4, 5, 7, 8, 9, 11, 14, 16, 18, 19, 22, 25});
+        mxzI = (f[9] - f[16] + f[19] - f[22] - f[25]) * (R(1.0) / rI);
+        myzI = (f[11] - f[18] + f[19] - f[22] + f[25]) * (R(1.0) / rI);
+        fam = 1;
+        break;
+    case 13:
+        rI = sum_pops(f, {0, 1, 2, 3, 4, 5, 7, 8, 9, 11, 13, 16, 18, 19, 22, 23});
+        mxzI = (f[9] - f[16] + f[19] - f[22] + f[23]) * (R(1.0) / rI);
+        myzI = (f[11] - f[18] + f[19] - f[22] - f[23]) * (R(1.0) / rI);
+        fam = 2;
+        break;
+    case 14:
+        rI = sum_pops(f, {0, 1, 2, 3, 4, 5, 8, 9, 11, 13, 14, 16, 18, 22, 23, 25});
+        mxzI = (f[9] - f[16] - f[22] + f[23] - f[25]) * (R(1.0) / rI);
+        myzI = (f[11] - f[18] - f[22] - f[23] + f[25]) * (R(1.0) / rI);
+        fam = 3;
+        break;
+    case 112:
+        rI = sum_pops(f, {0, 1, 2, 3, 4, 6, 7, 10, 12, 13, 14, 15, 17, 21, 24, 26});
+        mxzI = (f[10] - f[15] - f[21] + f[24] - f[26]) * (R(1.0) / rI);
+        myzI = (f[12] - f[17] - f[21] - f[24] + f[26]) * (R(1.0) / rI);
+        fam = 3;
+        break;
+    case 176:
+        rI = sum_pops(f, {0, 1, 2, 3, 4, 6, 7, 8, 10, 12, 14, 15, 17, 20, 21, 24});
+        mxzI = (f[10] - f[15] + f[20] - f[21] + f[24]) * (R(1.0) / rI);
+        myzI = (f[12] - f[17] + f[20] - f[21] - f[24]) * (R(1.0) / rI);
+        fam = 2;
+        break;
+    case 208:
+        rI = sum_pops(f, {0, 1, 2, 3, 4, 6, 7, 8, 10, 12, 13, 15, 17, 20, 21, 26});
+        mxzI = (f[10] - f[15] + f[20] - f[21] - f[26]) * (R(1.0) / rI);
+        myzI = (f[12] - f[17] + f[20] - f[21] + f[26]) * (R(1.0) / rI);
+        fam = 1;
+        break;
+    case 224:
+        rI = sum_pops(f, {0, 1, 2, 3, 4, 6, 8, 10, 12, 13, 14, 15, 17, 20, 24, 26});
+        mxzI = (f[10] - f[15] + f[20] + f[24] - f[26]) * (R(1.0) / rI);
+        myzI = (f[12] - f[17] + f[20] - f[24] + f[26]) * (R(1.0) / rI);
+        fam = 0;
+        break;
+    default:
+        return false;
+    }
+    const R om = omega;
+    R rho, mxz, myz;
+    if (fam == 0) {
+        rho = (R(108) * (R(10) + mxzI * (R(-1) + om) + myzI * (R(-1) + om)) * rI) / (R(874) + om);
+        mxz = (R(198) * mxzI * rI + R(18) * myzI * rI - rho) / (R(90.) * rho);
+        myz = (R(18) * mxzI * rI + R(198) * myzI * rI - rho) / (R(90.) * rho);
+    } else if (fam == 1) {
+        rho = (R(108) * (R(10) + mxzI + myzI * (R(-1) + om) - mxzI * om) * rI) / (R(874) + om);
+        mxz = (R(198) * mxzI * rI - R(18) * myzI * rI + rho) / (R(90.) * rho);
+        myz = -(R(18) * mxzI * rI - R(198) * myzI * rI + rho) / (R(90.) * rho);
+    } else if (fam == 2) {
+        rho = (R(108) * (R(10) + myzI + mxzI * (R(-1) + om) - myzI * om) * rI) / (R(874) + om);
+        mxz = -(R(-198) * mxzI * rI + R(18) * myzI * rI + rho) / (R(90.) * rho);
+        myz = (R(-18) * mxzI * rI + R(198) * myzI * rI + rho) / (R(90.) * rho);
+    } else {
+        rho = (R(-108) * (R(-10) + mxzI * (R(-1) + om) + myzI * (R(-1) + om)) * rI) / (R(874) + om);
+        mxz = (R(198) * mxzI * rI + R(18) * myzI * rI + rho) / (R(90.) * rho);
+        myz = (R(18) * mxzI * rI + R(198) * myzI * rI + rho) / (R(90.) * rho);
+    }
+    m[0] = rho;
+    m[1] = m[2] = m[3] = 0;
+    m[4] = m[5] = m[6] = 0;
+    m[7] = 0;
+    m[8] = mxz;
+    m[9] = myz;
+    return true;
 }
 
 // fluid_boundary_condition, cases/cylinder/boundaries.cuh:845-1037 (Equilibrium variant,
 // BCF_MASS_CONSERV at cases/cylinder/constants.cuh:18). Tags other than 17/34/68/136 leave the
 // node unchanged (the switch has no default).
 template <class R>
-bool bcfluid_bc(uint32_t tag, uint32_t in_mask, const R *f, R *m) {
+bool bcfluid_bc(uint32_t tag, uint32_t in_mask, R omega, bool strong, const R *f, R *m) {
     if (tag != TAG_BCF_1 && tag != TAG_BCF_2 && tag != TAG_BCF_3 && tag != TAG_BCF_4) return false;
     const R third = R(1) / R(3);
     R rhoI = 0, uxI = 0, uyI = 0, uzI = 0, mxxI = 0, myyI = 0, mzzI = 0, mxyI = 0, mxzI = 0, myzI = 0;
@@ -307,15 +520,29 @@ bool bcfluid_bc(uint32_t tag, uint32_t in_mask, const R *f, R *m) {
     {
         // Literal coefficients of the Equilibrium quadratic, boundaries.cuh:905-915 (TAG_BCF_1),
         // :942-951 (2), :978-987 (3), :1014-1023 (4); tags 2-4 equal tag 1 with
-        // u_x -> sx u_x, u_y -> sy u_y, m_xy -> sx sy m_xy.
-        const R a = R(-9998);
+        // u_x -> sx u_x, u_y -> sy u_y, m_xy -> sx sy m_xy. The Strong quadratic
+        // (BCF_MASS_CONSERV == MassBC::Strong, boundaries.cuh:894-904, 930-940, 966-976,
+        // 1002-1012) mirrors the same way; for tags 34 and 68 the reference writes it multiplied
+        // by -1, which swaps the labels of the two roots, so a, b, c carry that sign here too.
         const R ux = sx * uxI, uy = sy * uyI, mxy = sx * sy * mxyI;
-        const R b = R(6) * rhoI * (R(1734) + R(66) * mxxI + R(198) * mxy + R(66) * myyI + R(253) * ux + R(253) * uy);
-        const R c = R(3) * (rhoI * rhoI) *
-                    (R(45) * (mxxI * mxxI) + R(405) * (mxy * mxy) + R(45) * (myyI * myyI) + R(345) * myyI * ux +
-                     R(589) * (ux * ux) + R(345) * myyI * uy + R(1467) * ux * uy + R(589) * (uy * uy) +
-                     R(45) * mxy * (R(6) * myyI + R(23) * (ux + uy)) +
-                     R(15) * mxxI * (R(18) * mxy + R(6) * myyI + R(23) * (ux + uy)));
+        const R poly = R(45) * (mxxI * mxxI) + R(405) * (mxy * mxy) + R(45) * (myyI * myyI) + R(345) * myyI * ux +
+                       R(589) * (ux * ux) + R(345) * myyI * uy + R(1467) * ux * uy + R(589) * (uy * uy) +
+                       R(45) * mxy * (R(6) * myyI + R(23) * (ux + uy)) +
+                       R(15) * mxxI * (R(18) * mxy + R(6) * myyI + R(23) * (ux + uy));
+        R a, b, c;
+        if (strong) {
+            const R sg = (tag == TAG_BCF_2 || tag == TAG_BCF_3) ? R(-1) : R(1);
+            a = sg * (R(-9996) - R(206) * omega);
+            b = sg * (R(-6) * rhoI *
+                      (R(12) * mxxI * (R(-17) + R(20) * omega) -
+                       R(17) * (R(102) + R(36) * mxy + R(12) * myyI - R(5) * ux - R(5) * uy) +
+                       omega * (R(720) * mxy + R(240) * myyI + R(53) * (ux + uy))));
+            c = sg * (R(3) * omega * (rhoI * rhoI) * poly);
+        } else {
+            a = R(-9998);
+            b = R(6) * rhoI * (R(1734) + R(66) * mxxI + R(198) * mxy + R(66) * myyI + R(253) * ux + R(253) * uy);
+            c = R(3) * (rhoI * rhoI) * poly;
+        }
         const R disc = std::max(b * b - R(4) * a * c, R(0));
         const R rho1 = (-b + std::sqrt(disc)) / (R(2) * a);
         const R rho2 = (-b - std::sqrt(disc)) / (R(2) * a);
@@ -431,6 +658,73 @@ void curved_solution(R nx, R ny, uint32_t in_mask, const R *p7, R rhoI, R mxyI, 
     out[8] = mxzp * c - myzp * s;
     out[9] = mxzp * s + myzp * c;
     out[0] = rhoI / E;
+}
+
+// numerical_solution_strong_rotated, boundary/numerical_solution.cu:124-279 (MassBC::Strong):
+// A', E' and the B' tensor summed over the outgoing directions, D' and F' over the incoming ones;
+// F' is corrected by (1 - omega) m'_I B', the 2x2 solve uses G' = (1 - omega) A' + omega E', and
+// rho = rho_I / (G' + (1 - omega) m' : B').
+template <class R>
+void curved_solution_strong(R nx, R ny, uint32_t in_mask, uint32_t out_mask, R omega, const R *p7, R rhoI, R mxyI,
+                            R mxzI, R *out) {
+    const R c = nx, s = ny, third = R(1) / R(3);
+    const R s2 = R(2) * s * c, c2 = c * c - s * s;
+    const R uxp = p7[0], uyp = p7[1], uzp = p7[2], mxxp = p7[3], myyp = p7[4], mzzp = p7[5], myzp = p7[6];
+    R A = 0, E = 0, B11 = 0, B22 = 0, B33 = 0, B12 = 0, B13 = 0, B23 = 0, Dxy = 0, Dxz = 0;
+    R F11xy = 0, F22xy = 0, F33xy = 0, F12xy = 0, F13xy = 0, F23xy = 0;
+    R F11xz = 0, F22xz = 0, F33xz = 0, F12xz = 0, F13xz = 0, F23xz = 0;
+    const R base = R(0.5) * R(3) * R(3);
+    for (int q = 0; q < Q; ++q) {
+        const R cx = R(CX[q]), cy = R(CY[q]), cz = R(CZ[q]);
+        const R xp = cx * c + cy * s, yp = -cx * s + cy * c, zp = cz;
+        const R Hxx = xp * xp - third, Hyy = yp * yp - third, Hzz = zp * zp - third;
+        const R Hxy = xp * yp, Hxz = xp * zp, Hyz = yp * zp;
+        const R wq = R(weight(q));
+        const R cf = base * wq;
+        const R Ai = wq * (R(1) + R(3) * (uxp * xp + uyp * yp + uzp * zp));
+        if (out_mask & (1u << q)) {
+            A += Ai;
+            E += Ai + cf * (uxp * uxp * Hxx + uyp * uyp * Hyy + uzp * uzp * Hzz + R(2) * uxp * uyp * Hxy +
+                            R(2) * uxp * uzp * Hxz + R(2) * uyp * uzp * Hyz);
+            B11 += cf * Hxx; B22 += cf * Hyy; B33 += cf * Hzz;
+            B12 += cf * Hxy; B13 += cf * Hxz; B23 += cf * Hyz;
+        }
+        if (in_mask & (1u << q)) {
+            Dxy += Ai * Hxy;
+            Dxz += Ai * Hxz;
+            F11xy += (cf * Hxx) * Hxy; F22xy += (cf * Hyy) * Hxy; F33xy += (cf * Hzz) * Hxy;
+            F12xy += (cf * Hxy) * Hxy; F13xy += (cf * Hxz) * Hxy; F23xy += (cf * Hyz) * Hxy;
+            F11xz += (cf * Hxx) * Hxz; F22xz += (cf * Hyy) * Hxz; F33xz += (cf * Hzz) * Hxz;
+            F12xz += (cf * Hxy) * Hxz; F13xz += (cf * Hxz) * Hxz; F23xz += (cf * Hyz) * Hxz;
+        }
+    }
+    const R om1 = R(1) - omega;
+    const R G = om1 * A + omega * E;
+    const R oxy = om1 * mxyI, oxz = om1 * mxzI;
+    F11xy -= oxy * B11; F22xy -= oxy * B22; F33xy -= oxy * B33;
+    F12xy -= oxy * B12; F13xy -= oxy * B13; F23xy -= oxy * B23;
+    F11xz -= oxz * B11; F22xz -= oxz * B22; F33xz -= oxz * B33;
+    F12xz -= oxz * B12; F13xz -= oxz * B13; F23xz -= oxz * B23;
+    const R Rxy = G * mxyI - Dxy, Rxz = G * mxzI - Dxz;
+    const R a1 = R(2) * F12xy, b1 = R(2) * F13xy;
+    const R c1 = Rxy - mxxp * F11xy - myyp * F22xy - mzzp * F33xy - R(2) * myzp * F23xy;
+    const R a2 = R(2) * F12xz, b2 = R(2) * F13xz;
+    const R cc2 = Rxz - mxxp * F11xz - myyp * F22xz - mzzp * F33xz - R(2) * myzp * F23xz;
+    const R inv = R(1) / (a2 * b1 - a1 * b2);
+    const R mxyp = (b1 * cc2 - b2 * c1) * inv;
+    const R mxzp = (a2 * c1 - a1 * cc2) * inv;
+    out[1] = uxp * c - uyp * s;
+    out[2] = uxp * s + uyp * c;
+    out[3] = uzp;
+    out[4] = mxxp * c * c + myyp * s * s - mxyp * s2;
+    out[5] = mxxp * s * s + myyp * c * c + mxyp * s2;
+    out[6] = mzzp;
+    out[7] = (mxxp - myyp) * R(0.5) * s2 + mxyp * c2;
+    out[8] = mxzp * c - myzp * s;
+    out[9] = mxzp * s + myzp * c;
+    const R den = G + om1 * (mxxp * B11 + myyp * B22 + mzzp * B33 + R(2) * mxyp * B12 + R(2) * mxzp * B13 +
+                             R(2) * myzp * B23);
+    out[0] = rhoI / den;
 }
 
 // ---------------------------------------------------------------- initial-state hash
@@ -555,7 +849,6 @@ int Base::setup() {
         std::fill(type.begin(), type.end(), T_BULK);
     }
     if (d.kind == MRLBM_CASE_CYLINDER || d.kind == MRLBM_CASE_ANNULUS) {
-        if (!d.periodic_z) return fail(MRLBM_ERR_UNSUPPORTED, "oracle: curved walls need z-periodic");
         const bool ann = d.kind == MRLBM_CASE_ANNULUS;
         ngroups = ann ? 2 : 1;
         grp[0].radius = d.r_inner;
@@ -584,15 +877,23 @@ int Base::setup() {
             y1 = std::min(NY, int(std::floor(d.yc + d.r_inner)) + 5);
         }
         const double r_mid = 0.5 * (d.r_inner + d.r_outer);
-        // load_neighbors: z wraps, x/y outside the domain count as SOLID (nodeClass.cuh:27-60)
+        // load_neighbors (nodeClass.cuh:27-60): x/y outside the domain count as SOLID; z wraps
+        // when periodic, else outside counts as SOLID too
         auto neighbour = [&](int x, int y, int z, int q) -> uint32_t {
             const int xn = x + CX[q], yn = y + CY[q];
             int zn = z + CZ[q];
-            if (zn < 0) zn += NZ;
-            if (zn >= NZ) zn -= NZ;
+            if (d.periodic_z) {
+                if (zn < 0) zn += NZ;
+                if (zn >= NZ) zn -= NZ;
+            } else if (zn < 0 || zn >= NZ) {
+                return T_SOLID;
+            }
             if (xn < 0 || xn >= NX || yn < 0 || yn >= NY) return T_SOLID;
             return type[idx(xn, yn, zn)];
         };
+        // wall and BCFLUID scans skip the end planes without z periodicity (nodeClass.cuh:171-172,
+        // 220-221; the end-plane branch at :202 is never reached)
+        const int zb = d.periodic_z ? 0 : 1, ze = d.periodic_z ? NZ : NZ - 1;
         auto octant_bits = [&](int x, int y, int z, uint32_t want, int &count) {
             uint32_t bits = 0;
             count = 0;
@@ -612,7 +913,7 @@ int Base::setup() {
         };
         // classify_boundary_nodes_triangular (nodeClass.cuh:169-216): SOLID with BULK
         // neighbours in more than two octants becomes a wall node
-        for (int z = 0; z < NZ; ++z)
+        for (int z = zb; z < ze; ++z)
             for (int y = y0; y < y1; ++y)
                 for (int x = x0; x < x1; ++x) {
                     const int64_t i = idx(x, y, z);
@@ -623,7 +924,7 @@ int Base::setup() {
                 }
         // classify_bcfluid_nodes_triangular (nodeClass.cuh:218-252): BULK with a SOLID
         // neighbour; tag = octants holding SOLID
-        for (int z = 0; z < NZ; ++z)
+        for (int z = zb; z < ze; ++z)
             for (int y = y0; y < y1; ++y)
                 for (int x = x0; x < x1; ++x) {
                     const int64_t i = idx(x, y, z);
@@ -633,6 +934,24 @@ int Base::setup() {
                     type[i] = (ann && radius(x, y) >= r_mid) ? T_BCF_OUTER : T_BCF_INNER;
                     tag[i] = bits;
                 }
+        // classify_bcsolid_nodes_triangular (cylinder nodeClass.cuh:254-287, annulus
+        // nodeClass.cuh:309-357): without z periodicity, an end-wall (BACK/FRONT) node with BULK
+        // neighbours in fewer than four octants becomes BCSOLID, tag = those octants; the annulus
+        // splits inner/outer at the mid radius
+        if (!d.periodic_z) {
+            for (int z : {0, NZ - 1})
+                for (int y = y0; y < y1; ++y)
+                    for (int x = x0; x < x1; ++x) {
+                        const int64_t i = idx(x, y, z);
+                        if (type[i] != T_BACK && type[i] != T_FRONT) continue;
+                        int count = 0;
+                        const uint32_t bits = octant_bits(x, y, z, T_BULK, count);
+                        if (count < 4) {
+                            type[i] = (ann && radius(x, y) >= r_mid) ? T_BCS_OUTER : T_BCS_INNER;
+                            tag[i] = bits;
+                        }
+                    }
+        }
         // buildBoundaryList_updateBoundaryNodeType + assignBoundaryIndices
         // (cylinder_helpers.cuh:49-132): lists in scan order z, y, x
         for (int z = 0; z < NZ; ++z)
@@ -904,9 +1223,15 @@ struct Sim : Base {
                         incoming_rotated(R(g.nx[k]), R(g.ny[k]), g.in[k], f, m[0], m + 4);
                         // m holds (rho_I, u_old, rotated incoming m'); store order xx yy zz xy xz yz
                     } else if (t == T_BCF_INNER || t == T_BCF_OUTER) {
-                        bcfluid_bc(tag[i], bcf_in[tag[i]], f, m);
+                        bcfluid_bc(tag[i], bcf_in[tag[i]], omega, bool(d.compat_flags & MRLBM_COMPAT_BCF_MASS_STRONG),
+                                   f, m);
+                    } else if (t == T_BCS_INNER || t == T_BCS_OUTER) {
+                        bcsolid_bc(tag[i], omega, f, m);
                     } else if (cyl) {
-                        if (t == T_EAST || t == T_NORTH_EAST || t == T_SOUTH_EAST) load(old, idx(x - 1, y, z), mx);
+                        // the outlet types read the x-1 neighbour's moments
+                        if (t == T_EAST || t == T_NORTH_EAST || t == T_SOUTH_EAST || t == T_EAST_FRONT ||
+                            t == T_EAST_BACK || t == T_NEF || t == T_NEB || t == T_SEF || t == T_SEB)
+                            load(old, idx(x - 1, y, z), mx);
                         face_bc(t, omega, R(inlet.empty() ? 0.0 : inlet[y + int64_t(NY) * z]), f, mo, mx, m);
                     }
                     store(M, i, m);
@@ -941,7 +1266,10 @@ struct Sim : Base {
                 R mi[10];
                 load(M, i, mi);
                 R out[10];
-                curved_solution(nx, ny, g.in[k], p7, mi[0], mi[7], mi[8], out);
+                if (d.compat_flags & MRLBM_COMPAT_MASS_STRONG)
+                    curved_solution_strong(nx, ny, g.in[k], g.out[k], omega, p7, mi[0], mi[7], mi[8], out);
+                else
+                    curved_solution(nx, ny, g.in[k], p7, mi[0], mi[7], mi[8], out);
                 std::copy(out, out + 10, res[k].begin());
             }
             for (size_t k = 0; k < g.node.size(); ++k) store(M, g.node[k], res[k].data());
@@ -1193,13 +1521,22 @@ void oracle_collide(double omega, double *m10) { collide(omega, m10); }
 int oracle_node_bc(uint32_t type, uint32_t tag, uint32_t in_mask, double omega, double u_in, const double *f27,
                    const double *old10, const double *old10_xm1, double *out10) {
     std::copy(old10, old10 + 10, out10);
-    if (type == T_BCF_INNER || type == T_BCF_OUTER) return bcfluid_bc(tag, in_mask, f27, out10) ? 1 : 0;
+    if (type == T_BCF_INNER || type == T_BCF_OUTER) return bcfluid_bc(tag, in_mask, omega, false, f27, out10) ? 1 : 0;
+    if (type == T_BCS_INNER || type == T_BCS_OUTER) return bcsolid_bc(tag, omega, f27, out10) ? 1 : 0;
     return face_bc(type, omega, u_in, f27, old10, old10_xm1, out10) ? 1 : 0;
 }
 void oracle_incoming_rotated(double nx, double ny, uint32_t in_mask, const double *f27, double *rho, double *m6) {
     incoming_rotated(nx, ny, in_mask, f27, *rho, m6);
 }
 void oracle_extrapolation_coeffs(double delta, double *c3) { extrap_coeffs(delta, c3); }
+void oracle_curved_solution_strong(double nx, double ny, uint32_t in_mask, uint32_t out_mask, double omega,
+                                   const double *p7, double rho_i, double mxy_i, double mxz_i, double *out10) {
+    curved_solution_strong(nx, ny, in_mask, out_mask, omega, p7, rho_i, mxy_i, mxz_i, out10);
+}
+int oracle_bcfluid(uint32_t tag, uint32_t in_mask, double omega, int32_t strong, const double *f27, double *out10) {
+    for (int k = 0; k < 10; ++k) out10[k] = 0.0;
+    return bcfluid_bc(tag, in_mask, omega, strong != 0, f27, out10) ? 1 : 0;
+}
 void oracle_curved_solution(double nx, double ny, uint32_t in_mask, const double *p7, double rho_i, double mxy_i,
                             double mxz_i, double *out10) {
     curved_solution(nx, ny, in_mask, p7, rho_i, mxy_i, mxz_i, out10);

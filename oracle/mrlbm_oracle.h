@@ -61,6 +61,10 @@ void oracle_vti_points(int32_t layout, int32_t nx, int32_t ny, int32_t nz, doubl
                        const double *uz, const uint8_t *solid, float *out4);
 int oracle_write_vti(const char *path, int32_t layout, int32_t nx, int32_t ny, int32_t nz, const float *pts4);
 int oracle_write_master_pvd(const char *path, int32_t max_iter, int32_t save_every);
+void oracle_curved_solution_strong(double nx, double ny, uint32_t in_mask, uint32_t out_mask, double omega,
+                                   const double *prime7, double rho_i, double mxy_i, double mxz_i, double *out10);
+/* BCFLUID closed form of one node (tags 17/34/68/136), Equilibrium or Strong density */
+int oracle_bcfluid(uint32_t tag, uint32_t in_mask, double omega, int32_t strong, const double *f27, double *out10);
 void oracle_curved_solution(double nx, double ny, uint32_t in_mask, const double *prime7,
                             double rho_i, double mxy_i, double mxz_i, double *out10);
 

@@ -75,6 +75,19 @@ CASES = {
 }
 
 
+def closed_z(d):
+    """The same case with z-end walls (no z periodicity): FRONT/BACK faces, their edges and
+    corners, BCSOLID end-wall nodes."""
+    d = d.copy()
+    d.periodic_z = 0
+    return d
+
+
+CASES["closed_channel"] = closed_z(configs.cylinder(nx=64, ny=40, nz=6, diameter=10.0, xc=20.0, yc=20.0,
+                                                    precision=_abi.FP64))
+CASES["closed_annulus"] = closed_z(configs.annulus(nx=40, ny=40, nz=6, xc=20.0, yc=20.0, r_in=7.0, r_out=17.0))
+
+
 @pytest.mark.parametrize("name", sorted(CASES))
 def test_host_geometry_equals_oracle(name):
     d = CASES[name]
@@ -100,7 +113,6 @@ def test_host_geometry_equals_oracle(name):
     ("u_max", 0.3, _abi.ERR_MACH),
     ("kind", 7, _abi.ERR_INVALID),
     ("nx", 2, _abi.ERR_INVALID),
-    ("periodic_z", 0, _abi.ERR_UNSUPPORTED),
     ("r_outer", 5.0, _abi.ERR_GEOMETRY),
 ])
 def test_descriptor_validation_status(field, value, status):
@@ -110,6 +122,29 @@ def test_descriptor_validation_status(field, value, status):
         Simulation(d, geometry_only=True)
     assert e.value.status == status
     assert lib().mrlbm_last_error()
+
+
+def test_closed_z_node_types():
+    """Without z periodicity the cylinder channel gets every end-plane type the reference
+    defines: FRONT/BACK faces, their edges with the inlet, outlet and y walls, the eight domain
+    corners and BCSOLID nodes (tags of three octants) where the cylinder meets the end walls; no
+    wall or BCFLUID node sits on an end plane."""
+    d = CASES["closed_channel"]
+    t = Simulation(d, geometry_only=True).node_types()
+    typ, idx = t >> _abi.TYPE_SHIFT, t & 0xFFFFF
+    N = _abi.NODE
+    for name in ("FRONT", "BACK", "NORTH_FRONT", "NORTH_BACK", "SOUTH_FRONT", "SOUTH_BACK", "WEST_FRONT",
+                 "WEST_BACK", "EAST_FRONT", "EAST_BACK", "NORTH_WEST_FRONT", "SOUTH_EAST_BACK", "BCSOLID_INNER"):
+        assert (typ == N[name]).any(), name
+    ends = typ[[0, -1]]
+    for name in ("INNER", "BCFLUID_INNER", "BULK"):
+        assert not (ends == N[name]).any(), name
+    assert not (typ[1:-1] == N["BCSOLID_INNER"]).any()
+    tags = set(idx[typ == N["BCSOLID_INNER"]].tolist())
+    assert tags and all(bin(t).count("1") < 4 for t in tags)
+    # back plane tags use the +z octants (bits 4-7), front plane tags the -z octants (bits 0-3)
+    assert all(t >> 4 == 0 for t in idx[-1][typ[-1] == N["BCSOLID_INNER"]])
+    assert all(t & 15 == 0 for t in idx[0][typ[0] == N["BCSOLID_INNER"]])
 
 
 def test_geometry_only_rejects_device_calls():

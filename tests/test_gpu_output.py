@@ -10,6 +10,8 @@ Bar:
   float sum by the FP32-arithmetic kernels);
 * SOLID nodes are NaN in the velocity arrays exactly where the node types say SOLID.
 """
+import re
+
 import numpy as np
 import pytest
 
@@ -100,3 +102,38 @@ def test_batch_case_writes_its_state(tmp_path):
             names = _abi.VTI_FIELDS[c.default_vti_layout()]
             for k in (1, 2, 3):
                 assert np.array_equal(arrays[names[k]], ref[k], equal_nan=True)
+
+
+def test_statistics_text_files(tmp_path):
+    """forces.dat / mass_flux.dat (16 fixed decimals, one line per step record) and pressure.dat
+    (angle in degrees, z, mean wall pressure, 12 significant digits) hold the values the ABI
+    returns, in the reference's layouts (stat_header.cuh:16-51, cylinder_kernels.cuh:251-285)."""
+    from mrlbm_3d_cuda_amd.solver import MrlbmError, Simulation
+    d = small_case("cylinder", _abi.FP64)
+    with Simulation(d) as g:
+        g.init().step(30, forces=True, stats=True)
+        g.write_forces(str(tmp_path), 5, 20)
+        g.write_pressure(str(tmp_path))
+        F = g.forces(5, 20)
+        ps, n = g.surface_pressure()
+        node = g.wall_table(0)["node"]
+        with pytest.raises(MrlbmError) as e:
+            g.write_forces(str(tmp_path / "missing"), 0, 2)
+        assert e.value.status == _abi.ERR_IO
+    fixed16 = re.compile(r"-?\d+\.\d{16}")
+    rows = [ln.split() for ln in open(tmp_path / "forces.dat").read().splitlines()]
+    mass = [ln.split() for ln in open(tmp_path / "mass_flux.dat").read().splitlines()]
+    assert len(rows) == len(mass) == 20
+    for k, (r, m) in enumerate(zip(rows, mass)):
+        assert int(r[0]) == int(m[0]) == 5 + k
+        assert all(fixed16.fullmatch(v) for v in r[1:] + m[1:])
+        assert np.allclose([float(v) for v in r[1:]], F[k, :3], rtol=0, atol=6e-17)
+        assert abs(float(m[1]) - F[k, 3]) <= 6e-17
+    lines = open(tmp_path / "pressure.dat").read().splitlines()
+    assert lines[0] == "theta z Cp" and len(lines) == 1 + len(node)
+    x, y, z = node % d.nx, (node // d.nx) % d.ny, node // (d.nx * d.ny)
+    theta = np.degrees(np.arctan2(y - d.yc, x - d.xc)) % 360.0
+    got = np.array([[float(v) for v in ln.split()] for ln in lines[1:]])
+    np.testing.assert_allclose(got[:, 0], theta, rtol=1e-11, atol=1e-9)
+    assert np.array_equal(got[:, 1], z)
+    np.testing.assert_allclose(got[:, 2], ps, rtol=1e-11)

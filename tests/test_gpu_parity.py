@@ -141,6 +141,47 @@ def test_channel_cylinder_fp64_with_forces():
     assert rel_linf(FG[:, 0], FO[:, 0]) <= 1e-10
 
 
+@pytest.mark.parametrize("kind", ["channel", "annulus"])
+def test_strong_mass_closures_fp64(kind):
+    """MassBC::Strong for the curved walls and the BCFLUID density (compat flags): FP64 parity
+    with the oracle, forces included."""
+    d = small_channel() if kind == "channel" else small_annulus()
+    d.compat_flags |= _abi.COMPAT_MASS_STRONG | _abi.COMPAT_BCF_MASS_STRONG
+    G, O, FG, FO = run_pair(d, 120, forces=True)
+    assert_fp64_parity(G, O)
+    assert force_error(FG, FO, boundary_count(d)) <= 1e-12
+    # the closures change the state against the default ones
+    d0 = small_channel() if kind == "channel" else small_annulus()
+    from mrlbm_3d_cuda_amd.solver import Simulation
+    with Simulation(d0) as s:
+        s.init().step(120)
+        assert not np.array_equal(s.fields(), G)
+
+
+def closed_z(d):
+    d = d.copy()
+    d.periodic_z = 0
+    return d
+
+
+@pytest.mark.parametrize("kind", ["channel", "annulus"])
+def test_closed_z_fp64(kind):
+    """z-end walls: FRONT/BACK faces, their edges and corners with the inlet, outlet and y walls,
+    and BCSOLID nodes where the curved walls meet the end walls. FP64 parity, forces included."""
+    base = small_channel() if kind == "channel" else small_annulus()
+    d = closed_z(base)
+    G, O, FG, FO = run_pair(d, 150, forces=True)
+    assert_fp64_parity(G, O)
+    assert force_error(FG, FO, boundary_count(d)) <= 1e-12
+
+
+def test_closed_z_fp32_error_bound():
+    G, O, FG, FO = run_pair(closed_z(small_channel(_abi.FP32)), 150, forces=True)
+    errs = field_errors(G, O)
+    print(f"[gpu parity] closed-z FP32 vs FP64 oracle: {errs}", flush=True)
+    assert errs["u"] <= 1e-4, errs
+
+
 def test_channel_cylinder_fp32_error_bound():
     # FP32 storage and arithmetic against the FP64 oracle: velocity within 1e-4 of the peak,
     # drag within 1e-3 relative over the last steps

@@ -103,7 +103,9 @@ def eq_pops(oracle_lib, rho, u):
     return oracle_lib.pop_reconstruction(moment_vec(rho, u, np.outer(u, u)))
 
 
-@pytest.mark.parametrize("name", ["NORTH", "SOUTH", "NORTH_WEST", "SOUTH_WEST"])
+@pytest.mark.parametrize("name", ["NORTH", "SOUTH", "NORTH_WEST", "SOUTH_WEST", "FRONT", "BACK", "NORTH_FRONT",
+                                  "NORTH_BACK", "SOUTH_FRONT", "SOUTH_BACK", "WEST_FRONT", "WEST_BACK",
+                                  "NORTH_WEST_FRONT", "NORTH_WEST_BACK", "SOUTH_WEST_FRONT", "SOUTH_WEST_BACK"])
 @pytest.mark.parametrize("omega", [0.8, 1.25, 1.9])
 def test_no_slip_faces_at_rest(oracle_lib, name, omega):
     f = eq_pops(oracle_lib, 1.0, np.zeros(3))
@@ -129,7 +131,8 @@ def test_inlet_equilibrium(oracle_lib, U):
         assert not np.isclose(out2[0], 1.0)
 
 
-@pytest.mark.parametrize("name", ["EAST", "NORTH_EAST", "SOUTH_EAST"])
+@pytest.mark.parametrize("name", ["EAST", "NORTH_EAST", "SOUTH_EAST", "EAST_FRONT", "EAST_BACK", "NORTH_EAST_FRONT",
+                                  "NORTH_EAST_BACK", "SOUTH_EAST_FRONT", "SOUTH_EAST_BACK"])
 def test_outlet_blend_and_equilibrium(oracle_lib, name):
     U = 0.04 if name == "EAST" else 0.0
     u = np.array([U, 0, 0])
@@ -200,6 +203,47 @@ def test_bcfluid_other_tags_leave_node_unchanged(oracle_lib):
     assert not hit and np.array_equal(out, old)
 
 
+# ---- BCSOLID end-wall nodes (bcsolid_boundary_condition, boundaries.cuh:1039-1214)
+
+BCS_TAGS = [7, 11, 13, 14, 112, 176, 208, 224]
+
+
+@pytest.mark.parametrize("tag", BCS_TAGS)
+@pytest.mark.parametrize("omega", [0.8, 1.25, 1.9])
+def test_bcsolid_at_rest(oracle_lib, tag, omega):
+    f = eq_pops(oracle_lib, 1.0, np.zeros(3))
+    old = moment_vec(1.0, np.zeros(3), np.zeros((3, 3)))
+    out, hit = oracle_lib.node_bc(T["BCSOLID_INNER"], f, np.ones(10), tag=tag, omega=omega)
+    assert hit
+    assert np.allclose(out, old, atol=1e-14), out - old
+
+
+def test_bcsolid_tags_are_mirror_images(oracle_lib):
+    """The eight tags are one geometry seen through the lattice symmetries: inverting every
+    velocity maps tag 7 to 224, 11 to 208, 13 to 176, 14 to 112 (m_xz, m_yz even), and reflecting
+    x and y maps 7 to 14 and 11 to 13 (m_xz, m_yz odd). A sign slip in any closed form breaks one
+    of these."""
+    inv = np.array([int(np.where((C == -C[q]).all(1))[0][0]) for q in range(27)])
+    pxy = mirror_perm(0)[mirror_perm(1)]
+    even = np.ones(10)
+    odd_xy = np.array([1, -1, -1, 1, 1, 1, 1, 1, -1, -1])
+    f = oracle_lib.pop_reconstruction(random_moments(0.03))
+    for a, b, perm, sign in [(7, 224, inv, even), (11, 208, inv, even), (13, 176, inv, even), (14, 112, inv, even),
+                             (7, 14, pxy, odd_xy), (11, 13, pxy, odd_xy)]:
+        fa, _ = oracle_lib.node_bc(T["BCSOLID_INNER"], f, np.ones(10), tag=a, omega=1.3)
+        fm = np.empty(27)
+        fm[perm] = f
+        fb, _ = oracle_lib.node_bc(T["BCSOLID_INNER"], fm, np.ones(10), tag=b, omega=1.3)
+        assert np.allclose(fb, sign * fa, rtol=1e-12, atol=1e-15), (a, b, fb - sign * fa)
+
+
+def test_bcsolid_other_tags_leave_node_unchanged(oracle_lib):
+    f = oracle_lib.pop_reconstruction(random_moments())
+    old = random_moments()
+    out, hit = oracle_lib.node_bc(T["BCSOLID_INNER"], f, old, tag=3)
+    assert not hit and np.array_equal(out, old)
+
+
 # ---- curved wall
 
 def test_incoming_rotated_full_mask_is_moments(oracle_lib):
@@ -255,6 +299,80 @@ def test_curved_closure_reproduces_equilibrium(oracle_lib, theta):
     assert abs(out[0] - 1) < 1e-13
     assert np.allclose(out[1:4], u, atol=1e-15)
     assert np.allclose(out[4:], [uu[0, 0], uu[1, 1], uu[2, 2], uu[0, 1], uu[0, 2], uu[1, 2]], atol=1e-13)
+
+
+def opposite_mask(mask):
+    """Directions q whose opposite is in mask (find_incomings_outgoings' outgoing set) plus rest."""
+    opp = [int(np.where((C == -C[q]).all(1))[0][0]) for q in range(27)]
+    return 1 | sum(1 << q for q in range(27) if mask >> opp[q] & 1)
+
+
+def wall_mask(nx, ny):
+    """An incoming mask of a wall whose fluid side is +n (rest direction included)."""
+    mask = 1
+    for q in range(1, 27):
+        if -(C[q, 0] * nx + C[q, 1] * ny) <= 0.3:
+            mask |= 1 << q
+    return mask
+
+
+@pytest.mark.parametrize("theta,omega", [(0.0, 1.0), (0.4, 1.6), (2.2, 0.8), (-1.1, 1.95)])
+def test_curved_closure_strong_conserves_mass(oracle_lib, theta, omega):
+    """MassBC::Strong closure (numerical_solution.cu:124-279): after the closure and the
+    collision, the populations leaving the wall node along its outgoing directions carry exactly
+    the mass rho_I that arrived along the incoming ones, for any state; at rest it gives rho = 1,
+    m = 0."""
+    nx, ny = np.cos(theta), np.sin(theta)
+    mask = wall_mask(nx, ny)
+    out_mask = opposite_mask(mask)
+    rng = np.random.default_rng(int(1000 * omega))
+    f = oracle_lib.pop_reconstruction(random_moments(0.04))
+    rho_i, m6 = oracle_lib.incoming_rotated(nx, ny, mask, f)
+    prime7 = rng.uniform(-0.03, 0.03, 7)
+    out = oracle_lib.curved_solution_strong(nx, ny, mask, out_mask, omega, prime7, rho_i, m6[3], m6[4])
+    fpost = oracle_lib.pop_reconstruction(oracle_lib.collide(omega, out))
+    sent = sum(fpost[q] for q in range(27) if out_mask >> q & 1)
+    assert abs(sent - rho_i) < 1e-13 * rho_i, (sent, rho_i)
+    # the Equilibrium closure does not balance this (it sets rho = rho_I / E')
+    eq = oracle_lib.curved_solution(nx, ny, mask, prime7, rho_i, m6[3], m6[4])
+    sent_eq = sum(oracle_lib.pop_reconstruction(oracle_lib.collide(omega, eq))[q] for q in range(27)
+                  if out_mask >> q & 1)
+    assert abs(sent_eq - rho_i) > 1e-6
+    # at rest
+    f0 = eq_pops(oracle_lib, 1.0, np.zeros(3))
+    rho0_i, m60 = oracle_lib.incoming_rotated(nx, ny, mask, f0)
+    rest = oracle_lib.curved_solution_strong(nx, ny, mask, out_mask, omega, np.zeros(7), rho0_i, m60[3], m60[4])
+    assert abs(rest[0] - 1) < 1e-13 and np.allclose(rest[1:], 0, atol=1e-14)
+
+
+@pytest.mark.parametrize("tag", [17, 34, 68, 136])
+@pytest.mark.parametrize("omega", [0.7, 1.0, 1.9])
+def test_bcfluid_strong_at_rest(oracle_lib, tag, omega):
+    """The Strong BCFLUID quadratic (boundaries.cuh:894-904 and its mirrors) has rho = 1 as the
+    root selected for populations at rest, for any omega; u and m stay zero."""
+    masks = bcfluid_masks()
+    f = eq_pops(oracle_lib, 1.0, np.zeros(3))
+    out = oracle_lib.bcfluid(tag, masks[tag], omega, True, f)
+    assert out is not None
+    assert abs(out[0] - 1) < 1e-13
+    assert np.allclose(out[1:], 0, atol=1e-14)
+
+
+def test_bcfluid_strong_mirror_symmetry(oracle_lib):
+    masks = bcfluid_masks()
+    px, py = mirror_perm(0), mirror_perm(1)
+    f = oracle_lib.pop_reconstruction(random_moments(0.03))
+    base = oracle_lib.bcfluid(17, masks[17], 1.3, True, f)
+    sx = np.array([1, -1, 1, 1, 1, 1, 1, -1, -1, 1])
+    sy = np.array([1, 1, -1, 1, 1, 1, 1, -1, 1, -1])
+    for tag, perm, sign in [(34, px, sx), (68, py, sy), (136, px[py], sx * sy)]:
+        fm = np.empty(27)
+        fm[perm] = f
+        out = oracle_lib.bcfluid(tag, masks[tag], 1.3, True, fm)
+        assert np.allclose(out, sign * base, rtol=1e-12, atol=1e-15), (tag, out - sign * base)
+    # the Strong density differs from the Equilibrium one away from rest
+    eq = oracle_lib.bcfluid(17, masks[17], 1.3, False, f)
+    assert abs(eq[0] - base[0]) > 1e-9
 
 
 # ---- wall-pressure statistics (surface_pressure_extrapolation, extrapolation_utils.cuh:109-154)
