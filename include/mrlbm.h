@@ -245,6 +245,20 @@ int mrlbm_write_forces(mrlbm_sim *sim, const char *dir, int64_t first, int32_t c
 int mrlbm_write_pressure(mrlbm_sim *sim, const char *dir);
 
 /*
+ * Inlet / outlet plane means of the cylinder channel <- inlet_avg_density (K6,
+ * src/cases/cylinder/cylinder_kernels.cu:285-338) and outlet_avg_ux (K7, src/solver/mlbm.cu:18-81):
+ * every step queued with MRLBM_STEP_STATS files the mean density over the plane x = 0 and the
+ * mean u_x over x = nx - 2 of the state after the step (FP64 sums) in a 65,536-step ring beside
+ * the force records. mrlbm_get_plane_means returns count pairs for steps first .. first + count - 1
+ * (records of steps queued without the flag are not written); MRLBM_ERR_UNSUPPORTED for other
+ * case kinds. mrlbm_write_inlet_density writes <dir>/inlet_density.dat "iter rho_inlet
+ * rho_inlet_average" (write_average_inlet_density, src/statistics/stat_header.cuh:53-72) with the
+ * running mean taken over the records written.
+ */
+int mrlbm_get_plane_means(mrlbm_sim *sim, int64_t first, int32_t count, double *out2);
+int mrlbm_write_inlet_density(mrlbm_sim *sim, const char *dir, int64_t first, int32_t count);
+
+/*
  * Visualisation output <- write_vti_3d / write_vti_3d_annulus / write_master_pvd
  * (src/output/saveData.cu:3-230, called from src/cases/cylinder/cylinder_kernels.cu:133 and
  * src/simulation.cu:8,133). The point arrays are computed on the device from the current moments:

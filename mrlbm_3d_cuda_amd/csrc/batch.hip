@@ -249,9 +249,11 @@ class Batch final : public BatchBase {
         const int f = (flags & MRLBM_STEP_FORCES) ? 1 : 0;
         if (nwblocks_[f] > 0)
             launch_wall_batch<R, S>(d_wargs_[cur][f], d_wblocks_[f], nwblocks_[f], d_wcases_[f], nwcases_[f], stream_);
-        if (flags & MRLBM_STEP_STATS)  // wall-pressure statistics, one launch per case
-            for (size_t c = 0; c < cases.size(); ++c)
+        if (flags & MRLBM_STEP_STATS)  // wall-pressure and plane statistics, launches per case
+            for (size_t c = 0; c < cases.size(); ++c) {
                 if (sim(c).stats_on(flags)) launch_pressure<R, S>(sim(c).pressure_args(cur), stream_);
+                if (sim(c).planes_on(flags)) launch_planes<R, S>(sim(c).plane_args(cur), stream_);
+            }
     }
 
     int graph_for(uint32_t flags, hipGraphExec_t &exec) {

@@ -362,6 +362,35 @@ int mrlbm_write_forces(mrlbm_sim *s, const char *dir, int64_t first, int32_t cou
     return ok ? MRLBM_OK : set_error(MRLBM_ERR_IO, "write failed in " + base);
 }
 
+int mrlbm_get_plane_means(mrlbm_sim *s, int64_t first, int32_t count, double *out2) {
+    DEV_CHECK(s);
+    if (!out2 && count > 0) return set_error(MRLBM_ERR_INVALID, "null buffer");
+    return s->impl->get_plane_means(first, count, out2);
+}
+
+// inlet_density.dat: "<iter> <inlet mean density> <running mean>", 16 fixed decimals, the running
+// mean restarting at `first` (write_average_inlet_density, src/statistics/stat_header.cuh:53-72,
+// with the update of src/cases/cylinder/cylinder_kernels.cu:172).
+int mrlbm_write_inlet_density(mrlbm_sim *s, const char *dir, int64_t first, int32_t count) {
+    DEV_CHECK(s);
+    if (!dir || !*dir || count < 0) return set_error(MRLBM_ERR_INVALID, "bad directory or count");
+    std::vector<double> p(size_t(2) * size_t(count));
+    if (count > 0)
+        if (int rc = s->impl->get_plane_means(first, count, p.data())) return rc;
+    const std::string path = std::string(dir) + "/inlet_density.dat";
+    FILE *fp = std::fopen(path.c_str(), "w");
+    if (!fp) return set_error(MRLBM_ERR_IO, "cannot write " + path);
+    double avg = 0.0;
+    bool ok = true;
+    for (int32_t k = 0; k < count && ok; ++k) {
+        const double rho = p[2 * size_t(k)];
+        avg += (rho - avg) / double(k + 1);
+        ok = std::fprintf(fp, "%lld %.16f %.16f\n", static_cast<long long>(first + k), rho, avg) > 0;
+    }
+    ok = (std::fclose(fp) == 0) && ok;
+    return ok ? MRLBM_OK : set_error(MRLBM_ERR_IO, "write failed: " + path);
+}
+
 // pressure.dat: header "theta z Cp", then per inner-wall node (wall-table order) its angle about
 // the centre in degrees [0, 360), its z and the time-averaged wall pressure, 12 significant
 // digits (write_pressure, src/cases/cylinder/cylinder_kernels.cuh:251-285).

@@ -86,6 +86,20 @@ struct PressureArgs {
     double *__restrict__ sum;  // per inner-wall node, FP64 running sum
 };
 
+// Arguments of the plane-mean statistics pass (one block): mean density on the inlet plane x = 0
+// and mean u_x on the plane x = nx - 2 of the state after the step, filed in the plane ring at the
+// step the wall finisher just completed.
+template <class R, class S>
+struct PlaneArgs {
+    const S *__restrict__ rho;  // stored density deviation field
+    const S *__restrict__ ux;   // stored u_x field
+    int nx, ny, nz;
+    double rho0;
+    StoreScale<R> scale;
+    const unsigned long long *__restrict__ step_counter;  // already advanced past this step
+    double *__restrict__ ring;                             // 2 per step, FORCE_RING_STEPS records
+};
+
 // Steps of force history kept in the device ring.
 constexpr int FORCE_RING_STEPS = 1 << 16;
 
@@ -112,6 +126,9 @@ void launch_wall(const WallArgs<R, S> &a, hipStream_t stream);
 
 template <class R, class S>
 void launch_pressure(const PressureArgs<R, S> &a, hipStream_t stream);
+
+template <class R, class S>
+void launch_planes(const PlaneArgs<R, S> &a, hipStream_t stream);
 
 // Batched launches: one fused pass over the work items of many cases (item {case, tile |
 // special << 31, z0 | z1 << 16, 0}; tiles BX x BY in {32x8, 16x8, 32x4}), and one wall pass

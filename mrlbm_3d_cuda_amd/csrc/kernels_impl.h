@@ -965,6 +965,49 @@ void launch_pressure(const PressureArgs<R, S> &a, hipStream_t stream) {
     if (a.n > 0) hipLaunchKernelGGL((pressure_kernel<R, S>), dim3((a.n + 255) / 256), dim3(256), 0, stream, a);
 }
 
+// Plane means of one step (inlet_avg_density, src/cases/cylinder/cylinder_kernels.cu:285-338, and
+// outlet_avg_ux, src/solver/mlbm.cu:18-81): FP64 sums in a fixed order (strided per thread, then a
+// tree), so the records are deterministic.
+template <class R, class S>
+__global__ __launch_bounds__(256) void plane_kernel(PlaneArgs<R, S> a) {
+    __shared__ double red[2][256];
+    const int t = int(threadIdx.x);
+    const int n = a.ny * a.nz;
+    double sr = 0.0, su = 0.0;
+    for (int i = t; i < n; i += 256) {
+        const int y = i % a.ny, z = i / a.ny;
+        const uint32_t row = uint32_t(a.nx) * (uint32_t(y) + uint32_t(a.ny) * uint32_t(z));
+        double r = double(Conv<S>::to(ld_field(a.rho, row)));
+        double u = double(Conv<S>::to(ld_field(a.ux, row + uint32_t(a.nx - 2))));
+        if constexpr (Conv<S>::scaled) {
+            r *= double(a.scale.dec[0]);
+            u *= double(a.scale.dec[1]);
+        }
+        sr += a.rho0 + r;
+        su += u;
+    }
+    red[0][t] = sr;
+    red[1][t] = su;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w) {
+            red[0][t] += red[0][t + w];
+            red[1][t] += red[1][t + w];
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        const size_t slot = size_t((*a.step_counter - 1ull) % FORCE_RING_STEPS);
+        a.ring[2 * slot + 0] = red[0][0] / double(n);
+        a.ring[2 * slot + 1] = red[1][0] / double(n);
+    }
+}
+
+template <class R, class S>
+void launch_planes(const PlaneArgs<R, S> &a, hipStream_t stream) {
+    hipLaunchKernelGGL((plane_kernel<R, S>), dim3(1), dim3(256), 0, stream, a);
+}
+
 template <class R, class S>
 void launch_encode(const double *in, const FieldSet<S> &out, uint32_t n, const StoreScale<R> &sc, hipStream_t stream) {
     hipLaunchKernelGGL((encode_kernel<R, S>), dim3((n + 255) / 256), dim3(256), 0, stream, in, out, n, sc);
@@ -981,6 +1024,7 @@ void launch_decode(const FieldSet<const S> &in, double *out, uint32_t n, const S
     template void launch_fused<R, S>(const FusedArgs<R, S> &, int, int, int, hipStream_t);                  \
     template void launch_wall<R, S>(const WallArgs<R, S> &, hipStream_t);                                   \
     template void launch_pressure<R, S>(const PressureArgs<R, S> &, hipStream_t);                           \
+    template void launch_planes<R, S>(const PlaneArgs<R, S> &, hipStream_t);                                 \
     template void launch_fused_batch<R, S>(const FusedArgs<R, S> *, const uint4 *, int, int, int, int, int,     \
                                            hipStream_t);                                                    \
     template int prepare_fused_batch<R, S>(int, int, int);                                                  \

@@ -56,6 +56,7 @@ class SimBase {
     virtual int set_steps(int64_t n) = 0;
     virtual int stats_reset() = 0;
     virtual int get_surface_pressure(double *mean, int64_t *samples) = 0;
+    virtual int get_plane_means(int64_t first, int count, double *out2) = 0;
     // four float32 point arrays of a VTK file from the current state (MRLBM_VTI_* layout)
     virtual int vti_points(int layout, float vel_norm, float *host) = 0;
     virtual LaunchConfig current_launch() const = 0;
@@ -235,6 +236,7 @@ class Sim final : public SimBase {
         // the wall pass launches at most wall_blocks(nwall, nbcf) blocks (forces on)
         HIP_TRY(hipMalloc(&d_partials_, size_t(wall_blocks(nwall, nbcf_)) * 4 * sizeof(double)));
         HIP_TRY(hipMalloc(&d_ring_, size_t(FORCE_RING_STEPS) * 4 * sizeof(double)));
+        HIP_TRY(hipMalloc(&d_plane_ring_, size_t(FORCE_RING_STEPS) * 2 * sizeof(double)));
         // wall-pressure statistics of the inner wall: fluid points 1..3 spacings out must lie inside
         stats_ok_ = g.ngroups > 0 && !g.wall[0].node.empty();
         for (size_t k = 0; stats_ok_ && k < g.wall[0].node.size(); ++k) {
@@ -249,6 +251,7 @@ class Sim final : public SimBase {
         HIP_TRY(hipMalloc(&d_step_counter_, sizeof(unsigned long long)));
         HIP_TRY(hipMemsetAsync(d_step_counter_, 0, sizeof(unsigned long long), stream_));
         HIP_TRY(hipMemsetAsync(d_ring_, 0, size_t(FORCE_RING_STEPS) * 4 * sizeof(double), stream_));
+        HIP_TRY(hipMemsetAsync(d_plane_ring_, 0, size_t(FORCE_RING_STEPS) * 2 * sizeof(double), stream_));
         HIP_TRY(hipStreamSynchronize(stream_));
         return configure_launch(lc);
     }
@@ -365,6 +368,40 @@ class Sim final : public SimBase {
     }
 
     bool stats_on(uint32_t flags) const { return (flags & MRLBM_STEP_STATS) && stats_ok_; }
+    // inlet / outlet plane means: channel cases (which always run the wall pass that advances the
+    // device step counter)
+    bool planes_on(uint32_t flags) const { return (flags & MRLBM_STEP_STATS) && d.kind == MRLBM_CASE_CYLINDER; }
+
+    PlaneArgs<R, S> plane_args(int cur) const {
+        PlaneArgs<R, S> p{};
+        const FieldSet<S> fs = make_fields(buf_[cur ^ 1], stride_);
+        p.rho = fs.f[0];
+        p.ux = fs.f[1];
+        p.nx = g.nx;
+        p.ny = g.ny;
+        p.nz = g.nz;
+        p.rho0 = d.rho0;
+        p.scale = scale_;
+        p.step_counter = d_step_counter_;
+        p.ring = d_plane_ring_;
+        return p;
+    }
+
+    int get_plane_means(int64_t first, int count, double *out2) override {
+        if (d.kind != MRLBM_CASE_CYLINDER) return set_error(MRLBM_ERR_UNSUPPORTED, "plane means need an inlet and an outlet");
+        if (int rc = wait_batch()) return rc;
+        if (count < 0 || first < 0 || first + count > steps || first < steps - FORCE_RING_STEPS)
+            return set_error(MRLBM_ERR_INVALID, "plane record range not available");
+        HIP_TRY(hipStreamSynchronize(stream_));
+        for (int k = 0; k < count;) {
+            const int slot = int((first + k) % FORCE_RING_STEPS);
+            const int len = std::min(count - k, FORCE_RING_STEPS - slot);
+            HIP_TRY(hipMemcpy(out2 + 2 * size_t(k), d_plane_ring_ + 2 * size_t(slot), size_t(len) * 2 * sizeof(double),
+                              hipMemcpyDeviceToHost));
+            k += len;
+        }
+        return MRLBM_OK;
+    }
 
     // Arguments of the wall-pressure pass on the state after a step from parity `cur`.
     PressureArgs<R, S> pressure_args(int cur) const {
@@ -687,6 +724,7 @@ class Sim final : public SimBase {
             launch_pressure<R, S>(pressure_args(cur_), stream_);
             ++stats_samples_;
         }
+        if (planes_on(flags)) launch_planes<R, S>(plane_args(cur_), stream_);
         cur_ ^= 1;
         ++steps;
     }
@@ -731,7 +769,7 @@ class Sim final : public SimBase {
             if (g) (void)hipGraphExecDestroy(g);
         void *ptrs[] = {buf_[0], buf_[1], donor_, d_info_, d_items_, d_wconst_, d_wnx_, d_wny_, d_win_, d_wout_, d_wnode_,
                         d_wcoef_, d_stbase_, d_stfrac_, d_bcfnode_, d_bcftag_, d_bcfin_, d_bcfout_, d_inlet_,
-                        d_partials_, d_ring_, d_step_counter_, d_ps_sum_};
+                        d_partials_, d_ring_, d_step_counter_, d_ps_sum_, d_plane_ring_};
         for (void *p : ptrs)
             if (p) (void)hipFree(p);
         if (stream_) (void)hipStreamDestroy(stream_);
@@ -763,7 +801,7 @@ class Sim final : public SimBase {
     uint32_t *d_win_ = nullptr, *d_wout_ = nullptr, *d_wnode_ = nullptr, *d_stbase_ = nullptr;
     uint32_t *d_bcfnode_ = nullptr, *d_bcfin_ = nullptr, *d_bcfout_ = nullptr;
     uint8_t *d_bcftag_ = nullptr;
-    double *d_partials_ = nullptr, *d_ring_ = nullptr, *d_ps_sum_ = nullptr;
+    double *d_partials_ = nullptr, *d_ring_ = nullptr, *d_ps_sum_ = nullptr, *d_plane_ring_ = nullptr;
     bool stats_ok_ = false;
     int64_t stats_samples_ = 0;
     unsigned long long *d_step_counter_ = nullptr;
@@ -813,7 +851,7 @@ inline uint64_t case_device_bytes(const mrlbm_case_desc &d) {
     const uint64_t n = uint64_t(d.nx) * uint64_t(d.ny) * uint64_t(d.nz);
     const uint64_t stride = (n + 1 + 63) / 64 * 64;
     const uint64_t s = d.precision == MRLBM_FP64 ? 8 : (d.precision == MRLBM_FP32 ? 4 : 2);
-    return 30 * stride * s + 4 * n + uint64_t(FORCE_RING_STEPS) * 4 * sizeof(double);
+    return 30 * stride * s + 4 * n + uint64_t(FORCE_RING_STEPS) * 6 * sizeof(double);
 }
 
 }  // namespace mrlbm

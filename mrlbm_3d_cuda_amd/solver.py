@@ -170,6 +170,15 @@ class Simulation:
         """forces.dat and mass_flux.dat of the records first .. first + count - 1."""
         _check(self.lib.mrlbm_write_forces(self.h, os.fsencode(directory), first, count))
 
+    def plane_means(self, first, count):
+        """(count, 2) float64: mean inlet density (x = 0) and mean u_x at x = nx - 2 per step."""
+        out = np.empty((count, 2), dtype=np.float64)
+        _check(self.lib.mrlbm_get_plane_means(self.h, first, count, as_ptr(out, C.c_double)))
+        return out
+
+    def write_inlet_density(self, directory, first, count):
+        _check(self.lib.mrlbm_write_inlet_density(self.h, os.fsencode(directory), first, count))
+
     def write_pressure(self, directory):
         """pressure.dat: angle, z and mean wall pressure per inner-wall node."""
         _check(self.lib.mrlbm_write_pressure(self.h, os.fsencode(directory)))

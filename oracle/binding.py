@@ -28,6 +28,7 @@ _PROTOS = {
     "oracle_set_moments": (C.c_int, [C.c_void_p, C.c_int32, _dp]),
     "oracle_get_forces": (C.c_int, [C.c_void_p, C.c_int64, C.c_int32, _dp]),
     "oracle_get_node_types": (C.c_int, [C.c_void_p, _u32p]),
+    "oracle_get_plane_means": (C.c_int, [C.c_void_p, C.c_int64, C.c_int32, _dp]),
     "oracle_get_wall_table": (C.c_int, [C.c_void_p, C.c_int32, _i64p, _dp, _dp, _dp, _u32p, _u32p]),
     "oracle_get_bcfluid_table": (C.c_int, [C.c_void_p, C.c_int32, _i64p, _u32p, _u32p, _u32p]),
     "oracle_get_info": (C.c_int, [C.c_void_p, _P(Info)]),
@@ -146,6 +147,11 @@ class Oracle:
         """Instantaneous wall pressure of every inner-wall node (compute_surface_pressure)."""
         out = np.empty(self.info().nb_inner, np.float64)
         _check(self.lib.oracle_surface_pressure(self.h, as_ptr(out, C.c_double)), "surface_pressure")
+        return out
+
+    def plane_means(self, first, count):
+        out = np.empty((count, 2), np.float64)
+        _check(self.lib.oracle_get_plane_means(self.h, first, count, as_ptr(out, C.c_double)), "oracle_get_plane_means")
         return out
 
     def bcfluid_table(self, group):

@@ -788,6 +788,7 @@ struct Base {
     uint32_t bcf_in[256] = {0}, bcf_out[256] = {0};
     std::vector<double> inlet;        // prescribed inlet u_x per (y, z)
     std::vector<std::array<double, 4>> forces;
+    std::vector<std::array<double, 2>> planes;  // inlet mean density, outlet-plane mean u_x
     int64_t steps = 0;
     int active_tiles = 0, total_tiles = 0;
     int64_t idx(int x, int y, int z) const { return x + int64_t(NX) * (y + int64_t(NY) * z); }
@@ -1297,6 +1298,19 @@ struct Sim : Base {
         } else {
             forces.push_back({0.0, 0.0, 0.0, 0.0});
         }
+        // inlet_avg_density (cylinder_kernels.cu:285-338): mean of RHO_0 + rho over x = 0;
+        // outlet_avg_ux (mlbm.cu:18-81): mean u_x over x = NX - 2; both of the state after the step
+        std::array<double, 2> pm = {0.0, 0.0};
+        if ((flags & MRLBM_STEP_STATS) && d.kind == MRLBM_CASE_CYLINDER) {
+            for (int z = 0; z < NZ; ++z)
+                for (int y = 0; y < NY; ++y) {
+                    pm[0] += d.rho0 + double(M[0][idx(0, y, z)]);
+                    pm[1] += double(M[1][idx(NX - 2, y, z)]);
+                }
+            pm[0] /= double(NY) * double(NZ);
+            pm[1] /= double(NY) * double(NZ);
+        }
+        planes.push_back(pm);
         ++steps;
         return MRLBM_OK;
     }
@@ -1365,6 +1379,14 @@ int oracle_get_forces(oracle_sim *s, int64_t first, int32_t count, double *out4)
     if (first < 0 || first + count > int64_t(F.size())) return fail(MRLBM_ERR_INVALID, "oracle: force range");
     for (int32_t k = 0; k < count; ++k)
         for (int j = 0; j < 4; ++j) out4[4 * k + j] = F[first + k][j];
+    return MRLBM_OK;
+}
+
+int oracle_get_plane_means(oracle_sim *s, int64_t first, int32_t count, double *out2) {
+    auto &P = s->impl->planes;
+    if (first < 0 || first + count > int64_t(P.size())) return fail(MRLBM_ERR_INVALID, "oracle: plane range");
+    for (int32_t k = 0; k < count; ++k)
+        for (int j = 0; j < 2; ++j) out2[2 * k + j] = P[first + k][j];
     return MRLBM_OK;
 }
 

@@ -27,6 +27,7 @@ int oracle_step(oracle_sim *s, int32_t nsteps, uint32_t flags);
 int oracle_get_moments(oracle_sim *s, int32_t field, double *host);
 int oracle_set_moments(oracle_sim *s, int32_t field, const double *host);
 int oracle_get_forces(oracle_sim *s, int64_t first, int32_t count, double *out4);
+int oracle_get_plane_means(oracle_sim *s, int64_t first, int32_t count, double *out2);
 int oracle_get_node_types(oracle_sim *s, uint32_t *out);
 int oracle_get_wall_table(oracle_sim *s, int32_t group, int64_t *node, double *nx, double *ny,
                           double *delta, uint32_t *in_mask, uint32_t *out_mask);

@@ -166,14 +166,14 @@ def test_default_descriptors_match_configs():
 def test_batch_admission_plan():
     """Cases are admitted from the front while their device bytes stay within the budget; the
     byte count is two moment buffers and the donor snapshot (10 padded fields each), 4 B of node
-    code per node and the 65536-step FP64 force ring."""
+    code per node and the 65536-step FP64 force (4 values) and plane-mean (2 values) rings."""
     from mrlbm_3d_cuda_amd.solver import Batch
     descs = configs.c5_cases(8)
 
     def case_bytes(d):
         n = d.num_nodes
         stride = (n + 1 + 63) // 64 * 64
-        return 30 * stride * 2 + 4 * n + 65536 * 4 * 8
+        return 30 * stride * 2 + 4 * n + 65536 * 6 * 8
 
     sizes = [case_bytes(d) for d in descs]
     total = sum(sizes)

@@ -31,6 +31,10 @@ def check_same(got, ref, what):
     assert equal, f"{what}: max |difference| {diff:.3e}"
 
 
+def planes(sim, steps):
+    return sim.plane_means(0, steps) if sim.desc.kind == _abi.CASE_CYLINDER else None
+
+
 def standalone(desc, steps, stats=True):
     from mrlbm_3d_cuda_amd.solver import Simulation
     s = Simulation(desc).init()
@@ -38,8 +42,9 @@ def standalone(desc, steps, stats=True):
     F = s.fields()
     f = s.forces(0, steps)
     ps = s.surface_pressure() if s.info().nb_inner else None
+    pm = planes(s, steps) if stats else None
     s.close()
-    return F, f, ps
+    return F, f, ps, pm
 
 
 @pytest.mark.parametrize("precision", [_abi.FP64, _abi.FP32, _abi.FP32_BF16])
@@ -51,16 +56,18 @@ def test_batch_equals_standalone_runs(precision):
     with Batch(descs) as b:
         assert b.admitted == len(descs)
         b.init().step(steps, forces=True, stats=True)
-        got = [(c.fields(), c.forces(0, steps), c.surface_pressure() if c.info().nb_inner else None)
-               for c in b.cases]
+        got = [(c.fields(), c.forces(0, steps), c.surface_pressure() if c.info().nb_inner else None,
+                planes(c, steps)) for c in b.cases]
         assert all(c.info().steps_done == steps for c in b.cases)
     for i, d in enumerate(descs):
-        F, f, ps = standalone(d, steps)
+        F, f, ps, pm = standalone(d, steps)
         check_same(got[i][0], F, f"case {i} fields")
         check_same(got[i][1], f, f"case {i} forces")
         if ps is not None:
             check_same(got[i][2][0], ps[0], f"case {i} wall pressure")
             assert got[i][2][1] == ps[1] == steps
+        if pm is not None:
+            check_same(got[i][3], pm, f"case {i} plane means")
 
 
 def test_batch_autotune_keeps_results(monkeypatch):
@@ -85,7 +92,7 @@ def test_batch_autotune_keeps_results(monkeypatch):
     monkeypatch.setenv("MRLBM_FUSED_VARIANT", str(r["variant"]))
     monkeypatch.setenv("MRLBM_LZ", str(r["lz"]))
     for i, d in enumerate(descs):
-        F, f, _ = standalone(d, 20)
+        F, f, _, _ = standalone(d, 20)
         check_same(got[i][0], F, f"case {i} fields")
         check_same(got[i][1], f, f"case {i} forces")
 

@@ -137,3 +137,31 @@ def test_statistics_text_files(tmp_path):
     np.testing.assert_allclose(got[:, 0], theta, rtol=1e-11, atol=1e-9)
     assert np.array_equal(got[:, 1], z)
     np.testing.assert_allclose(got[:, 2], ps, rtol=1e-11)
+
+
+def test_plane_means_and_inlet_density_file(tmp_path):
+    """Inlet / outlet plane means (inlet_avg_density, outlet_avg_ux) of the statistics steps
+    against the FP64 oracle, and inlet_density.dat with its running mean
+    (stat_header.cuh:53-72, cylinder_kernels.cu:172)."""
+    from mrlbm_3d_cuda_amd.solver import MrlbmError, Simulation
+    from oracle.binding import Oracle
+    d = small_case("cylinder", _abi.FP64)
+    o = Oracle(d).init()
+    o.step(20, 0)
+    o.step(20, _abi.STEP_STATS)
+    ref = o.plane_means(20, 20)
+    with Simulation(d) as g:
+        g.init().step(20).step(20, stats=True)
+        got = g.plane_means(20, 20)
+        g.write_inlet_density(str(tmp_path), 20, 20)
+    np.testing.assert_allclose(got, ref, rtol=1e-13, atol=1e-16)
+    rows = [ln.split() for ln in open(tmp_path / "inlet_density.dat").read().splitlines()]
+    assert [int(r[0]) for r in rows] == list(range(20, 40))
+    vals = np.array([[float(r[1]), float(r[2])] for r in rows])
+    np.testing.assert_allclose(vals[:, 0], got[:, 0], rtol=0, atol=6e-17)
+    np.testing.assert_allclose(vals[:, 1], np.cumsum(got[:, 0]) / np.arange(1, 21), rtol=1e-14)
+    with Simulation(small_case("annulus", _abi.FP64)) as a:
+        a.init().step(2, stats=True)
+        with pytest.raises(MrlbmError) as e:
+            a.plane_means(0, 2)
+        assert e.value.status == _abi.ERR_UNSUPPORTED

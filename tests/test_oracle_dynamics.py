@@ -99,3 +99,22 @@ def test_annular_couette_profile_c1():
     gap = (types == _abi.NODE["BULK"]) & (r > Ri + 1.5) & (r < Ro - 1.5)
     err = np.max(np.abs(ut[gap] - analytic[gap])) / Ui
     assert err < 0.05, err
+
+
+def test_plane_means_follow_the_state_after_the_step():
+    """Inlet / outlet plane means (inlet_avg_density, outlet_avg_ux): the mean of rho0 + rho over
+    x = 0 and of u_x over x = nx - 2 of the state after each step queued with the statistics flag;
+    steps without the flag record zeros."""
+    from oracle.binding import Oracle
+    d = configs.cylinder(nx=48, ny=24, nz=4, diameter=6.0, xc=14.0, yc=12.0, u_max=0.05, re=40.0,
+                         precision=_abi.FP64)
+    o = Oracle(d).init()
+    o.step(3, _abi.STEP_STATS)
+    F = o.fields()
+    o.step(1, 0)
+    P = o.plane_means(0, 4)
+    assert np.array_equal(P[3], [0.0, 0.0])
+    rho_in = (d.rho0 + F[0][:, :, 0]).mean()
+    ux_out = F[1][:, :, d.nx - 2].mean()
+    assert abs(P[2, 0] - rho_in) < 1e-15 and abs(P[2, 1] - ux_out) < 1e-16
+    assert abs(P[2, 0] - 1.0) < 0.05 and P[0, 1] != P[1, 1]
