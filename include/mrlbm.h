@@ -216,6 +216,15 @@ int mrlbm_time_steps(mrlbm_sim *sim, int32_t nsteps, uint32_t flags, double *ms_
 int mrlbm_checkpoint_write(mrlbm_sim *sim, const char *path);
 int mrlbm_checkpoint_read(mrlbm_sim *sim, const char *path);
 
+/*
+ * Import a restart file written by the reference <- read_checkpoint (src/restart/restart_io.cu:
+ * 19-36): the iteration count and the 10 moment arrays of `real` (real_bytes 4 or 8) in the
+ * reference's 8 x 8 x 4 block-tiled order; the trailing ghost-plane arrays are size-checked and
+ * skipped (they are the collided populations of the stored moments). The file must match this
+ * case's grid exactly (MRLBM_ERR_CORRUPT otherwise); the step count becomes the iteration.
+ */
+int mrlbm_checkpoint_import_reference(mrlbm_sim *sim, const char *path, int32_t real_bytes);
+
 int mrlbm_destroy(mrlbm_sim *sim);
 
 /*

@@ -167,6 +167,7 @@ PRODUCT_PROTOTYPES = {
     "mrlbm_write_forces": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int64, C.c_int32]),
     "mrlbm_write_pressure": (C.c_int, [C.c_void_p, C.c_char_p]),
     "mrlbm_get_plane_means": (C.c_int, [C.c_void_p, C.c_int64, C.c_int32, _dp]),
+    "mrlbm_checkpoint_import_reference": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int32]),
     "mrlbm_write_inlet_density": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int64, C.c_int32]),
     "mrlbm_last_error": (C.c_char_p, []),
     "mrlbm_abi_version": (C.c_int32, []),

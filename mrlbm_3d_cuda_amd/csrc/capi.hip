@@ -277,6 +277,57 @@ int mrlbm_checkpoint_read(mrlbm_sim *s, const char *path) {
     return s->impl->set_steps(h.steps);
 }
 
+// Restart file of the reference (write_checkpoint, src/restart/restart_io.cu:3-74): int iteration;
+// the 10 moment arrays of `real` over the 8 x 8 x 4 block-tiled layout (IDX_BLOCK,
+// src/core/index.cuh:11-28; src/common/types.cuh:10-12), padded to whole tiles; then the six
+// ghost-plane arrays (X_WEST, X_EAST, Y_SOUTH, Y_NORTH, Z_BACK, Z_FRONT, 9 populations per face
+// node). The ghost planes hold the collided populations of the stored moments, which this solver
+// reconstructs itself, so they are size-checked and skipped.
+int mrlbm_checkpoint_import_reference(mrlbm_sim *s, const char *path, int32_t real_bytes) {
+    OWN_CHECK(s);
+    if (!path || (real_bytes != 4 && real_bytes != 8)) return set_error(MRLBM_ERR_INVALID, "bad path or real size");
+    const mrlbm::Geometry &g = s->impl->g;
+    const int64_t BX = 8, BY = 8, BZ = 4;
+    const int64_t GX = (g.nx + BX - 1) / BX, GY = (g.ny + BY - 1) / BY, GZ = (g.nz + BZ - 1) / BZ;
+    const int64_t blocks = GX * GY * GZ, T = blocks * BX * BY * BZ;
+    const int64_t halo = 2 * 9 * blocks * (BY * BZ + BX * BZ + BX * BY);
+    const uint64_t expect = sizeof(int32_t) + uint64_t(real_bytes) * uint64_t(10 * T + halo);
+    FILE *fp = std::fopen(path, "rb");
+    if (!fp) return set_error(MRLBM_ERR_IO, std::string("cannot open ") + path);
+    std::vector<unsigned char> raw;
+    int32_t iter = 0;
+    bool ok = std::fseek(fp, 0, SEEK_END) == 0;
+    const long size = ok ? std::ftell(fp) : -1;
+    ok = ok && size >= 0 && uint64_t(size) == expect && std::fseek(fp, 0, SEEK_SET) == 0;
+    if (ok) ok = std::fread(&iter, sizeof(iter), 1, fp) == 1 && iter >= 0;
+    if (ok) {
+        raw.resize(size_t(10 * T) * size_t(real_bytes));
+        ok = std::fread(raw.data(), 1, raw.size(), fp) == raw.size();
+    }
+    std::fclose(fp);
+    if (!ok) return set_error(MRLBM_ERR_CORRUPT, "reference restart file: size or iteration mismatch for this grid");
+    std::vector<double> all(size_t(10) * size_t(g.n));
+    for (int f = 0; f < 10; ++f)
+        for (int64_t z = 0; z < g.nz; ++z)
+            for (int64_t y = 0; y < g.ny; ++y)
+                for (int64_t x = 0; x < g.nx; ++x) {
+                    const int64_t t = (x % BX) + BX * ((y % BY) + BY * (z % BZ));
+                    const int64_t b = (x / BX) + GX * ((y / BY) + GY * (z / BZ));
+                    const size_t src = size_t(f) * size_t(T) + size_t(t + BX * BY * BZ * b);
+                    double v;
+                    if (real_bytes == 8) {
+                        std::memcpy(&v, raw.data() + src * 8, 8);
+                    } else {
+                        float fv;
+                        std::memcpy(&fv, raw.data() + src * 4, 4);
+                        v = double(fv);
+                    }
+                    all[size_t(f) * size_t(g.n) + size_t(x + g.nx * (y + g.ny * z))] = v;
+                }
+    if (int rc = s->impl->set_all(all)) return rc;
+    return s->impl->set_steps(iter);
+}
+
 int mrlbm_stats_reset(mrlbm_sim *s) {
     DEV_CHECK(s);
     return s->impl->stats_reset();

@@ -57,6 +57,8 @@ class SimBase {
     virtual int stats_reset() = 0;
     virtual int get_surface_pressure(double *mean, int64_t *samples) = 0;
     virtual int get_plane_means(int64_t first, int count, double *out2) = 0;
+    // all 10 fields at once, x fastest, field 0 = rho - rho0 (both buffers)
+    virtual int set_all(const std::vector<double> &all10) = 0;
     // four float32 point arrays of a VTK file from the current state (MRLBM_VTI_* layout)
     virtual int vti_points(int layout, float vel_norm, float *host) = 0;
     virtual LaunchConfig current_launch() const = 0;
@@ -735,6 +737,13 @@ class Sim final : public SimBase {
         HIP_TRY(hipMalloc(&dptr, bytes));
         if (!h.empty()) HIP_TRY(hipMemcpy(dptr, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
         return MRLBM_OK;
+    }
+
+    int set_all(const std::vector<double> &all10) override {
+        if (all10.size() != size_t(10) * size_t(g.n)) return set_error(MRLBM_ERR_INVALID, "field data size");
+        if (int rc = wait_batch()) return rc;
+        HIP_TRY(hipStreamSynchronize(stream_));
+        return put_all(all10);
     }
 
     int put_all(const std::vector<double> &h) {

@@ -147,6 +147,10 @@ class Simulation:
     def checkpoint_read(self, path):
         _check(self.lib.mrlbm_checkpoint_read(self.h, os.fsencode(path)))
 
+    def checkpoint_import_reference(self, path, real_bytes=4):
+        """Load a restart.bin written by the reference (block-tiled moments, `real` of real_bytes)."""
+        _check(self.lib.mrlbm_checkpoint_import_reference(self.h, os.fsencode(path), real_bytes))
+
     # -- output
     def default_vti_layout(self):
         return _abi.VTI_ANNULUS if self.desc.kind == _abi.CASE_ANNULUS else _abi.VTI_CARTESIAN

@@ -121,3 +121,52 @@ def test_wall_pressure_statistics(precision):
     err = float(np.max(np.abs(ps - ref)) / np.max(np.abs(ref)))
     print(f"[gpu stats] {_abi.PRECISION_NAMES[precision]}: wall-pressure mean rel. error {err:.3e}", flush=True)
     assert err <= (1e-12 if precision == _abi.FP64 else 1e-4)
+
+
+def write_reference_restart(path, fields, iteration, dtype, seed=3):
+    """A restart file in the reference's layout (write_checkpoint, restart_io.cu:3-74): int
+    iteration, the 10 moment arrays over 8 x 8 x 4 tiles (IDX_BLOCK order, padded to whole
+    tiles), then the six ghost-plane arrays (random here: the import skips them)."""
+    _, nz, ny, nx = fields.shape
+    GX, GY, GZ = -(-nx // 8), -(-ny // 8), -(-nz // 4)
+    blocks = GX * GY * GZ
+    z, y, x = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    idx = (x % 8 + 8 * (y % 8 + 8 * (z % 4))) + 256 * (x // 8 + GX * (y // 8 + GY * (z // 4)))
+    out = np.zeros((10, blocks * 256), dtype)
+    out[:, idx.ravel()] = fields.reshape(10, -1)
+    halo = np.random.default_rng(seed).standard_normal(18 * blocks * (32 + 32 + 64)).astype(dtype)
+    with open(path, "wb") as fh:
+        fh.write(np.int32(iteration).tobytes())
+        fh.write(out.tobytes())
+        fh.write(halo.tobytes())
+
+
+@pytest.mark.parametrize("precision,dtype", [(_abi.FP64, np.float64), (_abi.FP32, np.float32)])
+@pytest.mark.parametrize("shape", [(8, 16, 40), (6, 20, 36)])
+def test_import_reference_restart(tmp_path, precision, dtype, shape):
+    from mrlbm_3d_cuda_amd.solver import MrlbmError, Simulation
+    nz, ny, nx = shape
+    d = configs.cylinder(nx=nx, ny=ny, nz=nz, diameter=6.0, xc=12.0, yc=ny / 2.0, u_max=0.04, re=40.0,
+                         precision=precision)
+    rng = np.random.default_rng(7)
+    F = rng.uniform(-0.01, 0.01, (10,) + shape).astype(dtype)
+    path = str(tmp_path / "restart.bin")
+    write_reference_restart(path, F, 1234, dtype)
+    with Simulation(d) as g:
+        g.init()
+        g.checkpoint_import_reference(path, np.dtype(dtype).itemsize)
+        assert g.info().steps_done == 1234
+        assert np.array_equal(g.fields(), F.astype(np.float64))
+        g.step(3)
+        assert g.info().steps_done == 1237 and np.isfinite(g.fields()).all()
+        for bad, status in ((lambda: g.checkpoint_import_reference(path, 12 - np.dtype(dtype).itemsize),
+                             _abi.ERR_CORRUPT),
+                            (lambda: g.checkpoint_import_reference(str(tmp_path / "none.bin")), _abi.ERR_IO)):
+            with pytest.raises(MrlbmError) as e:
+                bad()
+            assert e.value.status == status
+        with open(path, "r+b") as fh:
+            fh.truncate(64)
+        with pytest.raises(MrlbmError) as e:
+            g.checkpoint_import_reference(path, np.dtype(dtype).itemsize)
+        assert e.value.status == _abi.ERR_CORRUPT
