@@ -55,11 +55,6 @@ class Batch final : public BatchBase {
                 for (int b = 0; b < nb; ++b) blocks.push_back(make_int2(int(c), b));
             }
             nwblocks_[f] = int(blocks.size());
-            std::vector<int> cases_of;  // cases with a wall pass
-            for (size_t c = 0; c < nc; ++c)
-                if (sim(c).has_wall_launch(flags)) cases_of.push_back(int(c));
-            nwcases_[f] = int(cases_of.size());
-            if (int rc = upload(d_wcases_[f], cases_of)) return rc;
             std::vector<int2> order(blocks.size());  // dispatch slot -> block (XCD-contiguous)
             for (size_t b = 0; b < blocks.size(); ++b) order[b] = blocks[size_t(xcd_contiguous(int(b), int(blocks.size())))];
             int rc = upload(d_wblocks_[f], order);
@@ -140,6 +135,9 @@ class Batch final : public BatchBase {
             queue_step(flags, cur_);
             advance(flags);
         }
+        // the last step's force records are filed here (the fused launch of a step files the
+        // previous step's)
+        if (flags & MRLBM_STEP_FORCES) launch_record_batch<R, S>(d_fargs_[cur_], int(cases.size()), stream_);
         HIP_TRY(hipGetLastError());
         return MRLBM_OK;
     }
@@ -244,11 +242,11 @@ class Batch final : public BatchBase {
 
     // One step from buffer parity `cur` (the caller advances the parity).
     void queue_step(uint32_t flags, int cur) {
-        launch_fused_batch<R, S>(d_fargs_[cur], d_items_, nitems_, groups_, launch.bx, launch.by, launch.variant,
-                                 stream_);
+        launch_fused_batch<R, S>(d_fargs_[cur], int(cases.size()), d_items_, nitems_, groups_, launch.bx, launch.by,
+                                 launch.variant, stream_);
         const int f = (flags & MRLBM_STEP_FORCES) ? 1 : 0;
         if (nwblocks_[f] > 0)
-            launch_wall_batch<R, S>(d_wargs_[cur][f], d_wblocks_[f], nwblocks_[f], d_wcases_[f], nwcases_[f], stream_);
+            launch_wall_batch<R, S>(d_wargs_[cur][f], d_wblocks_[f], nwblocks_[f], stream_);
         if (flags & MRLBM_STEP_STATS)  // wall-pressure and plane statistics, launches per case
             for (size_t c = 0; c < cases.size(); ++c) {
                 if (sim(c).stats_on(flags)) launch_pressure<R, S>(sim(c).pressure_args(cur), stream_);
@@ -292,7 +290,7 @@ class Batch final : public BatchBase {
         if (stream_) (void)hipStreamSynchronize(stream_);
         drop_graphs();
         void *ptrs[] = {d_fargs_[0], d_fargs_[1], d_wargs_[0][0], d_wargs_[0][1], d_wargs_[1][0], d_wargs_[1][1],
-                        d_wblocks_[0], d_wblocks_[1], d_wcases_[0], d_wcases_[1], d_items_};
+                        d_wblocks_[0], d_wblocks_[1], d_items_};
         for (void *p : ptrs)
             if (p) (void)hipFree(p);
         for (auto &c : cases) c->impl->batch_stream = nullptr;
@@ -307,8 +305,6 @@ class Batch final : public BatchBase {
     WallArgs<R, S> *d_wargs_[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
     int2 *d_wblocks_[2] = {nullptr, nullptr};
     int nwblocks_[2] = {0, 0};
-    int *d_wcases_[2] = {nullptr, nullptr};
-    int nwcases_[2] = {0, 0};
     uint4 *d_items_ = nullptr;
     int nitems_ = 0, groups_ = 1;
     hipGraphExec_t graphs_[8] = {};

@@ -12,6 +12,16 @@
 
 namespace mrlbm {
 
+// Per-step force / mass records. The wall pass of a step leaves one FP64 partial per block and
+// marks the record pending; the next launch on the case's stream (the next fused pass, or the
+// record launch that ends a step call) sums the partials in a fixed order into the ring.
+struct RecordArgs {
+    unsigned long long *__restrict__ pending;  // step + 1 of the record waiting (0: none); null: no records
+    const double *__restrict__ partials;       // 4 per wall block
+    double *__restrict__ ring;                 // 4 per step, FORCE_RING_STEPS records
+    int nblocks;                               // wall blocks of a step with forces
+};
+
 // Arguments of the fused stream/collide pass over one ping-pong buffer pair.
 template <class R, class S>
 struct FusedArgs {
@@ -33,12 +43,16 @@ struct FusedArgs {
     const uint32_t *__restrict__ bcf_in;  // 256 per-tag incoming masks
     const R *__restrict__ inlet;          // u_x per (y, z); inlet_count entries (at least 1)
     int wall_count, inlet_count;
-    // steps completed on the device (advanced by wall_step); seeds stochastic rounding of the
-    // 16-bit forms, null for round-to-nearest
+    // index of this step on the device (the step counter of this buffer parity, set by the
+    // previous step's wall pass); seeds stochastic rounding of the 16-bit forms, null for
+    // round-to-nearest
     const unsigned long long *__restrict__ step_counter;
     StoreScale<R> scale;
     R omega, rho0;
     int bcf_strong;  // BCFLUID density from the Strong quadratic
+    // records of the case: with a non-null `pending` the launch has one block more, which files
+    // the previous step's record
+    RecordArgs rec;
 };
 
 // Arguments of the curved-wall + force pass (one thread per wall node, then one per BCFLUID).
@@ -69,8 +83,12 @@ struct WallArgs {
     int strong;                     // MassBC::Strong closure instead of Equilibrium
     int nblocks;                    // blocks of this case in the wall pass
     double *__restrict__ partials;  // 4 per block
-    double *__restrict__ ring;      // 4 per step, FORCE_RING_STEPS records
-    unsigned long long *__restrict__ step_counter;  // steps completed on the device
+    // Step counters, one per buffer parity: this step reads its index from `step_counter` and
+    // sets the other parity's counter, read by the next step, to index + 1. No launch writes a
+    // counter that its own blocks read.
+    const unsigned long long *__restrict__ step_counter;
+    unsigned long long *__restrict__ next_counter;
+    unsigned long long *__restrict__ pending;  // set to index + 1 on steps with forces
 };
 
 // Arguments of the wall-pressure statistics pass (one thread per inner-wall node).
@@ -88,7 +106,7 @@ struct PressureArgs {
 
 // Arguments of the plane-mean statistics pass (one block): mean density on the inlet plane x = 0
 // and mean u_x on the plane x = nx - 2 of the state after the step, filed in the plane ring at the
-// step the wall finisher just completed.
+// step's index.
 template <class R, class S>
 struct PlaneArgs {
     const S *__restrict__ rho;  // stored density deviation field
@@ -96,7 +114,7 @@ struct PlaneArgs {
     int nx, ny, nz;
     double rho0;
     StoreScale<R> scale;
-    const unsigned long long *__restrict__ step_counter;  // already advanced past this step
+    const unsigned long long *__restrict__ step_counter;  // the step's index (its parity's counter)
     double *__restrict__ ring;                             // 2 per step, FORCE_RING_STEPS records
 };
 
@@ -130,17 +148,23 @@ void launch_pressure(const PressureArgs<R, S> &a, hipStream_t stream);
 template <class R, class S>
 void launch_planes(const PlaneArgs<R, S> &a, hipStream_t stream);
 
+// Files a pending force record (one block; nothing when none is pending).
+void launch_record(const RecordArgs &r, hipStream_t stream);
+
 // Batched launches: one fused pass over the work items of many cases (item {case, tile |
-// special << 31, z0 | z1 << 16, 0}; tiles BX x BY in {32x8, 16x8, 32x4}), and one wall pass
-// whose block b runs block blocks[b].y of case blocks[b].x, followed by one block per case in
-// cases_of that files the case's per-step record. Argument arrays live on the device.
+// special << 31, z0 | z1 << 16, 0}; tiles BX x BY in {32x8, 16x8, 32x4}) plus one block per case
+// that files the case's pending record, and one wall pass whose block b runs block blocks[b].y
+// of case blocks[b].x. Argument arrays live on the device.
 template <class R, class S>
-void launch_fused_batch(const FusedArgs<R, S> *cases, const uint4 *items, int nitems, int groups, int bx, int by,
-                        int variant, hipStream_t stream);
+void launch_fused_batch(const FusedArgs<R, S> *cases, int ncases, const uint4 *items, int nitems, int groups, int bx,
+                        int by, int variant, hipStream_t stream);
+
+// Files the pending records of the cases (one block per case).
+template <class R, class S>
+void launch_record_batch(const FusedArgs<R, S> *cases, int ncases, hipStream_t stream);
 
 template <class R, class S>
-void launch_wall_batch(const WallArgs<R, S> *cases, const int2 *blocks, int nblocks, const int *cases_of,
-                       int ncases, hipStream_t stream);
+void launch_wall_batch(const WallArgs<R, S> *cases, const int2 *blocks, int nblocks, hipStream_t stream);
 
 // Sets the dynamic-LDS limit of the batched fused kernel for this configuration (once, before any
 // launch or graph capture); 0 on success.

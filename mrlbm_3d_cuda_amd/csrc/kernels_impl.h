@@ -35,6 +35,28 @@ __device__ __forceinline__ void lds_barrier() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
+// Files the pending force / mass record of a case: the wall blocks' partials summed in a fixed
+// order (lane t adds blocks t, t + 64, ..., then the wave tree) into the ring slot of the step
+// that left them. Run by one whole wave, in the launch after that wall pass (stream order makes
+// the partials visible to plain loads); the pending mark is then cleared.
+__device__ __forceinline__ void file_record(const RecordArgs &r) {
+    const unsigned long long pend = *r.pending;
+    if (pend == 0) return;
+    double tot[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int blk = int(threadIdx.x); blk < r.nblocks; blk += 64)
+        for (int k = 0; k < 4; ++k) tot[k] += r.partials[4 * blk + k];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) tot[k] += __shfl_xor(tot[k], off, 64);
+    if (threadIdx.x == 0) {
+        const size_t slot = size_t((pend - 1ull) % FORCE_RING_STEPS);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) r.ring[4 * slot + k] = tot[k];
+        *r.pending = 0;
+    }
+}
+
 __device__ __forceinline__ int wrap(int v, int n) {
     v = v < 0 ? v + n : v;
     return v >= n ? v - n : v;
@@ -191,9 +213,18 @@ struct BoundaryPath {
 #pragma unroll
         for (int k = 0; k < 10; ++k) dst[k] = raw[k];
     }
+    // The donor snapshot is encoded only in waves that hold a donor node; the others still issue
+    // its stores (into the padding slot) so that every path keeps the same memory operations.
     __device__ void store_prev(const FusedArgs<R, S> &a, uint32_t seed) {
         store_node<R, S>(a.dst, idxp, a.scale, a.rho0, Mp, seed);
-        store_node_nm<R, S>(a.donor, didxp, a.scale, a.rho0, Mdp, seed ^ 0x5A5A5A5Au);
+        S v[10];
+        if (__any(didxp != a.pad)) {
+            encode_node<R, S>(didxp, a.scale, a.rho0, Mdp, seed ^ 0x5A5A5A5Au, v);
+        } else {
+#pragma unroll
+            for (int k = 0; k < 10; ++k) v[k] = Conv<S>::from(R(0));
+        }
+        store_raw_nm(a.donor, didxp, v);
     }
     // the x-1 neighbour's moments at the plane being updated: lane - 1 inside the tile row, the
     // west ring column (LDS) for the first column
@@ -206,11 +237,18 @@ struct BoundaryPath {
     }
     __device__ void update(const FusedArgs<R, S> &a, R *f, int tx, int ty, bool inside, bool east_tile, uint32_t node,
                            const R *wold) {
+        // the node's own and x-1 moments are only read by boundary nodes: waves made of bulk
+        // nodes skip their decode
         R old[10], oxm[10];
-        decode_node<R, S>(own_cur, a.scale, a.rho0, old);
+        if (__any((info_cur & INFO_CODE_MASK) != NC_BULK)) {
+            decode_node<R, S>(own_cur, a.scale, a.rho0, old);
 #pragma unroll
-        for (int k = 0; k < 10; ++k) oxm[k] = old[k];
-        if (east_tile) neighbour_xm1(old, tx, ty, wold, oxm);  // uniform per block
+            for (int k = 0; k < 10; ++k) oxm[k] = old[k];
+            if (east_tile) neighbour_xm1(old, tx, ty, wold, oxm);  // uniform per block
+        } else {
+#pragma unroll
+            for (int k = 0; k < 10; ++k) old[k] = oxm[k] = R(0);
+        }
         bool store, donor;
         boundary_node(a, f, old, oxm, info_cur, tab, Mp, Mdp, store, donor);
         idxp = (inside && store) ? node : a.pad;
@@ -379,7 +417,8 @@ __device__ __forceinline__ void fused_march(const FusedArgs<R, S> &a, R *lds, in
 // NBUF = 1 needs two barriers per plane and a quarter of the LDS of the ring march (more blocks
 // per CU); NBUF = 2 alternates two LDS planes and needs one barrier per plane. The z loop is
 // unrolled by two so the two c_z=+1 windows swap roles without register moves. Memory
-// operations are ordered and made unconditional as in fused_march.
+// operations are ordered and made unconditional as in fused_march. The prefetch past the item's
+// last halo plane z1 re-reads plane z1 (an L2 hit) instead of fetching a plane nobody uses.
 template <class R, class S, int BX, int BY, int NBUF, bool SPECIAL>
 __device__ __forceinline__ void window_march(const FusedArgs<R, S> &a, R *lds, int tile, int z0, int z1) {
     using F = Frame<BX, BY>;
@@ -403,11 +442,13 @@ __device__ __forceinline__ void window_march(const FusedArgs<R, S> &a, R *lds, i
                                : off0;
     const bool west1 = has1 && hx1 == 0 && hy1 >= 1 && hy1 <= BY;
 
-    S r0[10], r1[10];  // stored values of the plane in flight, per position
-    auto load_plane = [&](int z) {
+    // stored values of the plane in flight, per position (set B only holds plane z0 in the
+    // prologue, which loads it together with plane z0-1)
+    S ra0[10], ra1[10], rb0[10], rb1[10];
+    auto load_plane_to = [&](int z, S *d0, S *d1) {
         const uint32_t base = uint32_t(pmod(z, a.nz)) * plane;
-        load_raw(a.src, base + off0, r0);
-        load_raw(a.src, base + off1, r1);
+        load_raw(a.src, base + off0, d0);
+        load_raw(a.src, base + off1, d1);
     };
     // windows, per position: c_z=+1 groups of two planes (A, B), c_z=0 group of one plane
     R pa0[9], pb0[9], q0[9], pa1[9], pb1[9], q1[9];
@@ -422,27 +463,29 @@ __device__ __forceinline__ void window_march(const FusedArgs<R, S> &a, R *lds, i
     };
 
     BoundaryPath<R, S, BX, BY> bp;
-    // prologue: +1 group of plane z0-1 into A, +1 and 0 groups of plane z0 into B and q
+    // prologue: +1 group of plane z0-1 into A, +1 and 0 groups of plane z0 into B and q (the
+    // two planes are loaded together, then plane z0+1 while plane z0 is reconstructed)
     {
         R f[Q];
-        load_plane(z0 - 1);
-        recon(r0, f, false, z0 - 1);
+        load_plane_to(z0 - 1, ra0, ra1);
+        load_plane_to(z0, rb0, rb1);
+        recon(ra0, f, false, z0 - 1);
 #pragma unroll
         for (int i = 0; i < 9; ++i) pa0[i] = f[GROUP_ZP[i]];
         if (has1) {
-            recon(r1, f, false, z0 - 1);
+            recon(ra1, f, false, z0 - 1);
 #pragma unroll
             for (int i = 0; i < 9; ++i) pa1[i] = f[GROUP_ZP[i]];
         }
-        load_plane(z0);
-        recon(r0, f, false, z0);
+        load_plane_to(z0 + 1, ra0, ra1);
+        recon(rb0, f, false, z0);
 #pragma unroll
         for (int i = 0; i < 9; ++i) {
             pb0[i] = f[GROUP_ZP[i]];
             q0[i] = f[GROUP_Z0[i]];
         }
         if (has1) {
-            recon(r1, f, west1, z0);
+            recon(rb1, f, west1, z0);
 #pragma unroll
             for (int i = 0; i < 9; ++i) {
                 pb1[i] = f[GROUP_ZP[i]];
@@ -450,11 +493,10 @@ __device__ __forceinline__ void window_march(const FusedArgs<R, S> &a, R *lds, i
             }
         }
         if constexpr (SPECIAL) {
-            bp.copy_own(r0, bp.own_cur);
+            bp.copy_own(rb0, bp.own_cur);
             bp.load_info(a, off0 + uint32_t(z0) * plane, bp.info_cur);
             bp.idxp = bp.didxp = a.pad;
         }
-        load_plane(z0 + 1);
     }
 
     const int x = x0 + tx, y = y0 + ty;
@@ -486,16 +528,16 @@ __device__ __forceinline__ void window_march(const FusedArgs<R, S> &a, R *lds, i
     auto iteration = [&](int z, R *G, R *w0, R *w1) {
         if (NBUF == 1 && z > z0) lds_barrier();  // previous plane's pulls are done
         if constexpr (SPECIAL) load_tabs(a, bp.info_cur, uint32_t(y) + uint32_t(a.ny) * uint32_t(z), bp.tab);
-        put(r0, h, G, w0, q0, false, z + 1);
-        if (has1) put(r1, p1, G, w1, q1, west1, z + 1);
+        put(ra0, h, G, w0, q0, false, z + 1);
+        if (has1) put(ra1, p1, G, w1, q1, west1, z + 1);
         if constexpr (SPECIAL) {
-            bp.copy_own(r0, bp.own_nx);
+            bp.copy_own(ra0, bp.own_nx);
             bp.store_prev(a, seed);
             bp.load_info(a, off0 + uint32_t(pmod(z + 1, a.nz)) * plane, bp.info_nx);
         } else {
             store_node<R, S>(a.dst, idxp, a.scale, a.rho0, Mp, seed);
         }
-        load_plane(z + 2);  // wraps past the item
+        load_plane_to(min(z + 2, z1), ra0, ra1);  // consumed in the next iteration
         lds_barrier();
         R f[Q];
 #pragma unroll
@@ -535,6 +577,10 @@ __global__ __launch_bounds__(BX *BY, (NBUF == 1 ? 3 : 2)) void fused_window_kern
     extern __shared__ unsigned char lds_raw[];
     R *lds = reinterpret_cast<R *>(lds_raw);
     int b = blockIdx.x;
+    if (b >= a.nblocks) {  // the block after the work items files the previous step's record
+        if (threadIdx.x < 64) file_record(a.rec);
+        return;
+    }
     if (a.xcd_groups > 1) b = (b % a.xcd_groups) * (a.nblocks / a.xcd_groups) + b / a.xcd_groups;
     const uint2 item = a.items[b];
     const int tile = int(item.x & 0x7FFFFFFFu);
@@ -555,6 +601,10 @@ __global__ __launch_bounds__(BX *BY) void fused_step_ring6_kernel(FusedArgs<R, S
     // block -> work item; consecutive items share an XCD so that neighbouring tiles reuse each
     // other's ring loads from the same L2
     int b = blockIdx.x;
+    if (b >= a.nblocks) {  // the block after the work items files the previous step's record
+        if (threadIdx.x < 64) file_record(a.rec);
+        return;
+    }
     if (a.xcd_groups > 1) b = (b % a.xcd_groups) * (a.nblocks / a.xcd_groups) + b / a.xcd_groups;
     const uint2 item = a.items[b];
     const int tile = int(item.x & 0x7FFFFFFFu);
@@ -576,6 +626,11 @@ __global__ __launch_bounds__(BX *BY, (VARIANT == 3 ? 3 : 2)) void fused_batch_ke
     extern __shared__ unsigned char lds_raw[];
     R *lds = reinterpret_cast<R *>(lds_raw);
     int b = blockIdx.x;
+    if (b >= nitems) {  // blocks after the work items file the cases' previous records
+        const RecordArgs r = cases[b - nitems].rec;
+        if (r.pending && threadIdx.x < 64) file_record(r);
+        return;
+    }
     if (groups > 1) b = (b % groups) * (nitems / groups) + b / groups;
     const uint4 item = items[b];
     const int z0 = int(item.z & 0xFFFFu);
@@ -728,49 +783,25 @@ __device__ __forceinline__ void wall_body(const WallArgs<R, S> &a, int lb) {
             a.partials[4 * lb + 3] = p3;
         }
     }
-}
-
-// Per-step record of one case: the block partials of the wall pass summed in a fixed order
-// (thread t adds blocks t, t + WALL_THREADS, ..., then the block tree), filed under the device
-// step counter, which is then advanced. Runs after the wall pass, so plain loads see its
-// partials.
-template <class R, class S>
-__device__ __forceinline__ void wall_finish_body(const WallArgs<R, S> &a) {
-    const unsigned long long step = *a.step_counter;
-    if (a.forces) {
-        double tot[4] = {0.0, 0.0, 0.0, 0.0};
-        for (int blk = int(threadIdx.x); blk < a.nblocks; blk += WALL_THREADS)
-            for (int k = 0; k < 4; ++k) tot[k] += a.partials[4 * blk + k];
-        const double s0 = wall_block_sum(tot[0]), s1 = wall_block_sum(tot[1]), s2 = wall_block_sum(tot[2]),
-                     s3 = wall_block_sum(tot[3]);
-        if (threadIdx.x == 0) {
-            const size_t slot = size_t(step % FORCE_RING_STEPS);
-            a.ring[4 * slot + 0] = s0;
-            a.ring[4 * slot + 1] = s1;
-            a.ring[4 * slot + 2] = s2;
-            a.ring[4 * slot + 3] = s3;
-        }
+    // the next step's index, and this step's record marked pending
+    if (lb == 0 && threadIdx.x == 0) {
+        const unsigned long long step = *a.step_counter;
+        *a.next_counter = step + 1;
+        if (a.forces) *a.pending = step + 1;
     }
-    if (threadIdx.x == 0) *a.step_counter = step + 1;
 }
 
+__global__ __launch_bounds__(64) void record_kernel(RecordArgs r) { file_record(r); }
+
+template <class R, class S>
+__global__ __launch_bounds__(64) void record_batch_kernel(const FusedArgs<R, S> *__restrict__ cases) {
+    const RecordArgs r = cases[blockIdx.x].rec;
+    if (r.pending) file_record(r);
+}
 
 template <class R, class S>
 __global__ __launch_bounds__(WALL_THREADS) void wall_step_kernel(WallArgs<R, S> a) {
     wall_body(a, xcd_contiguous(int(blockIdx.x), int(gridDim.x)));
-}
-
-template <class R, class S>
-__global__ __launch_bounds__(WALL_THREADS) void wall_finish_kernel(WallArgs<R, S> a) {
-    wall_finish_body(a);
-}
-
-// Per-step records of a batch: block c finishes case cases_of[c].
-template <class R, class S>
-__global__ __launch_bounds__(WALL_THREADS) void wall_finish_batch_kernel(const WallArgs<R, S> *__restrict__ cases,
-                                                                         const int *__restrict__ cases_of) {
-    const WallArgs<R, S> a = cases[cases_of[blockIdx.x]];
-    wall_finish_body(a);
 }
 
 // Wall pass of a batch: block b runs block blocks[b].y of case blocks[b].x.
@@ -888,7 +919,7 @@ void launch_fused(const FusedArgs<R, S> &a, int bx, int by, int variant, hipStre
                 done.push_back(k);
             }
         }
-        hipLaunchKernelGGL(kernel, dim3(a.nblocks), dim3(nt), lds, stream, a);
+        hipLaunchKernelGGL(kernel, dim3(a.nblocks + (a.rec.pending ? 1 : 0)), dim3(nt), lds, stream, a);
     };
 #define MRLBM_FUSED_TILE(TX, TY)                                                     \
     if (bx == TX && by == TY) {                                                      \
@@ -938,26 +969,28 @@ int prepare_fused_batch(int bx, int by, int variant) {
 }
 
 template <class R, class S>
-void launch_fused_batch(const FusedArgs<R, S> *cases, const uint4 *items, int nitems, int groups, int bx, int by,
-                        int variant, hipStream_t stream) {
+void launch_fused_batch(const FusedArgs<R, S> *cases, int ncases, const uint4 *items, int nitems, int groups, int bx,
+                        int by, int variant, hipStream_t stream) {
     const size_t lds = fused_lds_bytes(bx, by, sizeof(R), variant);
     auto kernel = fused_batch_kernel_for<R, S>(bx, by, variant);
-    if (kernel) hipLaunchKernelGGL(kernel, dim3(nitems), dim3(bx * by), lds, stream, cases, items, nitems, groups);
+    if (kernel)
+        hipLaunchKernelGGL(kernel, dim3(nitems + ncases), dim3(bx * by), lds, stream, cases, items, nitems, groups);
 }
 
 template <class R, class S>
-void launch_wall_batch(const WallArgs<R, S> *cases, const int2 *blocks, int nblocks, const int *cases_of,
-                       int ncases, hipStream_t stream) {
+void launch_record_batch(const FusedArgs<R, S> *cases, int ncases, hipStream_t stream) {
+    if (ncases > 0) hipLaunchKernelGGL((record_batch_kernel<R, S>), dim3(ncases), dim3(64), 0, stream, cases);
+}
+
+template <class R, class S>
+void launch_wall_batch(const WallArgs<R, S> *cases, const int2 *blocks, int nblocks, hipStream_t stream) {
     hipLaunchKernelGGL((wall_batch_kernel<R, S>), dim3(nblocks), dim3(WALL_THREADS), 0, stream, cases, blocks);
-    hipLaunchKernelGGL((wall_finish_batch_kernel<R, S>), dim3(ncases), dim3(WALL_THREADS), 0, stream, cases,
-                       cases_of);
 }
 
 template <class R, class S>
 void launch_wall(const WallArgs<R, S> &a, hipStream_t stream) {
     const int blocks = wall_blocks(a.nwall, a.forces ? a.nbcf : 0);
     hipLaunchKernelGGL((wall_step_kernel<R, S>), dim3(blocks), dim3(WALL_THREADS), 0, stream, a);
-    hipLaunchKernelGGL((wall_finish_kernel<R, S>), dim3(1), dim3(WALL_THREADS), 0, stream, a);
 }
 
 template <class R, class S>
@@ -997,7 +1030,7 @@ __global__ __launch_bounds__(256) void plane_kernel(PlaneArgs<R, S> a) {
         __syncthreads();
     }
     if (t == 0) {
-        const size_t slot = size_t((*a.step_counter - 1ull) % FORCE_RING_STEPS);
+        const size_t slot = size_t(*a.step_counter % FORCE_RING_STEPS);
         a.ring[2 * slot + 0] = red[0][0] / double(n);
         a.ring[2 * slot + 1] = red[1][0] / double(n);
     }
@@ -1025,11 +1058,11 @@ void launch_decode(const FieldSet<const S> &in, double *out, uint32_t n, const S
     template void launch_wall<R, S>(const WallArgs<R, S> &, hipStream_t);                                   \
     template void launch_pressure<R, S>(const PressureArgs<R, S> &, hipStream_t);                           \
     template void launch_planes<R, S>(const PlaneArgs<R, S> &, hipStream_t);                                 \
-    template void launch_fused_batch<R, S>(const FusedArgs<R, S> *, const uint4 *, int, int, int, int, int,     \
+    template void launch_fused_batch<R, S>(const FusedArgs<R, S> *, int, const uint4 *, int, int, int, int, int, \
                                            hipStream_t);                                                    \
+    template void launch_record_batch<R, S>(const FusedArgs<R, S> *, int, hipStream_t);                      \
     template int prepare_fused_batch<R, S>(int, int, int);                                                  \
-    template void launch_wall_batch<R, S>(const WallArgs<R, S> *, const int2 *, int, const int *, int,        \
-                                          hipStream_t);                                                     \
+    template void launch_wall_batch<R, S>(const WallArgs<R, S> *, const int2 *, int, hipStream_t);            \
     template void launch_encode<R, S>(const double *, const FieldSet<S> &, uint32_t, const StoreScale<R> &,        \
                                       hipStream_t);                                                         \
     template void launch_decode<R, S>(const FieldSet<const S> &, double *, uint32_t, const StoreScale<R> &,  \

@@ -250,8 +250,10 @@ class Sim final : public SimBase {
         }
         HIP_TRY(hipMalloc(&d_ps_sum_, std::max<size_t>(g.wall[0].node.size(), 1) * sizeof(double)));
         HIP_TRY(hipMemsetAsync(d_ps_sum_, 0, std::max<size_t>(g.wall[0].node.size(), 1) * sizeof(double), stream_));
-        HIP_TRY(hipMalloc(&d_step_counter_, sizeof(unsigned long long)));
-        HIP_TRY(hipMemsetAsync(d_step_counter_, 0, sizeof(unsigned long long), stream_));
+        HIP_TRY(hipMalloc(&d_step_counter_, 2 * sizeof(unsigned long long)));
+        HIP_TRY(hipMemsetAsync(d_step_counter_, 0, 2 * sizeof(unsigned long long), stream_));
+        HIP_TRY(hipMalloc(&d_pending_, sizeof(unsigned long long)));
+        HIP_TRY(hipMemsetAsync(d_pending_, 0, sizeof(unsigned long long), stream_));
         HIP_TRY(hipMemsetAsync(d_ring_, 0, size_t(FORCE_RING_STEPS) * 4 * sizeof(double), stream_));
         HIP_TRY(hipMemsetAsync(d_plane_ring_, 0, size_t(FORCE_RING_STEPS) * 2 * sizeof(double), stream_));
         HIP_TRY(hipStreamSynchronize(stream_));
@@ -384,7 +386,7 @@ class Sim final : public SimBase {
         p.nz = g.nz;
         p.rho0 = d.rho0;
         p.scale = scale_;
-        p.step_counter = d_step_counter_;
+        p.step_counter = d_step_counter_ + cur;  // the index of the step the planes belong to
         p.ring = d_plane_ring_;
         return p;
     }
@@ -486,7 +488,8 @@ class Sim final : public SimBase {
         a.inlet = d_inlet_;
         a.wall_count = std::max(nwall_, 1);
         a.inlet_count = std::max<int>(int(g.inlet.size()), 1);
-        a.step_counter = stochastic_ ? d_step_counter_ : nullptr;
+        a.step_counter = stochastic_ ? d_step_counter_ + cur : nullptr;
+        a.rec = record_args();
         a.scale = scale_;
         a.omega = omega_;
         a.bcf_strong = (d.compat_flags & MRLBM_COMPAT_BCF_MASS_STRONG) ? 1 : 0;
@@ -525,8 +528,9 @@ class Sim final : public SimBase {
         w.forces = forces ? 1 : 0;
         w.nblocks = wall_blocks(nwall_, forces ? nbcf_ : 0);
         w.partials = d_partials_;
-        w.ring = d_ring_;
-        w.step_counter = d_step_counter_;
+        w.step_counter = d_step_counter_ + cur;
+        w.next_counter = d_step_counter_ + (cur ^ 1);
+        w.pending = d_pending_;
         return w;
     }
 
@@ -536,6 +540,20 @@ class Sim final : public SimBase {
         return nwall_ > 0 || ((flags & MRLBM_STEP_FORCES) && nbcf_ > 0) || Conv<S>::scaled;
     }
     bool forces_on(uint32_t flags) const { return (flags & MRLBM_STEP_FORCES) && g.ngroups > 0; }
+
+    // Force records of the case: the last wall pass of a step call leaves its record pending (the
+    // next fused launch would file it), so every step call ends with a record launch.
+    RecordArgs record_args() const {
+        RecordArgs r{};
+        r.pending = forces_on(MRLBM_STEP_FORCES) ? d_pending_ : nullptr;
+        r.partials = d_partials_;
+        r.ring = d_ring_;
+        r.nblocks = wall_blocks(nwall_, nbcf_);
+        return r;
+    }
+    void queue_record(uint32_t flags) {
+        if (forces_on(flags)) launch_record(record_args(), stream_);
+    }
 
     // Buffers swap after every step (driven by a batch that launches the kernels itself).
     void advance(uint32_t flags) {
@@ -580,14 +598,15 @@ class Sim final : public SimBase {
             }
         }
         for (; done < nsteps; ++done) queue_step(flags, nullptr);
+        queue_record(flags);
         HIP_TRY(hipGetLastError());
         return MRLBM_OK;
     }
 
     int set_steps(int64_t n) override {
         steps = n;
-        const unsigned long long v = static_cast<unsigned long long>(n);
-        HIP_TRY(hipMemcpy(d_step_counter_, &v, sizeof(v), hipMemcpyHostToDevice));
+        const unsigned long long v[2] = {static_cast<unsigned long long>(n), static_cast<unsigned long long>(n)};
+        HIP_TRY(hipMemcpy(d_step_counter_, v, sizeof(v), hipMemcpyHostToDevice));
         return MRLBM_OK;
     }
 
@@ -653,6 +672,7 @@ class Sim final : public SimBase {
         HIP_TRY(hipEventRecord(ev[0], stream_));
         if (ms_k) {
             for (int s = 0; s < n; ++s) queue_step(flags, &ev[2 + 4 * size_t(s)]);
+            queue_record(flags);
         } else {
             int rc = step(n, flags);
             if (rc) return rc;
@@ -778,7 +798,7 @@ class Sim final : public SimBase {
             if (g) (void)hipGraphExecDestroy(g);
         void *ptrs[] = {buf_[0], buf_[1], donor_, d_info_, d_items_, d_wconst_, d_wnx_, d_wny_, d_win_, d_wout_, d_wnode_,
                         d_wcoef_, d_stbase_, d_stfrac_, d_bcfnode_, d_bcftag_, d_bcfin_, d_bcfout_, d_inlet_,
-                        d_partials_, d_ring_, d_step_counter_, d_ps_sum_, d_plane_ring_};
+                        d_partials_, d_ring_, d_step_counter_, d_pending_, d_ps_sum_, d_plane_ring_};
         for (void *p : ptrs)
             if (p) (void)hipFree(p);
         if (stream_) (void)hipStreamDestroy(stream_);
@@ -813,7 +833,8 @@ class Sim final : public SimBase {
     double *d_partials_ = nullptr, *d_ring_ = nullptr, *d_ps_sum_ = nullptr, *d_plane_ring_ = nullptr;
     bool stats_ok_ = false;
     int64_t stats_samples_ = 0;
-    unsigned long long *d_step_counter_ = nullptr;
+    unsigned long long *d_step_counter_ = nullptr;  // step index, one counter per buffer parity
+    unsigned long long *d_pending_ = nullptr;       // step + 1 of a force record not yet filed
 };
 
 }  // namespace mrlbm

@@ -196,11 +196,8 @@ __device__ __forceinline__ void store_node(const FieldSet<S> &fs, uint32_t idx, 
 
 // Node-major buffers (the 10 values of node i at base[10 i .. 10 i + 9]): the donor snapshot,
 // read at scattered stencil corners, where one node's values then share a cache line.
-template <class R, class S>
-__device__ __forceinline__ void store_node_nm(S *__restrict__ base, uint32_t idx, const StoreScale<R> &sc, R rho0,
-                                              const R *M, uint32_t seed = 0) {
-    S v[10];
-    encode_node<R, S>(idx, sc, rho0, M, seed, v);
+template <class S>
+__device__ __forceinline__ void store_raw_nm(S *__restrict__ base, uint32_t idx, const S *v) {
 #pragma unroll
     for (int f = 0; f < 10; ++f) base[size_t(idx) * 10 + f] = v[f];
 }

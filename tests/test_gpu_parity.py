@@ -5,6 +5,7 @@ Bar (stated per test): FP64 runs match the FP64 oracle to 1e-12 relative L-inf p
 components); FP32 runs match the FP64 oracle within the FP32 bounds written in the test.
 """
 import os
+import re
 
 import numpy as np
 import pytest
@@ -211,6 +212,46 @@ def test_tile_shape_and_chunking_invariance():
     for F, R in results[1:]:
         assert np.array_equal(F, results[0][0])
         assert np.array_equal(R, results[0][1])
+
+
+def run_variant(desc, variant, steps):
+    from mrlbm_3d_cuda_amd.solver import Simulation
+
+    os.environ["MRLBM_FUSED_VARIANT"] = str(variant)
+    try:
+        s = Simulation(desc).init()
+    finally:
+        del os.environ["MRLBM_FUSED_VARIANT"]
+    assert re.search(rf"March variant\s*:?\s*{variant}\b", s.info_text()), s.info_text()
+    s.step(steps, forces=True)
+    out = (s.fields(), s.forces(0, steps))
+    s.close()
+    return out
+
+
+@pytest.mark.parametrize("z_ends", ["periodic", "walls"])
+def test_march_variants_fp64(z_ends):
+    """Every march variant (ring; register windows with one or two LDS planes) matches the FP64
+    oracle, with boundary tiles, two-plane z chunks and a one-plane last chunk (the prefetch past
+    a chunk's last halo plane is clamped to it)."""
+    from oracle.binding import Oracle
+
+    d = small_channel(_abi.FP64, nz=7)
+    if z_ends == "walls":
+        d = closed_z(d)
+    steps = 30
+    o = Oracle(d).init()
+    o.step(steps, _abi.STEP_FORCES)
+    O, FO = o.fields(), o.forces(0, steps)
+    os.environ["MRLBM_LZ"] = "2"
+    try:
+        for variant in (1, 3, 4):
+            print(f"[gpu parity] march variant {variant}", flush=True)
+            G, FG = run_variant(d, variant, steps)
+            assert_fp64_parity(G, O)
+            assert force_error(FG, FO, boundary_count(d)) <= 1e-13
+    finally:
+        del os.environ["MRLBM_LZ"]
 
 
 def test_deterministic_repeat():
