@@ -73,7 +73,7 @@ enum mrlbm_case_kind {
 enum mrlbm_precision {
     MRLBM_FP64 = 0,      /* store f64, compute f64 */
     MRLBM_FP32 = 1,      /* store f32, compute f32 (reference default: types.cuh:5) */
-    MRLBM_FP32_BF16 = 2, /* store bf16 scaled deviations, compute f32 */
+    MRLBM_FP32_BF16 = 2, /* store bf16 deviations, compute f32 */
     MRLBM_FP32_FP16 = 3  /* store fp16 scaled deviations, compute f32 */
 };
 

@@ -18,6 +18,9 @@ uint64_t fnv1a(const void *data, size_t n, uint64_t h = 1469598103934665603ull) 
     return h;
 }
 
+// Version 3: BF16 payloads hold unscaled deviations.
+constexpr uint32_t CHECKPOINT_VERSION = 3;
+
 struct CheckpointHeader {
     char magic[8];  // "MRLBMCK1"
     uint32_t version;
@@ -229,13 +232,13 @@ int mrlbm_checkpoint_write(mrlbm_sim *s, const char *path) {
     DEV_CHECK(s);
     if (!path) return set_error(MRLBM_ERR_INVALID, "null path");
     const mrlbm::Geometry &g = s->impl->g;
-    // payload: the moments exactly as stored (rho - rho0, u, m, or the scaled deviations of the
-    // 16-bit forms), so a restart continues bit for bit
+    // payload: the moments exactly as stored (rho - rho0, u, m, or the deviations of the 16-bit
+    // forms), so a restart continues bit for bit
     std::vector<unsigned char> payload(size_t(10) * g.n * s->impl->bytes_per_value);
     if (int rc = s->impl->get_raw(payload.data())) return rc;
     CheckpointHeader h{};
     std::memcpy(h.magic, "MRLBMCK1", 8);
-    h.version = 2;
+    h.version = CHECKPOINT_VERSION;
     h.precision = uint32_t(s->impl->d.precision);
     h.nx = g.nx;
     h.ny = g.ny;
@@ -261,7 +264,7 @@ int mrlbm_checkpoint_read(mrlbm_sim *s, const char *path) {
     CheckpointHeader h{};
     std::vector<unsigned char> payload(size_t(10) * g.n * s->impl->bytes_per_value);
     const bool head = std::fread(&h, sizeof(h), 1, fp) == 1;
-    bool ok = head && std::memcmp(h.magic, "MRLBMCK1", 8) == 0 && h.version == 2 &&
+    bool ok = head && std::memcmp(h.magic, "MRLBMCK1", 8) == 0 && h.version == CHECKPOINT_VERSION &&
               h.precision == uint32_t(s->impl->d.precision) && h.nx == g.nx && h.ny == g.ny && h.nz == g.nz &&
               h.fields == 10 && h.payload_bytes == payload.size();
     if (ok) ok = std::fread(payload.data(), 1, h.payload_bytes, fp) == h.payload_bytes;

@@ -97,9 +97,11 @@ class Sim final : public SimBase {
         omega_ = R(1.0 / d.tau);
         rho0_ = R(d.rho0);
         // 16-bit storage scales: deviations of O(1e-2) mapped to O(1)
-        scale_.enc[0] = R(64);
-        scale_.enc[1] = R(16);
-        scale_.enc[2] = R(256);
+        // 16-bit storage scales of the rho, u and m deviations (FP16 only; BF16 stores them as
+        // they are)
+        scale_.enc[0] = R(Conv<S>::unit_scale ? 1 : 64);
+        scale_.enc[1] = R(Conv<S>::unit_scale ? 1 : 16);
+        scale_.enc[2] = R(Conv<S>::unit_scale ? 1 : 256);
         for (int k = 0; k < 3; ++k) scale_.dec[k] = R(1) / scale_.enc[k];
 
         // ---- launch configuration of the fused pass (defaults, overridden by the environment)

@@ -1,8 +1,10 @@
 // storage.h - how the 10 moments of a node are kept in HBM.
 //
 // FP64/FP32 store the reference's fields as they are (rho - rho0, u, m; mlbm.cu:154-163).
-// The 16-bit forms store scaled deviations: (rho - rho0) * s_rho, u * s_u and the
-// non-equilibrium second moments (m - u u) * s_m, decoded in FP32.
+// The 16-bit forms store deviations: rho - rho0, u and the non-equilibrium second moments
+// m - u u, decoded in FP32. FP16 scales them by s_rho, s_u, s_m into its normal range; BF16 has
+// float's exponent range and stores them unscaled (the scales are powers of two, so scaling would
+// not change a single rounding).
 #pragma once
 
 #include <hip/hip_bf16.h>
@@ -26,6 +28,7 @@ struct Conv;
 template <>
 struct Conv<double> {
     static constexpr bool scaled = false;
+    static constexpr bool unit_scale = true;
     __device__ __host__ static double to(double v) { return v; }
     template <class R>
     __device__ __host__ static double from(R v) { return double(v); }
@@ -34,6 +37,7 @@ struct Conv<double> {
 template <>
 struct Conv<float> {
     static constexpr bool scaled = false;
+    static constexpr bool unit_scale = true;
     __device__ __host__ static float to(float v) { return v; }
     template <class R>
     __device__ __host__ static float from(R v) { return float(v); }
@@ -47,6 +51,7 @@ struct Conv<float> {
 template <>
 struct Conv<__hip_bfloat16> {
     static constexpr bool scaled = true;
+    static constexpr bool unit_scale = true;
     __device__ static float to(__hip_bfloat16 v) { return __bfloat162float(v); }
     __device__ static __hip_bfloat16 from(float v) { return __float2bfloat16(v); }
     __device__ static __hip_bfloat16 from_sr(float v, uint32_t r8) {
@@ -58,6 +63,7 @@ struct Conv<__hip_bfloat16> {
 template <>
 struct Conv<_Float16> {
     static constexpr bool scaled = true;
+    static constexpr bool unit_scale = false;
     __device__ static float to(_Float16 v) { return float(v); }
     __device__ static _Float16 from(float v) { return _Float16(v); }
     __device__ static _Float16 from_sr(float v, uint32_t r8) {
@@ -132,7 +138,10 @@ __device__ __forceinline__ void decode_node(const S *raw, const StoreScale<R> &s
         M[0] += rho0;
     } else {
 #pragma unroll
-        for (int f = 0; f < 10; ++f) M[f] = R(Conv<S>::to(raw[f])) * sc.dec[field_class(f)];
+        for (int f = 0; f < 10; ++f) {
+            M[f] = R(Conv<S>::to(raw[f]));
+            if constexpr (!Conv<S>::unit_scale) M[f] *= sc.dec[field_class(f)];
+        }
         M[0] += rho0;
         M[4] += M[1] * M[1];
         M[5] += M[2] * M[2];
@@ -173,13 +182,17 @@ __device__ __forceinline__ void encode_node(uint32_t idx, const StoreScale<R> &s
         v[7] = M[7] - M[1] * M[2];
         v[8] = M[8] - M[1] * M[3];
         v[9] = M[9] - M[2] * M[3];
+        if constexpr (!Conv<S>::unit_scale) {
+#pragma unroll
+            for (int f = 0; f < 10; ++f) v[f] *= sc.enc[field_class(f)];
+        }
         if (seed == 0) {
 #pragma unroll
-            for (int f = 0; f < 10; ++f) out[f] = Conv<S>::from(v[f] * sc.enc[field_class(f)]);
+            for (int f = 0; f < 10; ++f) out[f] = Conv<S>::from(v[f]);
         } else {
             const uint32_t h = sr_hash(idx, seed);
 #pragma unroll
-            for (int f = 0; f < 10; ++f) out[f] = Conv<S>::from_sr(v[f] * sc.enc[field_class(f)], sr_bits(h, f));
+            for (int f = 0; f < 10; ++f) out[f] = Conv<S>::from_sr(v[f], sr_bits(h, f));
         }
     }
 }

@@ -91,6 +91,29 @@ def test_checkpoint_round_trip_bitwise(tmp_path, precision):
     assert e.value.status == _abi.ERR_IO
 
 
+@pytest.mark.parametrize("precision", [_abi.FP32_BF16, _abi.FP32_FP16])
+def test_stored_deviation_layout(tmp_path, precision):
+    """The checkpoint payload holds the stored 16-bit values: BF16 keeps rho - rho0 and u as they
+    are, FP16 scales them by 64 and 16 (exact powers of two); the decoded fields equal them."""
+    from mrlbm_3d_cuda_amd.solver import Simulation
+    d = channel(precision)
+    with Simulation(d) as s:
+        s.init().step(12)
+        path = str(tmp_path / "ck.bin")
+        s.checkpoint_write(path)
+        F = s.fields()
+    n = d.nx * d.ny * d.nz
+    raw = np.frombuffer(open(path, "rb").read()[-20 * n:], dtype=np.uint16).reshape(10, n)
+    if precision == _abi.FP32_BF16:
+        stored = (raw.astype(np.uint32) << 16).view(np.float32).astype(np.float64)
+        scales = [1.0, 1.0, 1.0, 1.0]
+    else:
+        stored = raw.view(np.float16).astype(np.float64)
+        scales = [64.0, 16.0, 16.0, 16.0]
+    for f in range(4):
+        assert np.array_equal(stored[f] / scales[f], F[f].ravel()), f
+
+
 @pytest.mark.parametrize("precision", [_abi.FP64, _abi.FP32])
 def test_wall_pressure_statistics(precision):
     """Time-averaged wall pressure (FP64 sums): the mean over the statistics window equals the
