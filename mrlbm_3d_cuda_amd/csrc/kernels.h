@@ -135,9 +135,17 @@ __host__ __device__ inline int xcd_contiguous(int b, int n) {
     return j * q + (j < r ? j : r) + k;
 }
 
-// variant 1: ring march; 3, 4: register-window march with one / two LDS planes
+// Launches of the fused pass: every work item, or the interior / wall-side items of the
+// two-stream schedule.
+constexpr int PART_ALL = 0, PART_INTERIOR = 1, PART_WALL_SIDE = 2;
+
+// variant 1: ring march; 3, 4: register-window march with one / two LDS planes. part: PART_ALL,
+// or PART_INTERIOR / PART_WALL_SIDE where fused_part_supported.
 template <class R, class S>
-void launch_fused(const FusedArgs<R, S> &a, int bx, int by, int variant, hipStream_t stream);
+void launch_fused(const FusedArgs<R, S> &a, int bx, int by, int variant, int part, hipStream_t stream);
+
+template <class R, class S>
+bool fused_part_supported(int bx, int by, int variant);
 
 template <class R, class S>
 void launch_wall(const WallArgs<R, S> &a, hipStream_t stream);

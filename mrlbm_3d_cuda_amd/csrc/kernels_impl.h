@@ -571,8 +571,11 @@ __device__ __forceinline__ void window_march(const FusedArgs<R, S> &a, R *lds, i
     if (NBUF == 1) lds_barrier();  // the block's LDS is free again for the next item
 }
 
+// PART: which launch of the fused pass (PART_ALL, PART_INTERIOR, PART_WALL_SIDE). The interior
+// launch has no boundary tile, so the boundary path is not compiled into it; the two parts are
+// separate instantiations so that profiles tell the launches apart.
 // One LDS plane leaves room for three blocks per CU: registers are capped to match.
-template <class R, class S, int BX, int BY, int NBUF>
+template <class R, class S, int BX, int BY, int NBUF, int PART>
 __global__ __launch_bounds__(BX *BY, (NBUF == 1 ? 3 : 2)) void fused_window_kernel(FusedArgs<R, S> a) {
     extern __shared__ unsigned char lds_raw[];
     R *lds = reinterpret_cast<R *>(lds_raw);
@@ -588,13 +591,13 @@ __global__ __launch_bounds__(BX *BY, (NBUF == 1 ? 3 : 2)) void fused_window_kern
     const int z0 = int(item.y & 0xFFFFu);
     const int z1 = int(item.y >> 16);
     if (z0 >= z1) return;  // padding item (uniform per block)
-    if (special)
+    if (PART != PART_INTERIOR && special)
         window_march<R, S, BX, BY, NBUF, true>(a, lds, tile, z0, z1);
     else
         window_march<R, S, BX, BY, NBUF, false>(a, lds, tile, z0, z1);
 }
 
-template <class R, class S, int BX, int BY>
+template <class R, class S, int BX, int BY, int PART>
 __global__ __launch_bounds__(BX *BY) void fused_step_ring6_kernel(FusedArgs<R, S> a) {
     extern __shared__ unsigned char lds_raw[];
     R *lds = reinterpret_cast<R *>(lds_raw);
@@ -612,7 +615,7 @@ __global__ __launch_bounds__(BX *BY) void fused_step_ring6_kernel(FusedArgs<R, S
     const int z0 = int(item.y & 0xFFFFu);
     const int z1 = int(item.y >> 16);
     if (z0 >= z1) return;  // padding item (uniform per block)
-    if (special)
+    if (PART != PART_INTERIOR && special)
         fused_march<R, S, BX, BY, true>(a, lds, tile, z0, z1);
     else
         fused_march<R, S, BX, BY, false>(a, lds, tile, z0, z1);
@@ -905,7 +908,12 @@ __global__ void decode_kernel(FieldSet<const S> in, double *__restrict__ out, ui
 }  // namespace
 
 template <class R, class S>
-void launch_fused(const FusedArgs<R, S> &a, int bx, int by, int variant, hipStream_t stream) {
+bool fused_part_supported(int bx, int by, int variant) {
+    return ((bx == 32 && by == 8) || (bx == 16 && by == 8)) && (variant == 1 || variant == 4);
+}
+
+template <class R, class S>
+void launch_fused(const FusedArgs<R, S> &a, int bx, int by, int variant, int part, hipStream_t stream) {
     const size_t lds = fused_lds_bytes(bx, by, sizeof(R), variant);
     auto go = [&](void (*kernel)(FusedArgs<R, S>), int nt) {
         // kernels needing more than the default dynamic LDS limit get the attribute once each
@@ -921,23 +929,37 @@ void launch_fused(const FusedArgs<R, S> &a, int bx, int by, int variant, hipStre
         }
         hipLaunchKernelGGL(kernel, dim3(a.nblocks + (a.rec.pending ? 1 : 0)), dim3(nt), lds, stream, a);
     };
-#define MRLBM_FUSED_TILE(TX, TY)                                                     \
-    if (bx == TX && by == TY) {                                                      \
+#define MRLBM_FUSED_TILE(TX, TY, P)                                                  \
+    if (bx == TX && by == TY && part == P) {                                         \
         if (variant == 3)                                                            \
-            go(fused_window_kernel<R, S, TX, TY, 1>, TX * TY);                       \
+            go(fused_window_kernel<R, S, TX, TY, 1, P>, TX * TY);                    \
         else if (variant == 4)                                                       \
-            go(fused_window_kernel<R, S, TX, TY, 2>, TX * TY);                       \
+            go(fused_window_kernel<R, S, TX, TY, 2, P>, TX * TY);                    \
         else                                                                         \
-            go(fused_step_ring6_kernel<R, S, TX, TY>, TX * TY);                      \
+            go(fused_step_ring6_kernel<R, S, TX, TY, P>, TX * TY);                   \
         return;                                                                      \
     }
-    MRLBM_FUSED_TILE(32, 8)
-    MRLBM_FUSED_TILE(16, 8)
-    MRLBM_FUSED_TILE(32, 4)
-    MRLBM_FUSED_TILE(16, 16)
-    MRLBM_FUSED_TILE(64, 4)
-    MRLBM_FUSED_TILE(16, 4)
+    MRLBM_FUSED_TILE(32, 8, PART_ALL)
+    MRLBM_FUSED_TILE(16, 8, PART_ALL)
+    MRLBM_FUSED_TILE(32, 4, PART_ALL)
+    MRLBM_FUSED_TILE(16, 16, PART_ALL)
+    MRLBM_FUSED_TILE(64, 4, PART_ALL)
+    MRLBM_FUSED_TILE(16, 4, PART_ALL)
 #undef MRLBM_FUSED_TILE
+    // the two-stream schedule's launches: tiles 32x8 and 16x8, ring or two-plane window march
+#define MRLBM_FUSED_PART(TX, TY, P)                                                  \
+    if (bx == TX && by == TY && part == P) {                                         \
+        if (variant == 4)                                                            \
+            go(fused_window_kernel<R, S, TX, TY, 2, P>, TX * TY);                    \
+        else if (variant == 1)                                                       \
+            go(fused_step_ring6_kernel<R, S, TX, TY, P>, TX * TY);                   \
+        return;                                                                      \
+    }
+    MRLBM_FUSED_PART(32, 8, PART_INTERIOR)
+    MRLBM_FUSED_PART(16, 8, PART_INTERIOR)
+    MRLBM_FUSED_PART(32, 8, PART_WALL_SIDE)
+    MRLBM_FUSED_PART(16, 8, PART_WALL_SIDE)
+#undef MRLBM_FUSED_PART
 }
 
 // Kernel of the batched fused pass for a tile and variant (null if not instantiated).
@@ -1054,7 +1076,8 @@ void launch_decode(const FieldSet<const S> &in, double *out, uint32_t n, const S
 }  // namespace mrlbm
 
 #define MRLBM_INSTANTIATE(R, S)                                                                             \
-    template void launch_fused<R, S>(const FusedArgs<R, S> &, int, int, int, hipStream_t);                  \
+    template void launch_fused<R, S>(const FusedArgs<R, S> &, int, int, int, int, hipStream_t);             \
+    template bool fused_part_supported<R, S>(int, int, int);                                                \
     template void launch_wall<R, S>(const WallArgs<R, S> &, hipStream_t);                                   \
     template void launch_pressure<R, S>(const PressureArgs<R, S> &, hipStream_t);                           \
     template void launch_planes<R, S>(const PlaneArgs<R, S> &, hipStream_t);                                 \
