@@ -676,7 +676,7 @@ __device__ __forceinline__ void interp_rotate(const WallArgs<R, S> &a, uint32_t 
 #pragma unroll
         for (int k = 0; k < 9; ++k) {
             q[j][k] = R(Conv<S>::to(a.donor[size_t(corner[j]) * 10 + 1 + k]));
-            if constexpr (Conv<S>::scaled) q[j][k] *= a.scale.dec[field_class(k + 1)];
+            if constexpr (!Conv<S>::unit_scale) q[j][k] *= a.scale.dec[field_class(k + 1)];
         }
         if constexpr (Conv<S>::scaled) {
             // stored second moments are non-equilibrium parts: m = (m - u u) + u u is restored at
@@ -844,7 +844,7 @@ __global__ __launch_bounds__(256) void pressure_kernel(PressureArgs<R, S> a) {
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             q[j] = double(Conv<S>::to(ld_field(a.rho, c[j])));
-            if constexpr (Conv<S>::scaled) q[j] *= double(a.scale.dec[0]);
+            if constexpr (!Conv<S>::unit_scale) q[j] *= double(a.scale.dec[0]);
         }
         const double q0 = (1.0 - xd) * q[0] + xd * q[1];
         const double q1 = (1.0 - xd) * q[2] + xd * q[3];
