@@ -15,11 +15,13 @@ LIB = os.path.join(HERE, "libmrlbm.so")
 ARCH = os.environ.get("MRLBM_OFFLOAD_ARCH", "gfx950")
 
 # per-source extra flags: the FP64 kernels and the output arrays are compiled without FMA
-# contraction
+# contraction; the FP32-arithmetic kernels divide and take square roots with the hardware
+# reciprocal and square-root approximations (about 1 ulp, like the bulk path's reciprocal of rho)
+# instead of the correctly rounded sequences, which dominated the boundary-node closed forms
 SOURCES = {
     "geometry.cpp": [],
     "kernels_f64.hip": ["-ffp-contract=off"],
-    "kernels_f32.hip": [],
+    "kernels_f32.hip": ["-fno-hip-fp32-correctly-rounded-divide-sqrt"],
     "capi.hip": [],
     "batch.hip": [],
     "output.hip": ["-ffp-contract=off"],
