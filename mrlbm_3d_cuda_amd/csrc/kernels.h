@@ -222,11 +222,11 @@ inline bool fused_tile_supported(int bx, int by) {
 // Bytes of dynamic LDS one fused block uses over the tile and its ring: three planes of the
 // c_z=+1 population group, two of the c_z=0 group and one of the c_z=-1 group (ring march,
 // variant 1), or one / two planes of all 27 pulled populations (register-window march,
-// variants 3 / 4); then the moments of the tile's west ring column for two planes.
+// variants 3 / 4); then the moments of the tile's west ring column for three planes.
 inline size_t fused_lds_bytes(int bx, int by, size_t real_bytes, int variant) {
     const size_t planes = variant == 3 ? 27 : 54;
-    // + the west ring column's moments of two planes (boundary tiles)
-    return (planes * size_t(bx + 2) * size_t(by + 2) + 20 * size_t(by)) * real_bytes;
+    // + the west ring column's moments of three planes (boundary tiles)
+    return (planes * size_t(bx + 2) * size_t(by + 2) + 30 * size_t(by)) * real_bytes;
 }
 
 }  // namespace mrlbm

@@ -57,6 +57,12 @@ __device__ __forceinline__ void file_record(const RecordArgs &r) {
     }
 }
 
+// LDS slot of plane zp (zp >= -1) among the three planes of west-ring-column moments. Plane z is
+// written in the iteration before the one that reads it; with one barrier per plane (two-plane
+// window march) a fast wave may already write plane z+2 while a slow one still reads plane z, so
+// the two must not share a slot.
+__device__ __forceinline__ int west_slot(int zp) { return (zp + 3) % 3; }
+
 __device__ __forceinline__ int wrap(int v, int n) {
     v = v < 0 ? v + n : v;
     return v >= n ? v - n : v;
@@ -198,7 +204,8 @@ __device__ __forceinline__ void boundary_node(const FusedArgs<R, S> &a, R *f, co
 // values of its own node for the plane being updated and the next one, the node codes of those
 // planes, the table values of the plane being updated, and the previous plane's results (stored
 // at the start of the next iteration, like the bulk path's). The moments of the west ring column
-// go through LDS (wold) so that a node in the tile's first column can read its x-1 neighbour.
+// go through LDS (wold, three planes) so that a node in the tile's first column can read its x-1
+// neighbour.
 template <class R, class S, int BX, int BY>
 struct BoundaryPath {
     static constexpr int W = BX + 2;
@@ -272,7 +279,7 @@ __device__ __forceinline__ void fused_march(const FusedArgs<R, S> &a, R *lds, in
     using F = Frame<BX, BY>;
     constexpr int W = F::W;
     constexpr int NH = F::NH;
-    // group planes: zp[3][9][NH], z0[2][9][NH], zm[9][NH]; then the west-column moments of two planes
+    // group planes: zp[3][9][NH], z0[2][9][NH], zm[9][NH]; then the west-column moments of three planes
     R *const gp = lds;
     R *const g0 = lds + 27 * NH;
     R *const gm = lds + 45 * NH;
@@ -306,7 +313,7 @@ __device__ __forceinline__ void fused_march(const FusedArgs<R, S> &a, R *lds, in
         decode_node<R, S>(raw, a.scale, a.rho0, m);
         if (SPECIAL && west) {
 #pragma unroll
-            for (int k = 0; k < 10; ++k) wold[((zp & 1) * 10 + k) * BY + (hy1 - 1)] = m[k];
+            for (int k = 0; k < 10; ++k) wold[(west_slot(zp) * 10 + k) * BY + (hy1 - 1)] = m[k];
         }
         reconstruct_lds(m, f);
 #pragma unroll
@@ -392,7 +399,7 @@ __device__ __forceinline__ void fused_march(const FusedArgs<R, S> &a, R *lds, in
         pull(f);
         const uint32_t node = off0 + uint32_t(z) * plane;
         if constexpr (SPECIAL) {
-            bp.update(a, f, tx, ty, inside, east_tile, node, wold + (z & 1) * 10 * BY);
+            bp.update(a, f, tx, ty, inside, east_tile, node, wold + west_slot(z) * 10 * BY);
         } else {
             moments_lds(f, Mp);
             collide(a.omega, Mp);
@@ -457,7 +464,7 @@ __device__ __forceinline__ void window_march(const FusedArgs<R, S> &a, R *lds, i
         decode_node<R, S>(raw, a.scale, a.rho0, m);
         if (SPECIAL && west) {
 #pragma unroll
-            for (int k = 0; k < 10; ++k) wold[((zp & 1) * 10 + k) * BY + (hy1 - 1)] = m[k];
+            for (int k = 0; k < 10; ++k) wold[(west_slot(zp) * 10 + k) * BY + (hy1 - 1)] = m[k];
         }
         reconstruct_lds(m, f);
     };
@@ -544,7 +551,7 @@ __device__ __forceinline__ void window_march(const FusedArgs<R, S> &a, R *lds, i
         for (int q = 0; q < Q; ++q) f[q] = G[q * NH + h - LATTICE.cx[q] - LATTICE.cy[q] * W];
         const uint32_t node = off0 + uint32_t(z) * plane;
         if constexpr (SPECIAL) {
-            bp.update(a, f, tx, ty, inside, east_tile, node, wold + (z & 1) * 10 * BY);
+            bp.update(a, f, tx, ty, inside, east_tile, node, wold + west_slot(z) * 10 * BY);
             bp.copy_own(bp.own_nx, bp.own_cur);
             bp.info_cur = bp.info_nx;
         } else {

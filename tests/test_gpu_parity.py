@@ -229,16 +229,21 @@ def run_variant(desc, variant, steps):
     return out
 
 
-@pytest.mark.parametrize("z_ends", ["periodic", "walls"])
-def test_march_variants_fp64(z_ends):
+@pytest.mark.parametrize("case", ["periodic", "walls", "outlet_in_first_column"])
+def test_march_variants_fp64(case):
     """Every march variant (ring; register windows with one or two LDS planes) matches the FP64
     oracle, with boundary tiles, two-plane z chunks and a one-plane last chunk (the prefetch past
-    a chunk's last halo plane is clamped to it)."""
+    a chunk's last halo plane is clamped to it). With nx = 97 the outlet column is the first and
+    only column of the last x tile, so its x-1 neighbour comes through the west-column LDS planes
+    of every march."""
     from oracle.binding import Oracle
 
     d = small_channel(_abi.FP64, nz=7)
-    if z_ends == "walls":
+    if case == "walls":
         d = closed_z(d)
+    elif case == "outlet_in_first_column":
+        d = configs.cylinder(nx=97, ny=48, nz=7, diameter=10.0, xc=24.0, yc=24.0, u_max=0.05, re=50.0,
+                             precision=_abi.FP64)
     steps = 30
     o = Oracle(d).init()
     o.step(steps, _abi.STEP_FORCES)
