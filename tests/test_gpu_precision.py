@@ -1,4 +1,4 @@
-"""Reduced-precision storage (FP32 arithmetic, BF16/FP16 moments as scaled deviations) and the
+"""Reduced-precision storage (FP32 arithmetic, BF16/FP16 moments as deviations) and the
 checkpoint round trip, on the GPU.
 
 Every non-FP64 form reports its velocity L2 error and drag error against the FP64 run of the
