@@ -15,7 +15,8 @@
 // wall_step: curved-wall closure of the wall nodes (apply_bc_cylinder / apply_bc_annulus,
 // cases/cylinder/cylinder_kernels.cu:68-128, cases/annulus/annulus_kernels.cu:75-139) with
 // deferred collision, and the momentum-exchange force / mass-flux reduction
-// (statistics/forces.cu:25-133) in FP64.
+// (statistics/forces.cu:25-133) in FP64, as per-block partials that the next launch on the stream
+// (one block of the next fused pass, or record_kernel) sums into the per-step record.
 #pragma once
 
 #include <algorithm>
