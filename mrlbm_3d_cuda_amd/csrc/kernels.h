@@ -94,7 +94,7 @@ struct WallArgs {
 // Arguments of the wall-pressure statistics pass (one thread per inner-wall node).
 template <class R, class S>
 struct PressureArgs {
-    const S *__restrict__ rho;        // stored density field of the state after the step
+    FieldSet<const S> mom;            // state after the step (field 0: stored density)
     const uint32_t *__restrict__ node;  // inner-wall nodes
     const R *__restrict__ wnx;
     const R *__restrict__ wny;
@@ -109,8 +109,7 @@ struct PressureArgs {
 // step's index.
 template <class R, class S>
 struct PlaneArgs {
-    const S *__restrict__ rho;  // stored density deviation field
-    const S *__restrict__ ux;   // stored u_x field
+    FieldSet<const S> mom;  // state after the step (fields 0 and 1: density deviation, u_x)
     int nx, ny, nz;
     double rho0;
     StoreScale<R> scale;

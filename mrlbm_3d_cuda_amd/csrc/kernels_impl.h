@@ -851,7 +851,7 @@ __global__ __launch_bounds__(256) void pressure_kernel(PressureArgs<R, S> a) {
         const uint32_t c[4] = {b, b + 1, b + uint32_t(a.nx), b + uint32_t(a.nx) + 1};
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-            q[j] = double(Conv<S>::to(ld_field(a.rho, c[j])));
+            q[j] = double(Conv<S>::to(ld_field(a.mom, 0, c[j])));
             if constexpr (!Conv<S>::unit_scale) q[j] *= double(a.scale.dec[0]);
         }
         const double q0 = (1.0 - xd) * q[0] + xd * q[1];
@@ -886,14 +886,14 @@ __global__ void encode_kernel(const double *__restrict__ in, FieldSet<S> out, ui
     for (int f = 0; f < 10; ++f) v[f] = in[size_t(f) * n + i];
     if constexpr (!Conv<S>::scaled) {
 #pragma unroll
-        for (int f = 0; f < 10; ++f) st_field(out.f[f], i, Conv<S>::from(v[f]));
+        for (int f = 0; f < 10; ++f) st_field(out, f, i, Conv<S>::from(v[f]));
     } else {
         R M[10];
         M[0] = R(1) + R(v[0]);
 #pragma unroll
         for (int f = 1; f < 10; ++f) M[f] = R(v[f]);
         store_node<R, S>(out, i, sc, R(1), M);
-        st_field(out.f[0], i, Conv<S>::from(R(v[0]) * sc.enc[0]));
+        st_field(out, 0, i, Conv<S>::from(R(v[0]) * sc.enc[0]));
     }
 }
 
@@ -903,11 +903,11 @@ __global__ void decode_kernel(FieldSet<const S> in, double *__restrict__ out, ui
     if (i >= n) return;
     if constexpr (!Conv<S>::scaled) {
 #pragma unroll
-        for (int f = 0; f < 10; ++f) out[size_t(f) * n + i] = double(Conv<S>::to(ld_field(in.f[f], i)));
+        for (int f = 0; f < 10; ++f) out[size_t(f) * n + i] = double(Conv<S>::to(ld_field(in, f, i)));
     } else {
         R M[10];
         load_node<R, S>(in, i, sc, R(0), M);
-        out[i] = double(R(Conv<S>::to(ld_field(in.f[0], i))) * sc.dec[0]);
+        out[i] = double(R(Conv<S>::to(ld_field(in, 0, i))) * sc.dec[0]);
 #pragma unroll
         for (int f = 1; f < 10; ++f) out[size_t(f) * n + i] = double(M[f]);
     }
@@ -1040,8 +1040,8 @@ __global__ __launch_bounds__(256) void plane_kernel(PlaneArgs<R, S> a) {
     for (int i = t; i < n; i += 256) {
         const int y = i % a.ny, z = i / a.ny;
         const uint32_t row = uint32_t(a.nx) * (uint32_t(y) + uint32_t(a.ny) * uint32_t(z));
-        double r = double(Conv<S>::to(ld_field(a.rho, row)));
-        double u = double(Conv<S>::to(ld_field(a.ux, row + uint32_t(a.nx - 2))));
+        double r = double(Conv<S>::to(ld_field(a.mom, 0, row)));
+        double u = double(Conv<S>::to(ld_field(a.mom, 1, row + uint32_t(a.nx - 2))));
         if constexpr (Conv<S>::scaled) {
             r *= double(a.scale.dec[0]);
             u *= double(a.scale.dec[1]);

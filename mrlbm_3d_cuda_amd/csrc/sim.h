@@ -90,11 +90,12 @@ class Sim final : public SimBase {
         const int64_t n = g.n;
         // at least one padding slot after the last node: bulk threads outside the grid store there
         stride_ = size_t((n + 1 + 63) / 64 * 64);
-        // kernels address one field with 32-bit byte offsets
-        if (stride_ * sizeof(S) > size_t(0xFFFFFFFFu))
-            return set_error(MRLBM_ERR_UNSUPPORTED, "grid too large: one field must stay below 4 GiB");
-        HIP_TRY(hipMalloc(&buf_[0], 10 * stride_ * sizeof(S)));
-        HIP_TRY(hipMalloc(&buf_[1], 10 * stride_ * sizeof(S)));
+        // node indices are 32-bit
+        if (uint64_t(n) + 1 > uint64_t(0xFFFFFFFFu))
+            return set_error(MRLBM_ERR_UNSUPPORTED, "grid too large: node indices must fit 32 bits");
+        elems_ = field_buffer_elems<S>(n);
+        HIP_TRY(hipMalloc(&buf_[0], elems_ * sizeof(S)));
+        HIP_TRY(hipMalloc(&buf_[1], elems_ * sizeof(S)));
         HIP_TRY(hipMalloc(&donor_, 10 * stride_ * sizeof(S)));
         HIP_TRY(hipMemsetAsync(donor_, 0, 10 * stride_ * sizeof(S), stream_));
         omega_ = R(1.0 / d.tau);
@@ -412,9 +413,7 @@ class Sim final : public SimBase {
 
     PlaneArgs<R, S> plane_args(int cur) const {
         PlaneArgs<R, S> p{};
-        const FieldSet<S> fs = make_fields(buf_[cur ^ 1], stride_);
-        p.rho = fs.f[0];
-        p.ux = fs.f[1];
+        p.mom = as_const(make_fields(buf_[cur ^ 1]));
         p.nx = g.nx;
         p.ny = g.ny;
         p.nz = g.nz;
@@ -444,7 +443,7 @@ class Sim final : public SimBase {
     // Arguments of the wall-pressure pass on the state after a step from parity `cur`.
     PressureArgs<R, S> pressure_args(int cur) const {
         PressureArgs<R, S> p{};
-        p.rho = buf_[cur ^ 1];
+        p.mom = as_const(make_fields(buf_[cur ^ 1]));
         p.node = d_wnode_;
         p.wnx = d_wnx_;
         p.wny = d_wny_;
@@ -484,7 +483,7 @@ class Sim final : public SimBase {
             a.out = tmp;
             HIP_TRY(hipStreamSynchronize(stream_));
             HIP_TRY(hipMemcpy(tmp + 4 * n, bits.data(), words * sizeof(uint32_t), hipMemcpyHostToDevice));
-            launch_vti_points<R, S>(as_const(make_fields(buf_[cur_], stride_)), scale_, R(d.rho0), a, stream_);
+            launch_vti_points<R, S>(as_const(make_fields(buf_[cur_])), scale_, R(d.rho0), a, stream_);
             HIP_TRY(hipGetLastError());
             HIP_TRY(hipStreamSynchronize(stream_));
             HIP_TRY(hipMemcpy(host, tmp, 4 * n * sizeof(float), hipMemcpyDeviceToHost));
@@ -504,8 +503,8 @@ class Sim final : public SimBase {
     // interior / wall-side items of the two-stream schedule.
     FusedArgs<R, S> fused_args(int cur, int part = PART_ALL) const {
         FusedArgs<R, S> a{};
-        a.src = as_const(make_fields(buf_[cur], stride_));
-        a.dst = make_fields(buf_[cur ^ 1], stride_);
+        a.src = as_const(make_fields(buf_[cur]));
+        a.dst = make_fields(buf_[cur ^ 1]);
         a.donor = donor_;
         a.pad = uint32_t(g.n);
         a.nx = g.nx;
@@ -535,7 +534,7 @@ class Sim final : public SimBase {
     // Kernel arguments of the wall pass on the buffer the fused pass from parity `cur` wrote.
     WallArgs<R, S> wall_args(int cur, bool forces) const {
         WallArgs<R, S> w{};
-        w.mom = make_fields(buf_[cur ^ 1], stride_);
+        w.mom = make_fields(buf_[cur ^ 1]);
         w.donor = donor_;
         w.nwall = nwall_;
         w.nb_inner = nb_inner_;
@@ -661,19 +660,30 @@ class Sim final : public SimBase {
         return MRLBM_OK;
     }
 
+    // The raw payload is field-major (10 x n values as stored); the device buffers are
+    // node-chunked, so the copy goes through a host image of the buffer.
     int get_raw(void *h) override {
         if (int rc = wait_batch()) return rc;
         HIP_TRY(hipStreamSynchronize(stream_));
-        HIP_TRY(hipMemcpy2D(h, size_t(g.n) * sizeof(S), buf_[cur_], stride_ * sizeof(S), size_t(g.n) * sizeof(S), 10,
-                            hipMemcpyDeviceToHost));
+        std::vector<S> img(elems_);
+        HIP_TRY(hipMemcpy(img.data(), buf_[cur_], elems_ * sizeof(S), hipMemcpyDeviceToHost));
+        S *out = static_cast<S *>(h);
+        for (int f = 0; f < 10; ++f)
+            for (int64_t i = 0; i < g.n; ++i)
+                out[size_t(f) * g.n + i] = img[chunk_elem<S>(uint32_t(i)) + size_t(f) * kChunk<S>];
         return MRLBM_OK;
     }
 
     int set_raw(const void *h) override {
         HIP_TRY(hipStreamSynchronize(stream_));
+        std::vector<S> img(elems_);
+        HIP_TRY(hipMemcpy(img.data(), buf_[cur_], elems_ * sizeof(S), hipMemcpyDeviceToHost));  // keeps the padding
+        const S *in = static_cast<const S *>(h);
+        for (int f = 0; f < 10; ++f)
+            for (int64_t i = 0; i < g.n; ++i)
+                img[chunk_elem<S>(uint32_t(i)) + size_t(f) * kChunk<S>] = in[size_t(f) * g.n + i];
         for (int b = 0; b < 2; ++b)  // both buffers (frozen nodes)
-            HIP_TRY(hipMemcpy2D(buf_[b], stride_ * sizeof(S), h, size_t(g.n) * sizeof(S), size_t(g.n) * sizeof(S), 10,
-                                hipMemcpyHostToDevice));
+            HIP_TRY(hipMemcpy(buf_[b], img.data(), elems_ * sizeof(S), hipMemcpyHostToDevice));
         return MRLBM_OK;
     }
 
@@ -852,8 +862,8 @@ class Sim final : public SimBase {
         HIP_TRY(hipMalloc(&tmp, h.size() * sizeof(double)));
         HIP_TRY(hipMemcpy(tmp, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
         // both buffers: frozen nodes are never rewritten and must read the same from either
-        launch_encode<R, S>(tmp, make_fields(buf_[cur_], stride_), uint32_t(g.n), scale_, stream_);
-        launch_encode<R, S>(tmp, make_fields(buf_[cur_ ^ 1], stride_), uint32_t(g.n), scale_, stream_);
+        launch_encode<R, S>(tmp, make_fields(buf_[cur_]), uint32_t(g.n), scale_, stream_);
+        launch_encode<R, S>(tmp, make_fields(buf_[cur_ ^ 1]), uint32_t(g.n), scale_, stream_);
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipStreamSynchronize(stream_));
         HIP_TRY(hipFree(tmp));
@@ -865,7 +875,7 @@ class Sim final : public SimBase {
         h.resize(size_t(10) * g.n);
         double *tmp = nullptr;
         HIP_TRY(hipMalloc(&tmp, h.size() * sizeof(double)));
-        launch_decode<R, S>(as_const(make_fields(buf_[cur_], stride_)), tmp, uint32_t(g.n), scale_, stream_);
+        launch_decode<R, S>(as_const(make_fields(buf_[cur_])), tmp, uint32_t(g.n), scale_, stream_);
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipStreamSynchronize(stream_));
         HIP_TRY(hipMemcpy(h.data(), tmp, h.size() * sizeof(double), hipMemcpyDeviceToHost));
@@ -901,7 +911,8 @@ class Sim final : public SimBase {
     int two_stream_env_ = -1;  // MRLBM_TWO_STREAM: 1 on, 0 off, unset (-1) the precision's default
     S *buf_[2] = {nullptr, nullptr};
     S *donor_ = nullptr;
-    size_t stride_ = 0;
+    size_t stride_ = 0;  // donor snapshot: node slots (n + 1, rounded up)
+    size_t elems_ = 0;   // elements of one node-chunked moment buffer
     int cur_ = 0;
     R omega_ = 0, rho0_ = 1;
     StoreScale<R> scale_{};

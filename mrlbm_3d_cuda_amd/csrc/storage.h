@@ -89,44 +89,58 @@ __device__ __forceinline__ uint32_t sr_bits(uint32_t h, int f) { return (h >> ((
 // Field scale class: 0 for rho, 1 for u, 2 for the second moments.
 __device__ __host__ constexpr int field_class(int f) { return f == 0 ? 0 : (f < 4 ? 1 : 2); }
 
-// The 10 field arrays of one moment buffer, one base pointer each. Kernels receive the pointers
-// as arguments, so every load and store is a uniform 64-bit base plus a 32-bit per-lane byte
-// offset (the library limits one field to 4 GiB) and needs no per-lane 64-bit address math.
+// One moment buffer, node-chunked: the nodes are cut into chunks of kChunk<S> consecutive indices
+// (32, or 16 for FP64: one tile row of the default tiles), and chunk c holds field f of its
+// nodes at base[c * 10 kChunk + f kChunk ...]. A chunk's 10 fields are one contiguous run (1280
+// bytes in FP32), so the 10 loads (stores) of a tile row fall into consecutive lines instead of
+// 10 separate arrays. The fields are immediate offsets from one address per node.
 template <class S>
-struct FieldSet {
-    S *f[10];
-};
+constexpr uint32_t kChunk = sizeof(S) == 8 ? 16u : 32u;
 
 template <class S>
-__host__ __device__ inline FieldSet<S> make_fields(S *base, size_t stride) {
-    FieldSet<S> fs;
-    for (int k = 0; k < 10; ++k) fs.f[k] = base + size_t(k) * stride;
-    return fs;
+struct FieldSet {
+    S *base;
+};
+
+// Element offset of field 0 of node idx.
+template <class S>
+__host__ __device__ inline size_t chunk_elem(uint32_t idx) {
+    return size_t(idx / kChunk<S>) * (10 * kChunk<S>) + (idx % kChunk<S>);
+}
+
+// Elements of a buffer of n nodes plus the padding slot at index n.
+template <class S>
+inline size_t field_buffer_elems(int64_t n) {
+    return size_t((n + 1 + kChunk<S> - 1) / kChunk<S>) * (10 * kChunk<S>);
+}
+
+template <class S>
+__host__ __device__ inline FieldSet<S> make_fields(S *base) {
+    return FieldSet<S>{base};
 }
 
 template <class S>
 __host__ __device__ inline FieldSet<const S> as_const(const FieldSet<S> &fs) {
-    FieldSet<const S> c;
-    for (int k = 0; k < 10; ++k) c.f[k] = fs.f[k];
-    return c;
+    return FieldSet<const S>{fs.base};
 }
 
 template <class S>
-__device__ __forceinline__ S ld_field(const S *__restrict__ field, uint32_t idx) {
-    return *reinterpret_cast<const S *>(reinterpret_cast<const char *>(field) + idx * uint32_t(sizeof(S)));
+__device__ __forceinline__ S ld_field(const FieldSet<const S> &fs, int f, uint32_t idx) {
+    return fs.base[chunk_elem<S>(idx) + size_t(f) * kChunk<S>];
 }
 
 template <class S>
-__device__ __forceinline__ void st_field(S *__restrict__ field, uint32_t idx, S v) {
-    *reinterpret_cast<S *>(reinterpret_cast<char *>(field) + idx * uint32_t(sizeof(S))) = v;
+__device__ __forceinline__ void st_field(const FieldSet<S> &fs, int f, uint32_t idx, S v) {
+    fs.base[chunk_elem<S>(idx) + size_t(f) * kChunk<S>] = v;
 }
 
 // The 10 stored values of node `idx`, untouched: decoding is left to the consumer so that a
 // prefetch does not wait for its data.
 template <class S>
 __device__ __forceinline__ void load_raw(const FieldSet<const S> &fs, uint32_t idx, S *raw) {
+    const S *__restrict__ p = fs.base + chunk_elem<S>(idx);
 #pragma unroll
-    for (int f = 0; f < 10; ++f) raw[f] = ld_field(fs.f[f], idx);
+    for (int f = 0; f < 10; ++f) raw[f] = p[f * kChunk<S>];
 }
 
 // Moments from stored values (M[0] absolute density).
@@ -203,8 +217,9 @@ __device__ __forceinline__ void store_node(const FieldSet<S> &fs, uint32_t idx, 
                                            const R *M, uint32_t seed = 0) {
     S v[10];
     encode_node<R, S>(idx, sc, rho0, M, seed, v);
+    S *__restrict__ p = fs.base + chunk_elem<S>(idx);
 #pragma unroll
-    for (int f = 0; f < 10; ++f) st_field(fs.f[f], idx, v[f]);
+    for (int f = 0; f < 10; ++f) p[f * kChunk<S>] = v[f];
 }
 
 // Node-major buffers (the 10 values of node i at base[10 i .. 10 i + 9]): the donor snapshot,
