@@ -670,7 +670,7 @@ class Sim final : public SimBase {
         S *out = static_cast<S *>(h);
         for (int f = 0; f < 10; ++f)
             for (int64_t i = 0; i < g.n; ++i)
-                out[size_t(f) * g.n + i] = img[chunk_elem<S>(uint32_t(i)) + size_t(f) * kChunk<S>];
+                out[size_t(f) * g.n + i] = img[field_elem<S>(uint32_t(i), f)];
         return MRLBM_OK;
     }
 
@@ -681,7 +681,7 @@ class Sim final : public SimBase {
         const S *in = static_cast<const S *>(h);
         for (int f = 0; f < 10; ++f)
             for (int64_t i = 0; i < g.n; ++i)
-                img[chunk_elem<S>(uint32_t(i)) + size_t(f) * kChunk<S>] = in[size_t(f) * g.n + i];
+                img[field_elem<S>(uint32_t(i), f)] = in[size_t(f) * g.n + i];
         for (int b = 0; b < 2; ++b)  // both buffers (frozen nodes)
             HIP_TRY(hipMemcpy(buf_[b], img.data(), elems_ * sizeof(S), hipMemcpyHostToDevice));
         return MRLBM_OK;

@@ -94,18 +94,57 @@ __device__ __host__ constexpr int field_class(int f) { return f == 0 ? 0 : (f < 
 // nodes at base[c * 10 kChunk + f kChunk ...]. A chunk's 10 fields are one contiguous run (1280
 // bytes in FP32), so the 10 loads (stores) of a tile row fall into consecutive lines instead of
 // 10 separate arrays. The fields are immediate offsets from one address per node.
+// The 16-bit forms pair the fields: fields 2j and 2j+1 of a node share one 32-bit word (2j in the
+// low half), and chunk c holds word j of its nodes at word c * 5 kChunk + j kChunk + lane. A node
+// then moves in 5 word loads (stores) instead of 10 half-word ones, and each wave-wide access
+// covers 128-byte row segments, as in FP32.
 template <class S>
 constexpr uint32_t kChunk = sizeof(S) == 8 ? 16u : 32u;
+
+template <class S>
+constexpr bool kPaired = sizeof(S) == 2;
+
+template <class S>
+__device__ __host__ inline uint16_t bits16(S v) {
+    return __builtin_bit_cast(uint16_t, v);
+}
+
+template <class S>
+__device__ __host__ inline S from_bits16(uint32_t b) {
+    return __builtin_bit_cast(S, static_cast<uint16_t>(b));
+}
+
+__device__ __host__ inline uint32_t pack16(uint16_t lo, uint16_t hi) { return uint32_t(lo) | (uint32_t(hi) << 16); }
 
 template <class S>
 struct FieldSet {
     S *base;
 };
 
-// Element offset of field 0 of node idx.
+// Element offset of the chunk that holds node idx.
 template <class S>
-__host__ __device__ inline size_t chunk_elem(uint32_t idx) {
-    return size_t(idx / kChunk<S>) * (10 * kChunk<S>) + (idx % kChunk<S>);
+__host__ __device__ inline size_t chunk_base(uint32_t idx) {
+    return size_t(idx / kChunk<S>) * (10 * kChunk<S>);
+}
+
+// Element offset of field f of node idx.
+template <class S>
+__host__ __device__ inline size_t field_elem(uint32_t idx, int f) {
+    const uint32_t l = idx % kChunk<S>;
+    if constexpr (kPaired<S>)
+        return chunk_base<S>(idx) + size_t(f >> 1) * (2 * kChunk<S>) + 2 * l + uint32_t(f & 1);
+    else
+        return chunk_base<S>(idx) + size_t(f) * kChunk<S> + l;
+}
+
+// Word pointer of word 0 of node idx (16-bit forms); word j is at [j kChunk].
+template <class S>
+__host__ __device__ inline uint32_t *word_ptr(S *base, uint32_t idx) {
+    return reinterpret_cast<uint32_t *>(base + chunk_base<S>(idx)) + idx % kChunk<S>;
+}
+template <class S>
+__host__ __device__ inline const uint32_t *word_ptr(const S *base, uint32_t idx) {
+    return reinterpret_cast<const uint32_t *>(base + chunk_base<S>(idx)) + idx % kChunk<S>;
 }
 
 // Elements of a buffer of n nodes plus the padding slot at index n.
@@ -126,21 +165,31 @@ __host__ __device__ inline FieldSet<const S> as_const(const FieldSet<S> &fs) {
 
 template <class S>
 __device__ __forceinline__ S ld_field(const FieldSet<const S> &fs, int f, uint32_t idx) {
-    return fs.base[chunk_elem<S>(idx) + size_t(f) * kChunk<S>];
+    return fs.base[field_elem<S>(idx, f)];
 }
 
 template <class S>
 __device__ __forceinline__ void st_field(const FieldSet<S> &fs, int f, uint32_t idx, S v) {
-    fs.base[chunk_elem<S>(idx) + size_t(f) * kChunk<S>] = v;
+    fs.base[field_elem<S>(idx, f)] = v;
 }
 
 // The 10 stored values of node `idx`, untouched: decoding is left to the consumer so that a
 // prefetch does not wait for its data.
 template <class S>
 __device__ __forceinline__ void load_raw(const FieldSet<const S> &fs, uint32_t idx, S *raw) {
-    const S *__restrict__ p = fs.base + chunk_elem<S>(idx);
+    if constexpr (kPaired<S>) {
+        const uint32_t *__restrict__ p = word_ptr(fs.base, idx);
 #pragma unroll
-    for (int f = 0; f < 10; ++f) raw[f] = p[f * kChunk<S>];
+        for (int j = 0; j < 5; ++j) {
+            const uint32_t w = p[j * kChunk<S>];
+            raw[2 * j] = from_bits16<S>(w);
+            raw[2 * j + 1] = from_bits16<S>(w >> 16);
+        }
+    } else {
+        const S *__restrict__ p = fs.base + field_elem<S>(idx, 0);
+#pragma unroll
+        for (int f = 0; f < 10; ++f) raw[f] = p[f * kChunk<S>];
+    }
 }
 
 // Moments from stored values (M[0] absolute density).
@@ -217,25 +266,48 @@ __device__ __forceinline__ void store_node(const FieldSet<S> &fs, uint32_t idx, 
                                            const R *M, uint32_t seed = 0) {
     S v[10];
     encode_node<R, S>(idx, sc, rho0, M, seed, v);
-    S *__restrict__ p = fs.base + chunk_elem<S>(idx);
+    if constexpr (kPaired<S>) {
+        uint32_t *__restrict__ p = word_ptr(fs.base, idx);
 #pragma unroll
-    for (int f = 0; f < 10; ++f) p[f * kChunk<S>] = v[f];
+        for (int j = 0; j < 5; ++j) p[j * kChunk<S>] = pack16(bits16(v[2 * j]), bits16(v[2 * j + 1]));
+    } else {
+        S *__restrict__ p = fs.base + field_elem<S>(idx, 0);
+#pragma unroll
+        for (int f = 0; f < 10; ++f) p[f * kChunk<S>] = v[f];
+    }
 }
 
 // Node-major buffers (the 10 values of node i at base[10 i .. 10 i + 9]): the donor snapshot,
-// read at scattered stencil corners, where one node's values then share a cache line.
+// read at scattered stencil corners, where one node's values then share a cache line. A node's
+// 16-bit values start on a 4-byte boundary (20 i bytes) and move as 5 words.
 template <class S>
 __device__ __forceinline__ void store_raw_nm(S *__restrict__ base, uint32_t idx, const S *v) {
+    if constexpr (kPaired<S>) {
+        uint32_t *__restrict__ p = reinterpret_cast<uint32_t *>(base + size_t(idx) * 10);
 #pragma unroll
-    for (int f = 0; f < 10; ++f) base[size_t(idx) * 10 + f] = v[f];
+        for (int j = 0; j < 5; ++j) p[j] = pack16(bits16(v[2 * j]), bits16(v[2 * j + 1]));
+    } else {
+#pragma unroll
+        for (int f = 0; f < 10; ++f) base[size_t(idx) * 10 + f] = v[f];
+    }
 }
 
 template <class R, class S>
 __device__ __forceinline__ void load_node_nm(const S *__restrict__ base, uint32_t idx, const StoreScale<R> &sc, R rho0,
                                              R *M) {
     S raw[10];
+    if constexpr (kPaired<S>) {
+        const uint32_t *__restrict__ p = reinterpret_cast<const uint32_t *>(base + size_t(idx) * 10);
 #pragma unroll
-    for (int f = 0; f < 10; ++f) raw[f] = base[size_t(idx) * 10 + f];
+        for (int j = 0; j < 5; ++j) {
+            const uint32_t w = p[j];
+            raw[2 * j] = from_bits16<S>(w);
+            raw[2 * j + 1] = from_bits16<S>(w >> 16);
+        }
+    } else {
+#pragma unroll
+        for (int f = 0; f < 10; ++f) raw[f] = base[size_t(idx) * 10 + f];
+    }
     decode_node<R, S>(raw, sc, rho0, M);
 }
 
