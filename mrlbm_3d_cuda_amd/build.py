@@ -4,6 +4,7 @@
 
 Compiles every source under csrc/ with hipcc into one shared library next to this file.
 """
+import concurrent.futures
 import os
 import shutil
 import subprocess
@@ -52,14 +53,22 @@ def build(verbose=False, extra_flags=()):
     objs = []
     obj_dir = os.path.join(HERE, "_build")
     os.makedirs(obj_dir, exist_ok=True)
+    cmds = []
     for src, flags in SOURCES.items():
         obj = os.path.join(obj_dir, os.path.splitext(src)[0] + ".o")
         lang = ["-x", "hip"] if src.endswith(".cpp") else []
-        cmd = [hipcc(), *HIPCC_FLAGS, *flags, *extra_flags, *lang, "-c", os.path.join(CSRC, src), "-o", obj]
-        if verbose:
-            print(" ".join(cmd), flush=True)
-        subprocess.run(cmd, check=True)
+        cmds.append([hipcc(), *HIPCC_FLAGS, *flags, *extra_flags, *lang, "-c", os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)
+    # the sources compile independently: run the compilers side by side (a few at a time)
+    jobs = max(1, min(len(cmds), int(os.environ.get("MAX_JOBS", "4")), os.cpu_count() or 1))
+    with concurrent.futures.ThreadPoolExecutor(jobs) as pool:
+        futures = []
+        for cmd in cmds:
+            if verbose:
+                print(" ".join(cmd), flush=True)
+            futures.append(pool.submit(subprocess.run, cmd, check=True))
+        for f in futures:
+            f.result()
     tmp = LIB + ".tmp"
     cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", tmp]
     if verbose:
