@@ -90,31 +90,61 @@ __device__ __forceinline__ uint32_t sr_bits(uint32_t h, int f) { return (h >> ((
 __device__ __host__ constexpr int field_class(int f) { return f == 0 ? 0 : (f < 4 ? 1 : 2); }
 
 // One moment buffer, node-chunked: the nodes are cut into chunks of kChunk<S> consecutive indices
-// (32, or 16 for FP64: one tile row of the default tiles), and chunk c holds field f of its
-// nodes at base[c * 10 kChunk + f kChunk ...]. A chunk's 10 fields are one contiguous run (1280
-// bytes in FP32), so the 10 loads (stores) of a tile row fall into consecutive lines instead of
-// 10 separate arrays. The fields are immediate offsets from one address per node.
-// The 16-bit forms pair the fields: fields 2j and 2j+1 of a node share one 32-bit word (2j in the
-// low half), and chunk c holds word j of its nodes at word c * 5 kChunk + j kChunk + lane. A node
-// then moves in 5 word loads (stores) instead of 10 half-word ones, and each wave-wide access
-// covers 128-byte row segments, as in FP32.
+// (32, or 16 for FP64: one tile row of the default tiles). The fields are paired: fields 2j and
+// 2j+1 of a node sit side by side (2j first) in one pair word of two values, and chunk c holds
+// pair word j of its nodes at pair c * 5 kChunk + j kChunk + lane. A chunk is one contiguous run
+// (1280 bytes in FP32 and FP64, 640 bytes at 16-bit storage), so the loads (stores) of a tile row
+// fall into consecutive lines, and a node moves in 5 pair loads (stores): 32-bit words at 16-bit
+// storage, 64-bit in FP32, 128-bit in FP64. The pairs are immediate offsets from one address
+// per node.
 template <class S>
 constexpr uint32_t kChunk = sizeof(S) == 8 ? 16u : 32u;
 
+// Pair word of two stored values: the bits of each value and the word moved as one access.
+template <size_t B>
+struct PairBits;
+template <>
+struct PairBits<2> {
+    using U = uint16_t;
+    using W = uint32_t;
+};
+template <>
+struct PairBits<4> {
+    using U = uint32_t;
+    using W = uint64_t;
+};
+template <>
+struct PairBits<8> {
+    using U = uint64_t;
+    typedef uint64_t W __attribute__((ext_vector_type(2)));
+};
 template <class S>
-constexpr bool kPaired = sizeof(S) == 2;
+using pair_t = typename PairBits<sizeof(S)>::W;
 
 template <class S>
-__device__ __host__ inline uint16_t bits16(S v) {
-    return __builtin_bit_cast(uint16_t, v);
+__device__ __host__ inline void split_pair(pair_t<S> w, S &lo, S &hi) {
+    using U = typename PairBits<sizeof(S)>::U;
+    if constexpr (sizeof(S) == 8) {
+        lo = __builtin_bit_cast(S, U(w.x));
+        hi = __builtin_bit_cast(S, U(w.y));
+    } else {
+        lo = __builtin_bit_cast(S, static_cast<U>(w));
+        hi = __builtin_bit_cast(S, static_cast<U>(w >> (8 * sizeof(S))));
+    }
 }
 
 template <class S>
-__device__ __host__ inline S from_bits16(uint32_t b) {
-    return __builtin_bit_cast(S, static_cast<uint16_t>(b));
+__device__ __host__ inline pair_t<S> join_pair(S lo, S hi) {
+    using U = typename PairBits<sizeof(S)>::U;
+    if constexpr (sizeof(S) == 8) {
+        pair_t<S> w;
+        w.x = __builtin_bit_cast(U, lo);
+        w.y = __builtin_bit_cast(U, hi);
+        return w;
+    } else {
+        return pair_t<S>(__builtin_bit_cast(U, lo)) | (pair_t<S>(__builtin_bit_cast(U, hi)) << (8 * sizeof(S)));
+    }
 }
-
-__device__ __host__ inline uint32_t pack16(uint16_t lo, uint16_t hi) { return uint32_t(lo) | (uint32_t(hi) << 16); }
 
 template <class S>
 struct FieldSet {
@@ -130,21 +160,17 @@ __host__ __device__ inline size_t chunk_base(uint32_t idx) {
 // Element offset of field f of node idx.
 template <class S>
 __host__ __device__ inline size_t field_elem(uint32_t idx, int f) {
-    const uint32_t l = idx % kChunk<S>;
-    if constexpr (kPaired<S>)
-        return chunk_base<S>(idx) + size_t(f >> 1) * (2 * kChunk<S>) + 2 * l + uint32_t(f & 1);
-    else
-        return chunk_base<S>(idx) + size_t(f) * kChunk<S> + l;
+    return chunk_base<S>(idx) + size_t(f >> 1) * (2 * kChunk<S>) + 2 * (idx % kChunk<S>) + uint32_t(f & 1);
 }
 
-// Word pointer of word 0 of node idx (16-bit forms); word j is at [j kChunk].
+// Pointer to pair word 0 of node idx; pair word j is at [j kChunk].
 template <class S>
-__host__ __device__ inline uint32_t *word_ptr(S *base, uint32_t idx) {
-    return reinterpret_cast<uint32_t *>(base + chunk_base<S>(idx)) + idx % kChunk<S>;
+__host__ __device__ inline pair_t<S> *pair_ptr(S *base, uint32_t idx) {
+    return reinterpret_cast<pair_t<S> *>(base + chunk_base<S>(idx)) + idx % kChunk<S>;
 }
 template <class S>
-__host__ __device__ inline const uint32_t *word_ptr(const S *base, uint32_t idx) {
-    return reinterpret_cast<const uint32_t *>(base + chunk_base<S>(idx)) + idx % kChunk<S>;
+__host__ __device__ inline const pair_t<S> *pair_ptr(const S *base, uint32_t idx) {
+    return reinterpret_cast<const pair_t<S> *>(base + chunk_base<S>(idx)) + idx % kChunk<S>;
 }
 
 // Elements of a buffer of n nodes plus the padding slot at index n.
@@ -177,19 +203,9 @@ __device__ __forceinline__ void st_field(const FieldSet<S> &fs, int f, uint32_t 
 // prefetch does not wait for its data.
 template <class S>
 __device__ __forceinline__ void load_raw(const FieldSet<const S> &fs, uint32_t idx, S *raw) {
-    if constexpr (kPaired<S>) {
-        const uint32_t *__restrict__ p = word_ptr(fs.base, idx);
+    const pair_t<S> *__restrict__ p = pair_ptr(fs.base, idx);
 #pragma unroll
-        for (int j = 0; j < 5; ++j) {
-            const uint32_t w = p[j * kChunk<S>];
-            raw[2 * j] = from_bits16<S>(w);
-            raw[2 * j + 1] = from_bits16<S>(w >> 16);
-        }
-    } else {
-        const S *__restrict__ p = fs.base + field_elem<S>(idx, 0);
-#pragma unroll
-        for (int f = 0; f < 10; ++f) raw[f] = p[f * kChunk<S>];
-    }
+    for (int j = 0; j < 5; ++j) split_pair<S>(p[j * kChunk<S>], raw[2 * j], raw[2 * j + 1]);
 }
 
 // Moments from stored values (M[0] absolute density).
@@ -266,48 +282,28 @@ __device__ __forceinline__ void store_node(const FieldSet<S> &fs, uint32_t idx, 
                                            const R *M, uint32_t seed = 0) {
     S v[10];
     encode_node<R, S>(idx, sc, rho0, M, seed, v);
-    if constexpr (kPaired<S>) {
-        uint32_t *__restrict__ p = word_ptr(fs.base, idx);
+    pair_t<S> *__restrict__ p = pair_ptr(fs.base, idx);
 #pragma unroll
-        for (int j = 0; j < 5; ++j) p[j * kChunk<S>] = pack16(bits16(v[2 * j]), bits16(v[2 * j + 1]));
-    } else {
-        S *__restrict__ p = fs.base + field_elem<S>(idx, 0);
-#pragma unroll
-        for (int f = 0; f < 10; ++f) p[f * kChunk<S>] = v[f];
-    }
+    for (int j = 0; j < 5; ++j) p[j * kChunk<S>] = join_pair<S>(v[2 * j], v[2 * j + 1]);
 }
 
 // Node-major buffers (the 10 values of node i at base[10 i .. 10 i + 9]): the donor snapshot,
-// read at scattered stencil corners, where one node's values then share a cache line. A node's
-// 16-bit values start on a 4-byte boundary (20 i bytes) and move as 5 words.
+// read at scattered stencil corners, where one node's values then share a cache line. A node
+// starts on a pair-word boundary (10 i values) and moves as 5 pair words.
 template <class S>
 __device__ __forceinline__ void store_raw_nm(S *__restrict__ base, uint32_t idx, const S *v) {
-    if constexpr (kPaired<S>) {
-        uint32_t *__restrict__ p = reinterpret_cast<uint32_t *>(base + size_t(idx) * 10);
+    pair_t<S> *__restrict__ p = reinterpret_cast<pair_t<S> *>(base + size_t(idx) * 10);
 #pragma unroll
-        for (int j = 0; j < 5; ++j) p[j] = pack16(bits16(v[2 * j]), bits16(v[2 * j + 1]));
-    } else {
-#pragma unroll
-        for (int f = 0; f < 10; ++f) base[size_t(idx) * 10 + f] = v[f];
-    }
+    for (int j = 0; j < 5; ++j) p[j] = join_pair<S>(v[2 * j], v[2 * j + 1]);
 }
 
 template <class R, class S>
 __device__ __forceinline__ void load_node_nm(const S *__restrict__ base, uint32_t idx, const StoreScale<R> &sc, R rho0,
                                              R *M) {
     S raw[10];
-    if constexpr (kPaired<S>) {
-        const uint32_t *__restrict__ p = reinterpret_cast<const uint32_t *>(base + size_t(idx) * 10);
+    const pair_t<S> *__restrict__ p = reinterpret_cast<const pair_t<S> *>(base + size_t(idx) * 10);
 #pragma unroll
-        for (int j = 0; j < 5; ++j) {
-            const uint32_t w = p[j];
-            raw[2 * j] = from_bits16<S>(w);
-            raw[2 * j + 1] = from_bits16<S>(w >> 16);
-        }
-    } else {
-#pragma unroll
-        for (int f = 0; f < 10; ++f) raw[f] = base[size_t(idx) * 10 + f];
-    }
+    for (int j = 0; j < 5; ++j) split_pair<S>(p[j], raw[2 * j], raw[2 * j + 1]);
     decode_node<R, S>(raw, sc, rho0, M);
 }
 
