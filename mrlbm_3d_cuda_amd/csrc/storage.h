@@ -90,13 +90,13 @@ __device__ __forceinline__ uint32_t sr_bits(uint32_t h, int f) { return (h >> ((
 __device__ __host__ constexpr int field_class(int f) { return f == 0 ? 0 : (f < 4 ? 1 : 2); }
 
 // One moment buffer, node-chunked: the nodes are cut into chunks of kChunk<S> consecutive indices
-// (32, or 16 for FP64: one tile row of the default tiles). The fields are paired: fields 2j and
-// 2j+1 of a node sit side by side (2j first) in one pair word of two values, and chunk c holds
-// pair word j of its nodes at pair c * 5 kChunk + j kChunk + lane. A chunk is one contiguous run
-// (1280 bytes in FP32 and FP64, 640 bytes at 16-bit storage), so the loads (stores) of a tile row
-// fall into consecutive lines, and a node moves in 5 pair loads (stores): 32-bit words at 16-bit
-// storage, 64-bit in FP32, 128-bit in FP64. The pairs are immediate offsets from one address
-// per node.
+// (32, or 16 for FP64: one tile row of the default tiles), and the fields of a node are grouped so
+// that a node moves in few wide accesses. FP32 and 16-bit storage group fields 0-3, 4-7 and 8-9:
+// chunk c holds group g of its nodes at element c * 10 kChunk + g * 4 kChunk + width * lane, with
+// width 4, 4, 2 (3 accesses per node: 16 + 16 + 8 bytes in FP32, 8 + 8 + 4 at 16-bit storage).
+// FP64 pairs the fields (2j, 2j+1) at element c * 10 kChunk + j * 2 kChunk + 2 lane (5 accesses of
+// 16 bytes). A chunk is one contiguous run (1280 bytes in FP32 and FP64, 640 bytes at 16-bit
+// storage), so the loads (stores) of a tile row fall into consecutive lines.
 template <class S>
 constexpr uint32_t kChunk = sizeof(S) == 8 ? 16u : 32u;
 
@@ -146,6 +146,56 @@ __device__ __host__ inline pair_t<S> join_pair(S lo, S hi) {
     }
 }
 
+// Quad word of four stored values (FP32 and 16-bit storage).
+template <size_t B>
+struct QuadBits;
+template <>
+struct QuadBits<2> {
+    using U = uint16_t;
+    using W = uint64_t;
+};
+template <>
+struct QuadBits<4> {
+    using U = uint32_t;
+    typedef uint32_t W __attribute__((ext_vector_type(4)));
+};
+template <class S>
+using quad_t = typename QuadBits<sizeof(S)>::W;
+
+template <class S>
+__device__ __host__ inline void split_quad(quad_t<S> w, S *v) {
+    using U = typename QuadBits<sizeof(S)>::U;
+    if constexpr (sizeof(S) == 4) {
+        v[0] = __builtin_bit_cast(S, U(w.x));
+        v[1] = __builtin_bit_cast(S, U(w.y));
+        v[2] = __builtin_bit_cast(S, U(w.z));
+        v[3] = __builtin_bit_cast(S, U(w.w));
+    } else {  // two 32-bit halves, two values each
+        const uint32_t lo = uint32_t(w), hi = uint32_t(w >> 32);
+        v[0] = __builtin_bit_cast(S, static_cast<U>(lo));
+        v[1] = __builtin_bit_cast(S, static_cast<U>(lo >> 16));
+        v[2] = __builtin_bit_cast(S, static_cast<U>(hi));
+        v[3] = __builtin_bit_cast(S, static_cast<U>(hi >> 16));
+    }
+}
+
+template <class S>
+__device__ __host__ inline quad_t<S> join_quad(const S *v) {
+    using U = typename QuadBits<sizeof(S)>::U;
+    if constexpr (sizeof(S) == 4) {
+        quad_t<S> w;
+        w.x = __builtin_bit_cast(U, v[0]);
+        w.y = __builtin_bit_cast(U, v[1]);
+        w.z = __builtin_bit_cast(U, v[2]);
+        w.w = __builtin_bit_cast(U, v[3]);
+        return w;
+    } else {  // two 32-bit halves, two values each
+        const uint32_t lo = uint32_t(__builtin_bit_cast(U, v[0])) | (uint32_t(__builtin_bit_cast(U, v[1])) << 16);
+        const uint32_t hi = uint32_t(__builtin_bit_cast(U, v[2])) | (uint32_t(__builtin_bit_cast(U, v[3])) << 16);
+        return quad_t<S>(lo) | (quad_t<S>(hi) << 32);
+    }
+}
+
 template <class S>
 struct FieldSet {
     S *base;
@@ -157,13 +207,23 @@ __host__ __device__ inline size_t chunk_base(uint32_t idx) {
     return size_t(idx / kChunk<S>) * (10 * kChunk<S>);
 }
 
+// Four fields per access (FP32 and 16-bit storage) or two (FP64).
+template <class S>
+constexpr bool kQuads = sizeof(S) <= 4;
+
 // Element offset of field f of node idx.
 template <class S>
 __host__ __device__ inline size_t field_elem(uint32_t idx, int f) {
-    return chunk_base<S>(idx) + size_t(f >> 1) * (2 * kChunk<S>) + 2 * (idx % kChunk<S>) + uint32_t(f & 1);
+    const uint32_t l = idx % kChunk<S>;
+    if constexpr (kQuads<S>) {
+        const uint32_t g = uint32_t(f) >> 2;
+        return chunk_base<S>(idx) + g * (4 * kChunk<S>) + (g < 2 ? 4u : 2u) * l + uint32_t(f & 3);
+    } else {
+        return chunk_base<S>(idx) + size_t(f >> 1) * (2 * kChunk<S>) + 2 * l + uint32_t(f & 1);
+    }
 }
 
-// Pointer to pair word 0 of node idx; pair word j is at [j kChunk].
+// Pointer to pair word 0 of node idx (paired layout); pair word j is at [j kChunk].
 template <class S>
 __host__ __device__ inline pair_t<S> *pair_ptr(S *base, uint32_t idx) {
     return reinterpret_cast<pair_t<S> *>(base + chunk_base<S>(idx)) + idx % kChunk<S>;
@@ -203,9 +263,19 @@ __device__ __forceinline__ void st_field(const FieldSet<S> &fs, int f, uint32_t 
 // prefetch does not wait for its data.
 template <class S>
 __device__ __forceinline__ void load_raw(const FieldSet<const S> &fs, uint32_t idx, S *raw) {
-    const pair_t<S> *__restrict__ p = pair_ptr(fs.base, idx);
+    if constexpr (kQuads<S>) {
+        const S *cb = fs.base + chunk_base<S>(idx);
+        const uint32_t l = idx % kChunk<S>;
+        const quad_t<S> *__restrict__ q = reinterpret_cast<const quad_t<S> *>(cb) + l;
+        const pair_t<S> *__restrict__ p = reinterpret_cast<const pair_t<S> *>(cb + 8 * kChunk<S>) + l;
+        split_quad<S>(q[0], raw);
+        split_quad<S>(q[kChunk<S>], raw + 4);
+        split_pair<S>(p[0], raw[8], raw[9]);
+    } else {
+        const pair_t<S> *__restrict__ p = pair_ptr(fs.base, idx);
 #pragma unroll
-    for (int j = 0; j < 5; ++j) split_pair<S>(p[j * kChunk<S>], raw[2 * j], raw[2 * j + 1]);
+        for (int j = 0; j < 5; ++j) split_pair<S>(p[j * kChunk<S>], raw[2 * j], raw[2 * j + 1]);
+    }
 }
 
 // Moments from stored values (M[0] absolute density).
@@ -282,9 +352,19 @@ __device__ __forceinline__ void store_node(const FieldSet<S> &fs, uint32_t idx, 
                                            const R *M, uint32_t seed = 0) {
     S v[10];
     encode_node<R, S>(idx, sc, rho0, M, seed, v);
-    pair_t<S> *__restrict__ p = pair_ptr(fs.base, idx);
+    if constexpr (kQuads<S>) {
+        S *cb = fs.base + chunk_base<S>(idx);
+        const uint32_t l = idx % kChunk<S>;
+        quad_t<S> *__restrict__ q = reinterpret_cast<quad_t<S> *>(cb) + l;
+        pair_t<S> *__restrict__ p = reinterpret_cast<pair_t<S> *>(cb + 8 * kChunk<S>) + l;
+        q[0] = join_quad<S>(v);
+        q[kChunk<S>] = join_quad<S>(v + 4);
+        p[0] = join_pair<S>(v[8], v[9]);
+    } else {
+        pair_t<S> *__restrict__ p = pair_ptr(fs.base, idx);
 #pragma unroll
-    for (int j = 0; j < 5; ++j) p[j * kChunk<S>] = join_pair<S>(v[2 * j], v[2 * j + 1]);
+        for (int j = 0; j < 5; ++j) p[j * kChunk<S>] = join_pair<S>(v[2 * j], v[2 * j + 1]);
+    }
 }
 
 // Node-major buffers (the 10 values of node i at base[10 i .. 10 i + 9]): the donor snapshot,
