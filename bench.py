@@ -68,9 +68,9 @@ def copy_calibration(nbytes, reps=5):
     return 0
 
 
-def cpu_baseline(desc, steps=0, warmup=2, target_s=12.0):
+def cpu_baseline(desc, steps=0, warmup=2, target_s=16.0):
     """The oracle timed on the host cores. steps = 0 sizes the sample to about target_s seconds
-    from the warm-up steps' duration."""
+    from the warm-up steps' duration (the warm-up steps run slower, so the sample takes less)."""
     from oracle import binding
 
     d = desc.copy()
@@ -198,7 +198,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--precision", default="f32", choices=["f64", "f32", "bf16", "fp16"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=0, help="CPU baseline steps (0: about 12 s of work)")
+    ap.add_argument("--cpu-steps", type=int, default=0, help="CPU baseline steps (0: about 10-16 s of work)")
     ap.add_argument("--cd-steps", type=int, default=2000, help="c5: steps of the BF16-vs-FP64 drag check")
     ap.add_argument("--workload", default="c2", choices=["c2", "c3", "box", "c4", "c5"],
                     help="c2: confined cylinder 256x128x64 (default); c3: wake 512x256x256; "
