@@ -18,11 +18,13 @@ ARCH = os.environ.get("MRLBM_OFFLOAD_ARCH", "gfx950")
 # per-source extra flags: the FP64 kernels and the output arrays are compiled without FMA
 # contraction; the FP32-arithmetic kernels divide and take square roots with the hardware
 # reciprocal and square-root approximations (about 1 ulp, like the bulk path's reciprocal of rho)
-# instead of the correctly rounded sequences, which dominated the boundary-node closed forms
+# instead of the correctly rounded sequences, which dominated the boundary-node closed forms, and
+# flush FP32 denormals (the moments are O(1) and their deviations far above 1e-38), which removes
+# the operand scaling the approximations otherwise need to keep denormal results
 SOURCES = {
     "geometry.cpp": [],
     "kernels_f64.hip": ["-ffp-contract=off"],
-    "kernels_f32.hip": ["-fno-hip-fp32-correctly-rounded-divide-sqrt"],
+    "kernels_f32.hip": ["-fno-hip-fp32-correctly-rounded-divide-sqrt", "-fgpu-flush-denormals-to-zero"],
     "capi.hip": [],
     "batch.hip": [],
     "output.hip": ["-ffp-contract=off"],
