@@ -267,15 +267,17 @@ class Sim final : public SimBase {
         return configure_launch(lc);
     }
 
-    // Fused-pass defaults: 16x8 tiles in FP64 and 32x8 in FP32 (LDS-bound), the ring march
-    // except for 16-bit storage (register windows).
+    // Fused-pass defaults: 16x8 tiles in FP64 and 32x8 in FP32 (LDS-bound), the ring march in
+    // FP64 and register windows in FP32 arithmetic.
     LaunchConfig default_launch() const {
         LaunchConfig lc;
         lc.bx = sizeof(R) == 8 ? 16 : 32;
         lc.by = 8;
-        // register-window march with two LDS planes for FP32 arithmetic: one barrier per plane
-        // and no register cap (the one-plane form spills at its three-waves-per-SIMD bound)
-        lc.variant = sizeof(R) == 8 ? 1 : 4;
+        // register-window march for FP32 arithmetic: with FP32 storage two LDS planes (one
+        // barrier per plane, no register cap), with 16-bit storage one LDS plane, whose three
+        // waves per SIMD hide the decode and encode work better than the two-plane form's two
+        // (it spills a few registers at that bound, and still runs 10-17% faster)
+        lc.variant = sizeof(R) == 8 ? 1 : (Conv<S>::scaled ? 3 : 4);
         // z chunks short enough for about 1024 work items (a block's plane-by-plane march is
         // latency-bound, so small grids need many short chunks), at most 32 planes
         const int64_t tiles = int64_t((g.nx + lc.bx - 1) / lc.bx) * ((g.ny + lc.by - 1) / lc.by);
