@@ -43,6 +43,9 @@ struct Conv<float> {
     __device__ __host__ static float from(R v) { return float(v); }
 };
 
+// Two half values, the operand form of the half conversion with stochastic rounding.
+typedef __attribute__((__vector_size__(2 * sizeof(_Float16)))) _Float16 f16x2_t;
+
 // 16-bit forms round stochastically when given random bits: a uniform integer is added below
 // the kept mantissa bits before truncation, so the stored value is unbiased. Round-to-nearest
 // would drop every per-step increment smaller than half an ulp and stall slowly evolving flow.
@@ -66,10 +69,13 @@ struct Conv<_Float16> {
     static constexpr bool unit_scale = false;
     __device__ static float to(_Float16 v) { return float(v); }
     __device__ static _Float16 from(float v) { return _Float16(v); }
+    // v_cvt_sr_f16_f32 adds the high 13 bits of its random operand below the kept mantissa bits
+    // and truncates; with r8 in the top byte that is v + (r8 << 5) on the float's bits, truncated
+    // to half (13 float mantissa bits fall below the half mantissa in the normal range)
     __device__ static _Float16 from_sr(float v, uint32_t r8) {
-        // 13 float mantissa bits fall below the half mantissa (normal range)
-        const float t = __uint_as_float(__float_as_uint(v) + (r8 << 5));
-        return _Float16(__float2half_rz(t));
+        f16x2_t w = {};
+        w = __builtin_amdgcn_cvt_sr_f16_f32(w, v, r8 << 24, false);
+        return w[0];
     }
 };
 
