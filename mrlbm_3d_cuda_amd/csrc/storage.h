@@ -96,8 +96,8 @@ __device__ __forceinline__ uint32_t sr_bits(uint32_t h, int f) { return (h >> ((
 __device__ __host__ constexpr int field_class(int f) { return f == 0 ? 0 : (f < 4 ? 1 : 2); }
 
 // One moment buffer, node-chunked: the nodes are cut into chunks of kChunk<S> consecutive indices
-// (32, or 16 for FP64: one tile row of the default 32- and 16-wide tiles; two rows of the 16-wide
-// FP32 tiles of cache-resident grids), and the fields of a node are grouped so
+// (32, or 16 for FP64: one tile row of the default 32- and 16-wide tiles; a row of the 16-wide
+// FP32 tiles of cache-resident grids is half a chunk), and the fields of a node are grouped so
 // that a node moves in few wide accesses. FP32 and 16-bit storage group fields 0-3, 4-7 and 8-9:
 // chunk c holds group g of its nodes at element c * 10 kChunk + g * 4 kChunk + width * lane, with
 // width 4, 4, 2 (3 accesses per node: 16 + 16 + 8 bytes in FP32, 8 + 8 + 4 at 16-bit storage).
