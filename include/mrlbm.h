@@ -58,7 +58,7 @@ enum mrlbm_status {
     MRLBM_ERR_NOMEM = 6,       /* allocation failed */
     MRLBM_ERR_IO = 7,          /* checkpoint file missing or unreadable */
     MRLBM_ERR_CORRUPT = 8,     /* checkpoint header, size or checksum mismatch */
-    MRLBM_ERR_NONFINITE = 9,   /* NaN/Inf watchdog tripped */
+    MRLBM_ERR_NONFINITE = 9,   /* NaN/Inf watchdog tripped (see mrlbm_step) */
     MRLBM_ERR_UNSUPPORTED = 10 /* feature not available for this case/precision */
 };
 
@@ -166,10 +166,17 @@ int mrlbm_create_geometry_only(const mrlbm_case_desc *desc, mrlbm_sim **out);
 /* Seeded initial state: rho = rho0, u = U(-a, a) per component, m = u u (or 0, compat). */
 int mrlbm_init(mrlbm_sim *sim);
 
-/* Advance nsteps time steps (asynchronous on the simulation's stream). */
+/*
+ * Advance nsteps time steps (asynchronous on the simulation's stream).
+ * NaN/Inf watchdog (no reference counterpart: the reference has no state check): the fused pass
+ * raises a per-simulation flag when the density it computes at any node is not finite. Once the
+ * flag is up, mrlbm_step queues nothing and returns MRLBM_ERR_NONFINITE, and mrlbm_sync and
+ * mrlbm_get_forces (which still copy their data) return it too. Setting a new state clears it:
+ * mrlbm_init, mrlbm_set_moments, mrlbm_checkpoint_read, mrlbm_checkpoint_import_reference.
+ */
 int mrlbm_step(mrlbm_sim *sim, int32_t nsteps, uint32_t flags);
 
-/* Wait for all work queued on the simulation's stream. */
+/* Wait for all work queued on the simulation's stream (MRLBM_ERR_NONFINITE: see mrlbm_step). */
 int mrlbm_sync(mrlbm_sim *sim);
 
 /* Copy one moment field to / from a host array of nx*ny*nz doubles (x fastest). */
@@ -206,6 +213,18 @@ int mrlbm_get_info(mrlbm_sim *sim, mrlbm_info *info);
  */
 int mrlbm_time_steps(mrlbm_sim *sim, int32_t nsteps, uint32_t flags, double *ms_total,
                      double *ms_kernels);
+
+/*
+ * Time nsteps steps as mrlbm_step runs them, with the step graph captured and instantiated before
+ * the first event (so the timed region holds replays only): nsteps / 16 graph replays of 16 steps,
+ * an HIP event after each, then the remaining steps queued one by one. Writes the whole region's
+ * time to ms_total, the median over the replays of one replay's time divided by its 16 steps to
+ * ms_step_median (if non-null; 0 when nsteps < 16) and the number of replays to replays (if
+ * non-null). Replaces the reference's wall-clock MLUPS timer (calculate_mlups,
+ * src/utils/file_utils.cuh:211-221), which also counts initialisation and output.
+ */
+int mrlbm_time_replays(mrlbm_sim *sim, int32_t nsteps, uint32_t flags, double *ms_total,
+                       double *ms_step_median, int32_t *replays);
 
 /*
  * Checkpoint: a header (magic, version, precision, grid, step count, FNV-1a checksum) and the

@@ -155,6 +155,7 @@ PRODUCT_PROTOTYPES = {
     "mrlbm_get_bcfluid_table": (C.c_int, [C.c_void_p, C.c_int32, _i64p, _u32p, _u32p, _u32p]),
     "mrlbm_get_info": (C.c_int, [C.c_void_p, P(Info)]),
     "mrlbm_time_steps": (C.c_int, [C.c_void_p, C.c_int32, C.c_uint32, _dp, _dp]),
+    "mrlbm_time_replays": (C.c_int, [C.c_void_p, C.c_int32, C.c_uint32, _dp, _dp, P(C.c_int32)]),
     "mrlbm_checkpoint_write": (C.c_int, [C.c_void_p, C.c_char_p]),
     "mrlbm_checkpoint_read": (C.c_int, [C.c_void_p, C.c_char_p]),
     "mrlbm_destroy": (C.c_int, [C.c_void_p]),

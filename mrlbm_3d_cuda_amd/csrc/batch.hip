@@ -68,7 +68,7 @@ class Batch final : public BatchBase {
     // list of case c otherwise goes to group c mod 8; boundary-tile items lead each group.
     int configure(const LaunchConfig &lc) override {
         if (!batch_tile_supported(lc.bx, lc.by)) return set_error(MRLBM_ERR_INVALID, "unsupported batch tile");
-        if (int rc = sync()) return rc;
+        HIP_TRY(hipStreamSynchronize(stream_));
         if (prepare_fused_batch<R, S>(lc.bx, lc.by, lc.variant) != 0)
             return set_error(MRLBM_ERR_UNSUPPORTED, "batched fused kernel cannot get its LDS");
         for (size_t c = 0; c < cases.size(); ++c)
@@ -113,14 +113,23 @@ class Batch final : public BatchBase {
     }
 
     int init() override {
-        if (int rc = sync()) return rc;  // the cases' own streams take over
+        HIP_TRY(hipStreamSynchronize(stream_));  // the cases' own streams take over
         for (size_t c = 0; c < cases.size(); ++c)
             if (int rc = sim(c).init()) return rc;
         steps = 0;
         return MRLBM_OK;
     }
 
+    // A case whose NaN/Inf watchdog has tripped stops the whole batch (MRLBM_ERR_NONFINITE).
+    int check_finite() const {
+        for (const auto &c : cases)
+            if (c->impl->nonfinite())
+                return set_error(MRLBM_ERR_NONFINITE, "non-finite moments in a batched case: the watchdog tripped");
+        return MRLBM_OK;
+    }
+
     int step(int nsteps, uint32_t flags) override {
+        if (int rc = check_finite()) return rc;
         int done = 0;
         if (graph_steps_ > 0) {
             while (nsteps - done >= graph_steps_) {
@@ -144,14 +153,19 @@ class Batch final : public BatchBase {
 
     int sync() override {
         HIP_TRY(hipStreamSynchronize(stream_));
-        return MRLBM_OK;
+        return check_finite();
     }
 
     int time_steps(int n, uint32_t flags, double *ms_total) override {
         if (n <= 0) return set_error(MRLBM_ERR_INVALID, "nsteps must be positive");
+        if (graph_steps_ > 0 && n >= graph_steps_) {
+            hipGraphExec_t exec = nullptr;
+            if (int rc = graph_for(flags, exec)) return rc;  // captured outside the timed region
+        }
         hipEvent_t e0 = nullptr, e1 = nullptr;
         HIP_TRY(hipEventCreate(&e0));
         HIP_TRY(hipEventCreate(&e1));
+        HIP_TRY(hipStreamSynchronize(stream_));
         HIP_TRY(hipEventRecord(e0, stream_));
         int rc = step(n, flags);
         HIP_TRY(hipEventRecord(e1, stream_));

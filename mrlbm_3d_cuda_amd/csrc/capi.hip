@@ -166,6 +166,7 @@ int mrlbm_get_forces(mrlbm_sim *s, int64_t first, int32_t count, double *out4) {
 
 int mrlbm_get_node_types(mrlbm_sim *s, uint32_t *out) {
     SIM_CHECK(s);
+    if (!out) return set_error(MRLBM_ERR_INVALID, "null output");
     const mrlbm::Geometry &g = s->impl->g;
     for (int64_t i = 0; i < g.n; ++i) out[i] = (uint32_t(g.ref_type[i]) << 20) | (g.ref_index[i] & 0xFFFFFu);
     return MRLBM_OK;
@@ -176,6 +177,8 @@ int mrlbm_get_wall_table(mrlbm_sim *s, int32_t group, int64_t *node, double *nx,
     SIM_CHECK(s);
     if (group < 0 || group > 1) return set_error(MRLBM_ERR_INVALID, "bad wall group");
     const mrlbm::WallTable &w = s->impl->g.wall[group];
+    if (!w.node.empty() && (!node || !nx || !ny || !delta || !in_mask || !out_mask))
+        return set_error(MRLBM_ERR_INVALID, "null output");
     for (size_t k = 0; k < w.node.size(); ++k) {
         node[k] = w.node[k];
         nx[k] = w.nx[k];
@@ -193,6 +196,8 @@ int mrlbm_get_bcfluid_table(mrlbm_sim *s, int32_t group, int64_t *node, uint32_t
     if (group < 0 || group > 1) return set_error(MRLBM_ERR_INVALID, "bad wall group");
     const mrlbm::Geometry &g = s->impl->g;
     const mrlbm::WallTable &w = g.wall[group];
+    if ((!w.bcf_node.empty() && (!node || !tag)) || !in256 || !out256)
+        return set_error(MRLBM_ERR_INVALID, "null output");
     for (size_t k = 0; k < w.bcf_node.size(); ++k) {
         node[k] = w.bcf_node[k];
         tag[k] = w.bcf_tag[k];
@@ -206,6 +211,7 @@ int mrlbm_get_bcfluid_table(mrlbm_sim *s, int32_t group, int64_t *node, uint32_t
 
 int mrlbm_get_info(mrlbm_sim *s, mrlbm_info *info) {
     SIM_CHECK(s);
+    if (!info) return set_error(MRLBM_ERR_INVALID, "null output");
     const mrlbm::Geometry &g = s->impl->g;
     std::memset(info, 0, sizeof(*info));
     info->num_nodes = g.n;
@@ -226,6 +232,16 @@ int mrlbm_time_steps(mrlbm_sim *s, int32_t nsteps, uint32_t flags, double *ms_to
     OWN_CHECK(s);
     if (!ms_total) return set_error(MRLBM_ERR_INVALID, "null output");
     return s->impl->time_steps(nsteps, flags, ms_total, ms_kernels);
+}
+
+int mrlbm_time_replays(mrlbm_sim *s, int32_t nsteps, uint32_t flags, double *ms_total, double *ms_step_median,
+                       int32_t *replays) {
+    OWN_CHECK(s);
+    if (!ms_total) return set_error(MRLBM_ERR_INVALID, "null output");
+    int nrep = 0;
+    const int rc = s->impl->time_replays(nsteps, flags, ms_total, ms_step_median, &nrep);
+    if (replays) *replays = nrep;
+    return rc;
 }
 
 int mrlbm_checkpoint_write(mrlbm_sim *s, const char *path) {

@@ -53,6 +53,9 @@ struct FusedArgs {
     // records of the case: with a non-null `pending` the launch has one block more, which files
     // the previous step's record
     RecordArgs rec;
+    // NaN/Inf watchdog: set to 1 by any wave that computes a non-finite density (mapped host
+    // memory, read by the host without synchronising; null: no watchdog)
+    uint32_t *__restrict__ nonfinite;
 };
 
 // Arguments of the curved-wall + force pass (one thread per wall node, then one per BCFLUID).
@@ -145,6 +148,11 @@ void launch_fused(const FusedArgs<R, S> &a, int bx, int by, int variant, int par
 
 template <class R, class S>
 bool fused_part_supported(int bx, int by, int variant);
+
+// Gives the fused kernel of this configuration the dynamic LDS it needs (once per kernel, before
+// any launch or graph capture); 0 on success, -1 for a configuration that is not instantiated.
+template <class R, class S>
+int prepare_fused(int bx, int by, int variant, int part);
 
 template <class R, class S>
 void launch_wall(const WallArgs<R, S> &a, hipStream_t stream);

@@ -20,7 +20,6 @@
 #pragma once
 
 #include <algorithm>
-#include <mutex>
 #include <vector>
 
 #include "kernels.h"
@@ -100,6 +99,15 @@ struct Frame {
 __device__ __forceinline__ int pmod(int v, int n) {
     const int r = v % n;
     return r < 0 ? r + n : r;
+}
+
+// NaN/Inf watchdog: a wave in which some node's new density is not finite raises the case's flag
+// (every lane of such a wave stores the same 1). A non-finite moment anywhere reaches the density of
+// its own node within a step: the rest population is part of the node's density sum.
+template <class R, class S>
+__device__ __forceinline__ void watch_finite(const FusedArgs<R, S> &a, R rho) {
+    if (__builtin_expect(__any(!__builtin_isfinite(rho)), 0))
+        if (a.nonfinite) *a.nonfinite = 1u;
 }
 
 // Per-node table values of a boundary node, loaded one iteration before its update with
@@ -401,9 +409,11 @@ __device__ __forceinline__ void fused_march(const FusedArgs<R, S> &a, R *lds, in
         const uint32_t node = off0 + uint32_t(z) * plane;
         if constexpr (SPECIAL) {
             bp.update(a, f, tx, ty, inside, east_tile, node, wold + west_slot(z) * 10 * BY);
+            watch_finite(a, bp.Mp[0]);
         } else {
             moments_lds(f, Mp);
             collide(a.omega, Mp);
+            watch_finite(a, Mp[0]);
             idxp = inside ? node : a.pad;
         }
         lds_barrier();
@@ -553,11 +563,13 @@ __device__ __forceinline__ void window_march(const FusedArgs<R, S> &a, R *lds, i
         const uint32_t node = off0 + uint32_t(z) * plane;
         if constexpr (SPECIAL) {
             bp.update(a, f, tx, ty, inside, east_tile, node, wold + west_slot(z) * 10 * BY);
+            watch_finite(a, bp.Mp[0]);
             bp.copy_own(bp.own_nx, bp.own_cur);
             bp.info_cur = bp.info_nx;
         } else {
             moments_lds(f, Mp);
             collide(a.omega, Mp);
+            watch_finite(a, Mp[0]);
             idxp = inside ? node : a.pad;
         }
     };
@@ -920,32 +932,14 @@ bool fused_part_supported(int bx, int by, int variant) {
     return ((bx == 32 && by == 8) || (bx == 16 && by == 8)) && (variant == 1 || variant == 4);
 }
 
+// Kernel of the fused pass for a tile, march variant and launch part (null if not instantiated).
 template <class R, class S>
-void launch_fused(const FusedArgs<R, S> &a, int bx, int by, int variant, int part, hipStream_t stream) {
-    const size_t lds = fused_lds_bytes(bx, by, sizeof(R), variant);
-    auto go = [&](void (*kernel)(FusedArgs<R, S>), int nt) {
-        // kernels needing more than the default dynamic LDS limit get the attribute once each
-        static std::mutex mu;
-        static std::vector<const void *> done;
-        if (lds > 65536) {
-            std::lock_guard<std::mutex> lock(mu);
-            const void *k = reinterpret_cast<const void *>(kernel);
-            if (std::find(done.begin(), done.end(), k) == done.end()) {
-                (void)hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
-                done.push_back(k);
-            }
-        }
-        hipLaunchKernelGGL(kernel, dim3(a.nblocks + (a.rec.pending ? 1 : 0)), dim3(nt), lds, stream, a);
-    };
+auto fused_kernel_for(int bx, int by, int variant, int part) -> void (*)(FusedArgs<R, S>) {
 #define MRLBM_FUSED_TILE(TX, TY, P)                                                  \
     if (bx == TX && by == TY && part == P) {                                         \
-        if (variant == 3)                                                            \
-            go(fused_window_kernel<R, S, TX, TY, 1, P>, TX * TY);                    \
-        else if (variant == 4)                                                       \
-            go(fused_window_kernel<R, S, TX, TY, 2, P>, TX * TY);                    \
-        else                                                                         \
-            go(fused_step_ring6_kernel<R, S, TX, TY, P>, TX * TY);                   \
-        return;                                                                      \
+        if (variant == 3) return fused_window_kernel<R, S, TX, TY, 1, P>;            \
+        if (variant == 4) return fused_window_kernel<R, S, TX, TY, 2, P>;            \
+        return fused_step_ring6_kernel<R, S, TX, TY, P>;                             \
     }
     MRLBM_FUSED_TILE(32, 8, PART_ALL)
     MRLBM_FUSED_TILE(16, 8, PART_ALL)
@@ -957,17 +951,37 @@ void launch_fused(const FusedArgs<R, S> &a, int bx, int by, int variant, int par
     // the two-stream schedule's launches: tiles 32x8 and 16x8, ring or two-plane window march
 #define MRLBM_FUSED_PART(TX, TY, P)                                                  \
     if (bx == TX && by == TY && part == P) {                                         \
-        if (variant == 4)                                                            \
-            go(fused_window_kernel<R, S, TX, TY, 2, P>, TX * TY);                    \
-        else if (variant == 1)                                                       \
-            go(fused_step_ring6_kernel<R, S, TX, TY, P>, TX * TY);                   \
-        return;                                                                      \
+        if (variant == 4) return fused_window_kernel<R, S, TX, TY, 2, P>;            \
+        if (variant == 1) return fused_step_ring6_kernel<R, S, TX, TY, P>;           \
+        return nullptr;                                                              \
     }
     MRLBM_FUSED_PART(32, 8, PART_INTERIOR)
     MRLBM_FUSED_PART(16, 8, PART_INTERIOR)
     MRLBM_FUSED_PART(32, 8, PART_WALL_SIDE)
     MRLBM_FUSED_PART(16, 8, PART_WALL_SIDE)
 #undef MRLBM_FUSED_PART
+    return nullptr;
+}
+
+template <class R, class S>
+int prepare_fused(int bx, int by, int variant, int part) {
+    auto kernel = fused_kernel_for<R, S>(bx, by, variant, part);
+    if (!kernel) return -1;
+    const size_t lds = fused_lds_bytes(bx, by, sizeof(R), variant);
+    if (lds <= 65536) return 0;
+    return hipFuncSetAttribute(reinterpret_cast<const void *>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
+                               int(lds)) == hipSuccess
+               ? 0
+               : -1;
+}
+
+// The kernel's dynamic LDS limit is set by prepare_fused when the configuration is chosen.
+template <class R, class S>
+void launch_fused(const FusedArgs<R, S> &a, int bx, int by, int variant, int part, hipStream_t stream) {
+    const size_t lds = fused_lds_bytes(bx, by, sizeof(R), variant);
+    auto kernel = fused_kernel_for<R, S>(bx, by, variant, part);
+    if (kernel)
+        hipLaunchKernelGGL(kernel, dim3(a.nblocks + (a.rec.pending ? 1 : 0)), dim3(bx * by), lds, stream, a);
 }
 
 // Kernel of the batched fused pass for a tile and variant (null if not instantiated).
@@ -1085,6 +1099,7 @@ void launch_decode(const FieldSet<const S> &in, double *out, uint32_t n, const S
 
 #define MRLBM_INSTANTIATE(R, S)                                                                             \
     template void launch_fused<R, S>(const FusedArgs<R, S> &, int, int, int, int, hipStream_t);             \
+    template int prepare_fused<R, S>(int, int, int, int);                                                   \
     template bool fused_part_supported<R, S>(int, int, int);                                                \
     template void launch_wall<R, S>(const WallArgs<R, S> &, hipStream_t);                                   \
     template void launch_pressure<R, S>(const PressureArgs<R, S> &, hipStream_t);                           \

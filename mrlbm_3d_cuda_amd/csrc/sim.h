@@ -62,6 +62,18 @@ class SimBase {
     // four float32 point arrays of a VTK file from the current state (MRLBM_VTI_* layout)
     virtual int vti_points(int layout, float vel_norm, float *host) = 0;
     virtual LaunchConfig current_launch() const = 0;
+    // Steps as the library runs them with every graph captured beforehand (the timed region holds
+    // no capture): ms_total for all n steps, ms_step the median over the graph replays of one
+    // replay's duration per step (0 without a replay), replays their count.
+    virtual int time_replays(int n, uint32_t flags, double *ms_total, double *ms_step, int *replays) = 0;
+    // NaN/Inf watchdog: raised by the fused pass when a node's density is not finite; cleared when
+    // a new state is set (init, set_moments, checkpoint reads).
+    virtual bool nonfinite() const = 0;
+    int device = 0;  // HIP device the simulation's memory lives on
+    int check_finite() const {
+        return nonfinite() ? set_error(MRLBM_ERR_NONFINITE, "non-finite moments: the NaN/Inf watchdog tripped")
+                           : int(MRLBM_OK);
+    }
     mrlbm_case_desc d{};
     Geometry g;
     int64_t steps = 0;
@@ -83,10 +95,17 @@ class Sim final : public SimBase {
 
     int setup() override {
         bytes_per_value = int(sizeof(S));
+        HIP_TRY(hipGetDevice(&device));
         HIP_TRY(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
         HIP_TRY(hipStreamCreateWithFlags(&stream2_, hipStreamNonBlocking));
         HIP_TRY(hipEventCreateWithFlags(&ev_wall_side_, hipEventDisableTiming));
         HIP_TRY(hipEventCreateWithFlags(&ev_interior_, hipEventDisableTiming));
+        // watchdog flag in mapped, coherent host memory: kernels store to it, the host reads it
+        // at any time
+        HIP_TRY(hipHostMalloc(reinterpret_cast<void **>(&h_nonfinite_), sizeof(uint32_t),
+                              hipHostMallocMapped | hipHostMallocCoherent));
+        *h_nonfinite_ = 0u;
+        HIP_TRY(hipHostGetDevicePointer(reinterpret_cast<void **>(&d_nonfinite_), h_nonfinite_, 0));
         const int64_t n = g.n;
         // at least one padding slot after the last node: bulk threads outside the grid store there
         stride_ = size_t((n + 1 + 63) / 64 * 64);
@@ -306,6 +325,8 @@ class Sim final : public SimBase {
         HIP_TRY(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
         if (fused_lds_bytes(lc.bx, lc.by, sizeof(R), lc.variant) > size_t(max_lds))
             return set_error(MRLBM_ERR_UNSUPPORTED, "fused tile needs more LDS than a block may use");
+        if (prepare_fused<R, S>(lc.bx, lc.by, lc.variant, PART_ALL) != 0)
+            return set_error(MRLBM_ERR_UNSUPPORTED, "fused kernel of this configuration cannot get its LDS");
         const int tiles_x = (g.nx + lc.bx - 1) / lc.bx;
         const int tiles_y = (g.ny + lc.by - 1) / lc.by;
         const int ntiles = tiles_x * tiles_y;
@@ -385,13 +406,17 @@ class Sim final : public SimBase {
         xcd_ = groups;
         lists_ = std::move(lists[0]);
         launch_ = lc;
-        // two streams per step where the fused pass reads no step counter (no stochastic
-        // rounding), the case has curved walls and both launches have work. By default only in
-        // FP64, whose steps are long enough for the hidden wall pass to outweigh the two
-        // cross-stream waits per step (measured on the 256x128x64 channel: FP64 +4%, FP32 -18%).
-        const bool two = two_stream_env_ < 0 ? sizeof(R) == 8 : two_stream_env_ > 0;
+        // two streams per step (MRLBM_TWO_STREAM=1 only) where the fused pass reads no step
+        // counter (no stochastic rounding), the case has curved walls and both launches have work.
+        // Off by default: with grouped fields it no longer gains in FP64 (256x128x64 channel,
+        // 12,444 / 12,428 MLUPS one stream against 12,334 / 12,475 two streams) and loses 18% in
+        // FP32.
+        const bool two = two_stream_env_ > 0;
         two_stream_ = two && !stochastic_ && nwall_ > 0 && nitems_[1] > 0 && nitems_[2] > 0 &&
                       fused_part_supported<R, S>(lc.bx, lc.by, lc.variant);
+        if (two_stream_ && (prepare_fused<R, S>(lc.bx, lc.by, lc.variant, PART_INTERIOR) != 0 ||
+                            prepare_fused<R, S>(lc.bx, lc.by, lc.variant, PART_WALL_SIDE) != 0))
+            return set_error(MRLBM_ERR_UNSUPPORTED, "two-stream fused kernels cannot get their LDS");
         return MRLBM_OK;
     }
 
@@ -507,6 +532,9 @@ class Sim final : public SimBase {
     int nbcf() const { return nbcf_; }
     int parity() const { return cur_; }
     hipStream_t stream() const { return stream_; }
+    // the buffer holding the current state and its storage scales
+    FieldSet<const S> state() const { return as_const(make_fields(buf_[cur_])); }
+    const StoreScale<R> &store_scale() const { return scale_; }
 
     // Kernel arguments of the fused pass from buffer parity `cur`, over every work item or the
     // interior / wall-side items of the two-stream schedule.
@@ -537,6 +565,7 @@ class Sim final : public SimBase {
         a.omega = omega_;
         a.bcf_strong = (d.compat_flags & MRLBM_COMPAT_BCF_MASS_STRONG) ? 1 : 0;
         a.rho0 = rho0_;
+        a.nonfinite = d_nonfinite_;
         return a;
     }
 
@@ -603,7 +632,11 @@ class Sim final : public SimBase {
         cur_ ^= 1;
         ++steps;
         if (stats_on(flags)) ++stats_samples_;
-        launched_ = true;
+    }
+
+    bool nonfinite() const override { return h_nonfinite_ && __atomic_load_n(h_nonfinite_, __ATOMIC_ACQUIRE) != 0u; }
+    void clear_nonfinite() {
+        if (h_nonfinite_) __atomic_store_n(h_nonfinite_, 0u, __ATOMIC_RELEASE);
     }
 
     // Seeded initial state: u_a = a (2 r - 1), r = hash(seed, a N + i); rho = rho0; m = u u
@@ -626,10 +659,12 @@ class Sim final : public SimBase {
     }
 
     // Steps are queued eagerly for short runs; longer runs replay a captured graph of
-    // graph_steps_ steps (an even number, so the ping-pong parity is unchanged).
+    // graph_steps_ steps (an even number, so the ping-pong parity is unchanged). A simulation
+    // whose watchdog has tripped queues nothing.
     int step(int nsteps, uint32_t flags) override {
+        if (int rc = check_finite()) return rc;
         int done = 0;
-        if (graph_steps_ > 0 && launched_) {
+        if (graph_steps_ > 0) {
             while (nsteps - done >= graph_steps_) {
                 hipGraphExec_t exec = nullptr;
                 int rc = graph_for(flags, exec);
@@ -658,7 +693,7 @@ class Sim final : public SimBase {
     int sync() override {
         if (int rc = wait_batch()) return rc;
         HIP_TRY(hipStreamSynchronize(stream_));
-        return MRLBM_OK;
+        return check_finite();
     }
 
     int get(int f, double *h) override {
@@ -693,6 +728,7 @@ class Sim final : public SimBase {
                 img[field_elem<S>(uint32_t(i), f)] = in[size_t(f) * g.n + i];
         for (int b = 0; b < 2; ++b)  // both buffers (frozen nodes)
             HIP_TRY(hipMemcpy(buf_[b], img.data(), elems_ * sizeof(S), hipMemcpyHostToDevice));
+        clear_nonfinite();  // a new state
         return MRLBM_OK;
     }
 
@@ -717,13 +753,18 @@ class Sim final : public SimBase {
                               hipMemcpyDeviceToHost));
             k += len;
         }
-        return MRLBM_OK;
+        // the records are written either way; the status tells whether the state blew up
+        return check_finite();
     }
 
     int time_steps(int n, uint32_t flags, double *ms_total, double *ms_k) override {
         if (n <= 0) return set_error(MRLBM_ERR_INVALID, "nsteps must be positive");
         std::vector<hipEvent_t> ev(size_t(4) * n + 2);
         for (auto &e : ev) HIP_TRY(hipEventCreate(&e));
+        if (!ms_k && graph_steps_ > 0 && n >= graph_steps_) {
+            hipGraphExec_t exec = nullptr;
+            if (int rc = graph_for(flags, exec)) return rc;  // captured outside the timed region
+        }
         HIP_TRY(hipStreamSynchronize(stream_));
         HIP_TRY(hipEventRecord(ev[0], stream_));
         if (ms_k) {
@@ -757,6 +798,52 @@ class Sim final : public SimBase {
         }
         for (auto &e : ev) (void)hipEventDestroy(e);
         return MRLBM_OK;
+    }
+
+    int time_replays(int n, uint32_t flags, double *ms_total, double *ms_step, int *replays) override {
+        if (n <= 0) return set_error(MRLBM_ERR_INVALID, "nsteps must be positive");
+        if (int rc = check_finite()) return rc;
+        const int nrep = graph_steps_ > 0 ? n / graph_steps_ : 0;
+        hipGraphExec_t exec = nullptr;
+        if (nrep > 0)
+            if (int rc = graph_for(flags, exec)) return rc;  // an even replay leaves the parity as it is
+        std::vector<hipEvent_t> ev(size_t(nrep) + 2, nullptr);
+        auto done = [&](int rc) {
+            for (auto &e : ev)
+                if (e) (void)hipEventDestroy(e);
+            return rc;
+        };
+        for (auto &e : ev)
+            if (hipEventCreate(&e) != hipSuccess) return done(set_error(MRLBM_ERR_DEVICE, "hipEventCreate failed"));
+        HIP_TRY(hipStreamSynchronize(stream_));
+        HIP_TRY(hipEventRecord(ev[0], stream_));
+        for (int r = 0; r < nrep; ++r) {
+            HIP_TRY(hipGraphLaunch(exec, stream_));
+            steps += graph_steps_;
+            if (stats_on(flags)) stats_samples_ += graph_steps_;
+            HIP_TRY(hipEventRecord(ev[size_t(r) + 1], stream_));
+        }
+        open_streams();
+        for (int s = nrep * graph_steps_; s < n; ++s) queue_step(flags, nullptr);
+        join_streams();
+        queue_record(flags);
+        HIP_TRY(hipEventRecord(ev[size_t(nrep) + 1], stream_));
+        HIP_TRY(hipEventSynchronize(ev[size_t(nrep) + 1]));
+        HIP_TRY(hipGetLastError());
+        float ms = 0.f;
+        HIP_TRY(hipEventElapsedTime(&ms, ev[0], ev[size_t(nrep) + 1]));
+        *ms_total = ms;
+        std::vector<double> per;
+        for (int r = 0; r < nrep; ++r) {
+            float t = 0.f;
+            HIP_TRY(hipEventElapsedTime(&t, ev[size_t(r)], ev[size_t(r) + 1]));
+            per.push_back(double(t) / graph_steps_);
+        }
+        std::sort(per.begin(), per.end());
+        if (ms_step) *ms_step = per.empty() ? 0.0 : (per.size() % 2 ? per[per.size() / 2]
+                                                                   : 0.5 * (per[per.size() / 2 - 1] + per[per.size() / 2]));
+        if (replays) *replays = nrep;
+        return done(check_finite());
     }
 
   private:
@@ -795,7 +882,6 @@ class Sim final : public SimBase {
         }
         if (ev) (void)hipEventRecord(ev[0], stream_);
         launch_fused<R, S>(fused_args(cur_), bx_, by_, variant_, PART_ALL, stream_);
-        launched_ = true;
         if (ev) (void)hipEventRecord(ev[1], stream_);
         if (has_wall_launch(flags)) {
             if (ev) (void)hipEventRecord(ev[2], stream_);
@@ -826,7 +912,6 @@ class Sim final : public SimBase {
         (void)hipEventRecord(ev_wall_side_, stream_);
         launch_fused<R, S>(fused_args(cur_, PART_INTERIOR), bx_, by_, variant_, PART_INTERIOR, stream2_);
         (void)hipEventRecord(ev_interior_, stream2_);
-        launched_ = true;
         if (has_wall_launch(flags)) launch_wall<R, S>(wall_args(cur_, forces_on(flags)), stream_);
         if (stats_on(flags) || planes_on(flags)) {
             (void)hipStreamWaitEvent(stream_, ev_interior_, 0);
@@ -863,7 +948,7 @@ class Sim final : public SimBase {
         if (all10.size() != size_t(10) * size_t(g.n)) return set_error(MRLBM_ERR_INVALID, "field data size");
         if (int rc = wait_batch()) return rc;
         HIP_TRY(hipStreamSynchronize(stream_));
-        return put_all(all10);
+        return put_all(all10);  // clears the watchdog
     }
 
     int put_all(const std::vector<double> &h) {
@@ -876,6 +961,7 @@ class Sim final : public SimBase {
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipStreamSynchronize(stream_));
         HIP_TRY(hipFree(tmp));
+        clear_nonfinite();  // a new state
         return MRLBM_OK;
     }
 
@@ -903,6 +989,8 @@ class Sim final : public SimBase {
                         d_partials_, d_ring_, d_step_counter_, d_pending_, d_ps_sum_, d_plane_ring_};
         for (void *p : ptrs)
             if (p) (void)hipFree(p);
+        if (h_nonfinite_) (void)hipHostFree(h_nonfinite_);
+        h_nonfinite_ = d_nonfinite_ = nullptr;
         if (ev_wall_side_) (void)hipEventDestroy(ev_wall_side_);
         if (ev_interior_) (void)hipEventDestroy(ev_interior_);
         if (stream2_) (void)hipStreamDestroy(stream2_);
@@ -928,8 +1016,9 @@ class Sim final : public SimBase {
     int variant_ = 1;
     bool stochastic_ = Conv<S>::scaled;
     int graph_steps_ = 16;
-    bool launched_ = false;
     hipGraphExec_t graphs_[8] = {};
+    uint32_t *h_nonfinite_ = nullptr;  // watchdog flag (mapped host memory) and its device address
+    uint32_t *d_nonfinite_ = nullptr;
     int bx_ = 32, by_ = 8, lz_ = 16, lz_special_ = 16, tiles_x_ = 1, tiles_y_ = 1, ntiles_ = 1, xcd_ = 1;
     int nitems_[3] = {1, 0, 0};  // work items (blocks) of the three item lists
     std::vector<uint32_t> special_cols_;  // x + nx y of columns holding non-bulk nodes

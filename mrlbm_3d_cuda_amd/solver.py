@@ -120,6 +120,14 @@ class Simulation:
                                          per if per_kernel else None))
         return total.value, ((per[0], per[1]) if per_kernel else None)
 
+    def time_replays(self, n, forces=False):
+        """Device time of n steps run as graph replays captured beforehand: (ms for all n steps,
+        median ms per step over the replays, number of replays)."""
+        total, med, reps = C.c_double(), C.c_double(), C.c_int32()
+        _check(self.lib.mrlbm_time_replays(self.h, n, _abi.STEP_FORCES if forces else 0, C.byref(total),
+                                           C.byref(med), C.byref(reps)))
+        return total.value, med.value, reps.value
+
     # -- state
     def field(self, f):
         fi = _abi.FIELDS.index(f) if isinstance(f, str) else int(f)

@@ -1,0 +1,262 @@
+"""The benchmark configurations themselves on the GPU, at their stated sizes.
+
+* C2, confined cylinder 256x128x64 (nz = 64: one z slab per XCD): FP64 bit for bit against the CPU
+  oracle over 100 steps (forces included), FP32 within the bounds stated below of the same oracle
+  run, and the 20,000-step per-step drag/lift record in FP32 against the GPU FP64 run of the case.
+* A 64x48x40 channel (nz >= 32, ragged slabs of 5 planes): FP64 bit for bit against the oracle.
+* C3, cylinder wake 512x256x256: FP32, BF16 and FP16 storage against the GPU FP64 run of the same
+  case (velocity L2 error and mean-drag error, bounds stated per precision).
+* C4, all 64 seeded annuli as one FP32 batch for 5,000 steps against the FP64 batch, and the FP64
+  batch against the oracle on the smallest case over all 5,000 steps.
+* C5, the first 16 seeded channels as BF16 / FP16 / FP32 batches for 10,000 steps against the FP64
+  batch (mean-drag error), and the 4 smallest channels as an FP64 batch against the oracle.
+
+The FP64 GPU runs are the oracle's stand-in on the large cases: the same kernel equals the CPU
+oracle bit for bit on C1, C2 (above) and the small cases.
+"""
+import numpy as np
+import pytest
+
+from mrlbm_3d_cuda_amd import _abi, configs
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-12
+
+
+def rel_linf(a, b):
+    scale = np.max(np.abs(b))
+    return float(np.max(np.abs(a - b)) / scale) if scale > 0 else float(np.max(np.abs(a - b)))
+
+
+def field_errors(G, O):
+    return {name: rel_linf(G[idx], O[idx]) for name, idx in {"rho": [0], "u": [1, 2, 3],
+                                                             "m": [4, 5, 6, 7, 8, 9]}.items()}
+
+
+def boundary_count(o):
+    i = o.info()
+    return i.nb_inner + i.nb_outer + i.nb_fluid_inner + i.nb_fluid_outer
+
+
+def force_error(FG, FO, n_boundary, rho0=1.0):
+    return float(np.max(np.abs(FG - FO)) / (2.0 * rho0 * n_boundary))
+
+
+def drag(forces, d):
+    """Cd = 2 Fx / (rho0 U^2 D Lz) and Cl likewise with Fy, per record."""
+    k = 2.0 / (d.rho0 * d.u_max ** 2 * 2.0 * d.r_inner * d.nz)
+    return forces[:, 0] * k, forces[:, 1] * k
+
+
+def gpu_run(desc, steps, fields=True):
+    from mrlbm_3d_cuda_amd.solver import Simulation
+    print(f"[gpu configs] {desc.nx}x{desc.ny}x{desc.nz} {_abi.PRECISION_NAMES[desc.precision]} {steps} steps",
+          flush=True)
+    with Simulation(desc) as s:
+        s.init().step(steps, forces=True)
+        s.sync()
+        F = s.fields() if fields else None
+        f = s.forces(0, steps)
+    return F, f
+
+
+@pytest.fixture(scope="module")
+def c2_oracle():
+    from oracle.binding import Oracle
+    d = configs.c2(precision=_abi.FP64)
+    print("[gpu configs] C2 oracle, 100 FP64 steps on the host", flush=True)
+    o = Oracle(d).init()
+    o.step(100, _abi.STEP_FORCES)
+    out = (o.fields(), o.forces(0, 100), boundary_count(o))
+    o.close()
+    return out
+
+
+def test_c2_fp64_bitwise_vs_oracle(c2_oracle):
+    O, FO, nb = c2_oracle
+    G, FG = gpu_run(configs.c2(precision=_abi.FP64), 100)
+    errs = field_errors(G, O)
+    print(f"[gpu configs] C2 FP64 vs oracle: {errs}; bitwise-equal values {np.mean(G == O):.6f}", flush=True)
+    assert all(e <= FP64_TOL for e in errs.values()), errs
+    assert force_error(FG, FO, nb) <= FP64_TOL
+    assert np.array_equal(G, O)
+
+
+def test_c2_fp32_vs_oracle(c2_oracle):
+    """FP32 storage and arithmetic at C2 against the FP64 oracle: every physical field within 1e-4
+    of its peak, per-step drag within 1e-3 of the gross boundary flux."""
+    O, FO, nb = c2_oracle
+    G, FG = gpu_run(configs.c2(precision=_abi.FP32), 100)
+    errs = field_errors(G, O)
+    print(f"[gpu configs] C2 FP32 vs FP64 oracle: {errs}", flush=True)
+    assert errs["u"] <= 1e-4 and errs["rho"] <= 1e-4 and errs["m"] <= 1e-4, errs
+    assert force_error(FG, FO, nb) <= 1e-5
+
+
+def test_channel_nz40_fp64_bitwise_vs_oracle():
+    """nz = 40 >= 32: the work items are cut into one z slab per XCD (8 slabs of 5 planes) and
+    merged round robin; FP64 results equal the oracle's bit for bit."""
+    from oracle.binding import Oracle
+    d = configs.cylinder(nx=64, ny=48, nz=40, diameter=10.0, xc=20.0, yc=24.0, u_max=0.05, re=60.0,
+                         precision=_abi.FP64, seed=configs.SEED_BASE | 0x40)
+    G, FG = gpu_run(d, 120)
+    o = Oracle(d).init()
+    o.step(120, _abi.STEP_FORCES)
+    O, FO = o.fields(), o.forces(0, 120)
+    assert np.array_equal(G, O), field_errors(G, O)
+    assert force_error(FG, FO, boundary_count(o)) <= FP64_TOL
+
+
+def test_c2_20k_step_drag_lift_record(tmp_path):
+    """C2 as specified: 20,000 FP32 steps with per-step Cd and Cl, written as forces.dat in the
+    reference's format, against the GPU FP64 run of the case: mean Cd over the last 10,000 steps
+    within 0.5%, r.m.s. lift and the period of its dominant frequency within 2%."""
+    from mrlbm_3d_cuda_amd.solver import Simulation
+    steps = 20000
+    rec = {}
+    for prec in (_abi.FP64, _abi.FP32):
+        d = configs.c2(precision=prec)
+        print(f"[gpu configs] C2 {_abi.PRECISION_NAMES[prec]} {steps} steps", flush=True)
+        with Simulation(d) as s:
+            s.init().step(steps, forces=True)
+            s.sync()
+            rec[prec] = s.forces(0, steps)
+            if prec == _abi.FP32:
+                s.write_forces(str(tmp_path), 0, steps)
+                lines = open(tmp_path / "forces.dat").read().splitlines()
+                assert len(lines) == steps and lines[-1].split()[0] == str(steps - 1)
+    out = {}
+    for prec, f in rec.items():
+        cd, cl = drag(f, configs.c2(precision=prec))
+        tail = cl[steps // 2:] - cl[steps // 2:].mean()
+        spec = np.abs(np.fft.rfft(tail))
+        k = int(np.argmax(spec[1:])) + 1  # dominant lift frequency over the last 10,000 steps
+        out[prec] = dict(cd=float(cd[steps // 2:].mean()), cl_rms=float(np.sqrt(np.mean(tail ** 2))),
+                         period=float(len(tail) / k))
+    print(f"[gpu configs] C2 20k steps: FP64 {out[_abi.FP64]}, FP32 {out[_abi.FP32]}", flush=True)
+    r64, r32 = out[_abi.FP64], out[_abi.FP32]
+    assert np.isfinite(rec[_abi.FP32]).all()
+    assert 0.5 < r64["cd"] < 5.0
+    assert abs(r32["cd"] - r64["cd"]) / r64["cd"] < 5e-3
+    assert abs(r32["cl_rms"] - r64["cl_rms"]) / r64["cl_rms"] < 2e-2
+    assert abs(r32["period"] - r64["period"]) / r64["period"] < 2e-2
+
+
+C3_STEPS = 10000
+
+
+def c3_velocity_and_drag(precision):
+    from mrlbm_3d_cuda_amd.solver import Simulation
+    d = configs.c3(precision=precision)
+    print(f"[gpu configs] C3 {_abi.PRECISION_NAMES[precision]} {C3_STEPS} steps", flush=True)
+    with Simulation(d) as s:
+        s.init().step(C3_STEPS, forces=True)
+        s.sync()
+        u = np.stack([s.field(k) for k in ("ux", "uy", "uz")])
+        f = s.forces(0, C3_STEPS)
+    return u, f
+
+
+@pytest.fixture(scope="module")
+def c3_reference():
+    u, f = c3_velocity_and_drag(_abi.FP64)
+    assert np.isfinite(u).all()
+    return u, f
+
+
+@pytest.mark.parametrize("precision,u_l2_bound,cd_bound", [
+    (_abi.FP32, 1e-2, 1e-3),
+    (_abi.FP32_FP16, 5e-2, 5e-3),
+    (_abi.FP32_BF16, 1e-1, 5e-3),
+])
+def test_c3_storage_precision(c3_reference, precision, u_l2_bound, cd_bound):
+    """C3 at its stated size and length: velocity L2 error ||u - u64|| / ||u64|| over the whole grid
+    and the error of the mean drag over the last 5,000 steps, against the GPU FP64 run."""
+    u64, f64 = c3_reference
+    u, f = c3_velocity_and_drag(precision)
+    d = configs.c3(precision=precision)
+    u_l2 = float(np.sqrt(np.sum((u - u64) ** 2) / np.sum(u64 ** 2)))
+    cd = drag(f, d)[0][C3_STEPS // 2:].mean()
+    cd64 = drag(f64, d)[0][C3_STEPS // 2:].mean()
+    cd_err = abs(cd - cd64) / abs(cd64)
+    print(f"[gpu configs] C3 {_abi.PRECISION_NAMES[precision]}: velocity L2 error {u_l2:.3e}, mean Cd {cd:.6f} "
+          f"(FP64 {cd64:.6f}), drag error {cd_err:.3e}", flush=True)
+    assert np.isfinite(u).all()
+    assert u_l2 < u_l2_bound
+    assert cd_err < cd_bound
+
+
+def batch_run(descs, steps, fields=False):
+    from mrlbm_3d_cuda_amd.solver import Batch
+    print(f"[gpu configs] batch of {len(descs)} {_abi.PRECISION_NAMES[descs[0].precision]} cases, {steps} steps",
+          flush=True)
+    with Batch(descs) as b:
+        assert b.admitted == len(descs)
+        b.init().step(steps, forces=True)
+        b.sync()
+        out = [(c.fields() if fields else None, c.forces(0, steps)) for c in b.cases]
+    return out
+
+
+def with_precision(descs, precision):
+    out = []
+    for d in descs:
+        d = d.copy()
+        d.precision = precision
+        out.append(d)
+    return out
+
+
+def test_c4_batched_annuli():
+    """All 64 seeded annuli, 5,000 steps, as one FP32 batch against the FP64 batch: velocity
+    within 2e-4 of the inner wall speed per case; the smallest case's FP64 batch state equals the
+    oracle's bit for bit after all 5,000 steps (force records to 1e-12 of the gross flux)."""
+    from oracle.binding import Oracle
+    steps = 5000
+    descs = configs.c4_cases()
+    r32 = batch_run(descs, steps, fields=True)
+    r64 = batch_run(with_precision(descs, _abi.FP64), steps, fields=True)
+    worst = 0.0
+    for i, d in enumerate(descs):
+        err = float(np.max(np.abs(r32[i][0][1:4] - r64[i][0][1:4]))) / d.u_wall_inner[1]
+        worst = max(worst, err)
+    print(f"[gpu configs] C4 FP32 vs FP64: worst velocity error / wall speed {worst:.3e}", flush=True)
+    assert worst < 2e-4
+    k = int(np.argmin([d.num_nodes for d in descs]))
+    d = with_precision(descs, _abi.FP64)[k]
+    print(f"[gpu configs] C4 case {k} ({d.nx}x{d.ny}x{d.nz}) oracle, {steps} steps", flush=True)
+    o = Oracle(d).init()
+    o.step(steps, _abi.STEP_FORCES)
+    assert np.array_equal(r64[k][0], o.fields())
+    # force records are FP64 sums in another order: compared against the gross boundary flux
+    assert force_error(r64[k][1], o.forces(0, steps), boundary_count(o)) <= FP64_TOL
+
+
+def test_c5_batched_channels_drag():
+    """The first 16 seeded C5 channels, 10,000 steps, as BF16, FP16 and FP32 batches against the
+    FP64 batch: mean drag over the last 5,000 steps (bounds per precision below); the 4 smallest
+    C5 channels as an FP64 batch equal the oracle bit for bit after 200 steps (force records to 1e-12
+    of the gross flux)."""
+    from oracle.binding import Oracle
+    steps = 10000
+    descs = configs.c5_cases()[:16]
+    ref = batch_run(with_precision(descs, _abi.FP64), steps)
+    cd64 = np.array([drag(f, d)[0][steps // 2:].mean() for (_, f), d in zip(ref, descs)])
+    bounds = {_abi.FP32: (1e-4, 5e-4), _abi.FP32_FP16: (5e-3, 2e-2), _abi.FP32_BF16: (5e-3, 2e-2)}
+    for prec, (mean_bound, max_bound) in bounds.items():
+        got = batch_run(with_precision(descs, prec), steps)
+        cd = np.array([drag(f, d)[0][steps // 2:].mean() for (_, f), d in zip(got, descs)])
+        err = np.abs(cd - cd64) / np.abs(cd64)
+        print(f"[gpu configs] C5 {_abi.PRECISION_NAMES[prec]}: drag error mean {err.mean():.3e} max {err.max():.3e}",
+              flush=True)
+        assert err.mean() < mean_bound and err.max() < max_bound
+    all5 = configs.c5_cases()
+    small = sorted(range(len(all5)), key=lambda i: all5[i].num_nodes)[:4]
+    sdescs = with_precision([all5[i] for i in small], _abi.FP64)
+    got = batch_run(sdescs, 200, fields=True)
+    for (F, f), d in zip(got, sdescs):
+        o = Oracle(d).init()
+        o.step(200, _abi.STEP_FORCES)
+        assert np.array_equal(F, o.fields())
+        assert force_error(f, o.forces(0, 200), boundary_count(o)) <= FP64_TOL
