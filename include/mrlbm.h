@@ -359,6 +359,20 @@ int mrlbm_batch_destroy(mrlbm_batch *batch);
 const char *mrlbm_last_error(void);
 int32_t mrlbm_abi_version(void);
 
+/*
+ * Read-back that allocates nothing. No counterpart in the reference; it is for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight monitoring loops, harnesses that
+ * forbid hipMalloc), where mrlbm_get_moments allocates a device staging array on every call.
+ * Copies field `field` of the nodes first .. first + count - 1 (x-fastest index; field 0 is
+ * rho - rho0) as doubles to host, with the values mrlbm_get_moments returns, through a fixed
+ * staging array of MRLBM_RANGE_MAX_VALUES doubles (16 MiB) per GPU that the library holds until
+ * the process ends. count above MRLBM_RANGE_MAX_VALUES returns MRLBM_ERR_INVALID (read longer
+ * spans in several calls). Calls that overlap in time on one GPU share that array and race:
+ * serialise them.
+ */
+#define MRLBM_RANGE_MAX_VALUES (1 << 21)
+int mrlbm_get_moments_range(mrlbm_sim *sim, int32_t field, int64_t first, int64_t count, double *host);
+
 #ifdef __cplusplus
 }
 #endif

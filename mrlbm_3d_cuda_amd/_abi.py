@@ -172,7 +172,10 @@ PRODUCT_PROTOTYPES = {
     "mrlbm_write_inlet_density": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int64, C.c_int32]),
     "mrlbm_last_error": (C.c_char_p, []),
     "mrlbm_abi_version": (C.c_int32, []),
+    "mrlbm_get_moments_range": (C.c_int, [C.c_void_p, C.c_int32, C.c_int64, C.c_int64, _dp]),
 }
+
+RANGE_MAX_VALUES = 1 << 21
 
 BATCH_PROTOTYPES = {
     "mrlbm_batch_plan": (C.c_int, [P(CaseDesc), C.c_int32, C.c_uint64, P(C.c_int32), P(C.c_uint64)]),

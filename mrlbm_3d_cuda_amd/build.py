@@ -28,6 +28,8 @@ SOURCES = {
     "capi.hip": [],
     "batch.hip": [],
     "output.hip": ["-ffp-contract=off"],
+    # the allocation-free read-back decodes with the FP32-arithmetic kernels' flags
+    "noalloc.hip": ["-fno-hip-fp32-correctly-rounded-divide-sqrt", "-fgpu-flush-denormals-to-zero"],
 }
 
 HIPCC_FLAGS = [

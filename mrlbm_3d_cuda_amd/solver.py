@@ -138,6 +138,14 @@ class Simulation:
     def fields(self):
         return np.stack([self.field(f) for f in range(_abi.NUM_FIELDS)])
 
+    def field_range(self, f, first, count):
+        """Field f of the nodes first .. first + count - 1 (x-fastest index) through the library's
+        fixed staging array (no device allocation; count <= _abi.RANGE_MAX_VALUES)."""
+        fi = _abi.FIELDS.index(f) if isinstance(f, str) else int(f)
+        out = np.empty(count, dtype=np.float64)
+        _check(self.lib.mrlbm_get_moments_range(self.h, fi, first, count, as_ptr(out, C.c_double)))
+        return out
+
     def set_field(self, f, values):
         fi = _abi.FIELDS.index(f) if isinstance(f, str) else int(f)
         arr = np.ascontiguousarray(values, dtype=np.float64).reshape(-1)
