@@ -228,12 +228,13 @@ def time_case(desc, warmup, steps, per_kernel_steps):
     ms_total, ms_median, replays = sim.time_replays(steps, forces=True)
     log(f"timed {steps} steps: {ms_total:.2f} ms ({replays} replays, median {ms_median * 1e3:.2f} us/step)")
     _, (ms_fused, ms_wall) = sim.time_steps(per_kernel_steps, forces=True, per_kernel=True)
+    ms_twin = sim.time_memory_twin(20)
     done = warmup + steps + per_kernel_steps
     k = min(200, done)
     forces = sim.forces(done - k, k)
     sim.close()
     return {"ms_total": ms_total, "ms_median": ms_median, "replays": replays, "ms_fused": ms_fused,
-            "ms_wall": ms_wall, "forces": forces}
+            "ms_wall": ms_wall, "ms_twin": ms_twin, "forces": forces}
 
 
 def main():
@@ -302,6 +303,7 @@ def main():
         "graph_replays": r["replays"],
         "ms_fused_kernel": round(r["ms_fused"], 5),
         "ms_wall_kernel": round(r["ms_wall"], 5),
+        "ms_memory_twin": round(r["ms_twin"], 5),
     }
     if cd is not None:
         config["cd_mean_last_records"] = float(cd.mean())
@@ -315,6 +317,7 @@ def main():
             "ratio_to_fp32": round(mlups_bf16 / mlups, 4),
             "ms_per_step_median_replay": round(rb["ms_median"], 5),
             "ms_fused_kernel": round(rb["ms_fused"], 5),
+            "ms_memory_twin": round(rb["ms_twin"], 5),
             "fused_frac_of_hbm_peak": round(20 * 2 * n / (rb["ms_fused"] / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
         }
         # the batched cylinder sweep (C5): tuned launch, cases/s at 10,000 steps per case
@@ -351,6 +354,9 @@ def main():
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic,
             "alg_bytes_per_launch": bytes_per_launch,
+            # the same loads and stores without arithmetic (memory-only twin of the fused kernel)
+            "twin_gbs": round(bytes_per_launch / (r["ms_twin"] / 1e3) / 1e9, 1),
+            "frac_of_twin": round(r["ms_twin"] / r["ms_fused"], 4),
         },
     }
     if not args.no_cpu_baseline:

@@ -207,6 +207,50 @@ int mrlbm_get_bcfluid_table(mrlbm_sim *sim, int32_t group, int64_t *node, uint32
 int mrlbm_get_info(mrlbm_sim *sim, mrlbm_info *info);
 
 /*
+ * Caller-supplied geometry <- the case module's initialize hook (struct Case, src/common/
+ * globalStructs.cuh:46-62, filled at src/cases/cylinder/cylinder_case.cu:14-26): the node types
+ * of initialize_nodeType / boundary_definitions (src/solver/initializeLBM.cuh:58-80,
+ * src/cases/cylinder/boundaries.cuh:8-94) in the type << 20 | index encoding
+ * (src/common/nodeTypeMap.cuh:82-98), the boundaryVar wall lists of cylinder_initialize
+ * (src/cases/cylinder/cylinder_structs.h:5-21, cylinder_kernels.cu:5-49) and the BCFLUID mask
+ * tables (setup_bcfluid_masks, src/cases/cylinder/cylinder_helpers.cuh:230-299).
+ *
+ * One wall group (0 inner, 1 outer): nb wall nodes; node[k] is the x-fastest index of entry k,
+ * whose node type must be (NODE_INNER + group) << 20 | k; unit normal (nx, ny) and depth delta of
+ * the wall point (xc, yc) + radius * n on the group's circle; incoming and outgoing masks (bit q);
+ * fluid_side +1 when the fluid lies outside the circle (cylinder, annulus inner wall), -1 inside
+ * (annulus outer wall); wall-frame velocity (normal, tangential, axial). The two groups may be two
+ * bodies with their own centres (the wall-pressure statistics follow group 0).
+ */
+typedef struct mrlbm_wall_desc {
+    int32_t nb;
+    int32_t reserved;
+    const int64_t *node;
+    const double *nx;
+    const double *ny;
+    const double *delta;
+    const uint32_t *in_mask;
+    const uint32_t *out_mask;
+    double xc, yc;
+    double radius;
+    double fluid_side;
+    double u_wall[3];
+} mrlbm_wall_desc;
+
+/*
+ * Replace the geometry of a simulation (from mrlbm_create or mrlbm_create_geometry_only) with
+ * node_types (nx*ny*nz values, x fastest), ngroups wall tables and, when BCFLUID nodes are
+ * present, the 256-entry per-tag BCFLUID masks. BCFLUID lists follow the node types in x-fastest
+ * order (tag = index field). Node codes and closed forms are those of mrlbm_create for the case
+ * kind (faces get closed forms in the cylinder kind only). The grid, descriptor and inlet profile
+ * are kept; a device simulation is re-initialised as by mrlbm_init. Inconsistent tables give
+ * MRLBM_ERR_GEOMETRY and leave the simulation unchanged; refused for cases owned by a batch.
+ */
+int mrlbm_set_geometry(mrlbm_sim *sim, const uint32_t *node_types, int32_t ngroups,
+                       const mrlbm_wall_desc *walls, const uint32_t *bcf_in_mask256,
+                       const uint32_t *bcf_out_mask256);
+
+/*
  * Time `nsteps` steps on device with HIP events (inputs resident in HBM). Writes the wall
  * time in ms to ms_total and, if non-null, the mean duration of one launch of each kernel
  * (fused step, curved wall) measured by per-launch events to ms_kernels[0..1].
@@ -225,6 +269,16 @@ int mrlbm_time_steps(mrlbm_sim *sim, int32_t nsteps, uint32_t flags, double *ms_
  */
 int mrlbm_time_replays(mrlbm_sim *sim, int32_t nsteps, uint32_t flags, double *ms_total,
                        double *ms_step_median, int32_t *replays);
+
+/*
+ * The fused pass's attainable memory floor on this device (no reference counterpart): a
+ * memory-only twin of the fused kernel -- the same work items, per plane the same loads of each
+ * tile and its one-node ring and the same stores of every node, without arithmetic, LDS or
+ * barriers -- launched `launches` times after one untimed launch. Writes the mean duration of one
+ * launch (ms). The simulation's state is unchanged (the twin copies it into the buffer the next
+ * step overwrites).
+ */
+int mrlbm_time_memory_twin(mrlbm_sim *sim, int32_t launches, double *ms_per_launch);
 
 /*
  * Checkpoint: a header (magic, version, precision, grid, step count, FNV-1a checksum) and the

@@ -135,6 +135,24 @@ class TuneResult(C.Structure):
     ]
 
 
+class WallDesc(C.Structure):
+    _fields_ = [
+        ("nb", C.c_int32),
+        ("reserved", C.c_int32),
+        ("node", C.POINTER(C.c_int64)),
+        ("nx", C.POINTER(C.c_double)),
+        ("ny", C.POINTER(C.c_double)),
+        ("delta", C.POINTER(C.c_double)),
+        ("in_mask", C.POINTER(C.c_uint32)),
+        ("out_mask", C.POINTER(C.c_uint32)),
+        ("xc", C.c_double),
+        ("yc", C.c_double),
+        ("radius", C.c_double),
+        ("fluid_side", C.c_double),
+        ("u_wall", C.c_double * 3),
+    ]
+
+
 P = C.POINTER
 _dp = P(C.c_double)
 _u32p = P(C.c_uint32)
@@ -154,8 +172,10 @@ PRODUCT_PROTOTYPES = {
     "mrlbm_get_wall_table": (C.c_int, [C.c_void_p, C.c_int32, _i64p, _dp, _dp, _dp, _u32p, _u32p]),
     "mrlbm_get_bcfluid_table": (C.c_int, [C.c_void_p, C.c_int32, _i64p, _u32p, _u32p, _u32p]),
     "mrlbm_get_info": (C.c_int, [C.c_void_p, P(Info)]),
+    "mrlbm_set_geometry": (C.c_int, [C.c_void_p, _u32p, C.c_int32, P(WallDesc), _u32p, _u32p]),
     "mrlbm_time_steps": (C.c_int, [C.c_void_p, C.c_int32, C.c_uint32, _dp, _dp]),
     "mrlbm_time_replays": (C.c_int, [C.c_void_p, C.c_int32, C.c_uint32, _dp, _dp, P(C.c_int32)]),
+    "mrlbm_time_memory_twin": (C.c_int, [C.c_void_p, C.c_int32, _dp]),
     "mrlbm_checkpoint_write": (C.c_int, [C.c_void_p, C.c_char_p]),
     "mrlbm_checkpoint_read": (C.c_int, [C.c_void_p, C.c_char_p]),
     "mrlbm_destroy": (C.c_int, [C.c_void_p]),

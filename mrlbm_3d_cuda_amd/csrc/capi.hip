@@ -18,8 +18,9 @@ uint64_t fnv1a(const void *data, size_t n, uint64_t h = 1469598103934665603ull) 
     return h;
 }
 
-// Version 3: BF16 payloads hold unscaled deviations.
-constexpr uint32_t CHECKPOINT_VERSION = 3;
+// Version 4: 16-bit payloads hold u_x - u_ref (u_ref in the header), and the BF16 form's velocity
+// is FP16 bits times 16.
+constexpr uint32_t CHECKPOINT_VERSION = 4;
 
 struct CheckpointHeader {
     char magic[8];  // "MRLBMCK1"
@@ -29,7 +30,8 @@ struct CheckpointHeader {
     int32_t fields;
     int64_t steps;
     uint64_t payload_bytes;
-    uint64_t checksum;  // FNV-1a over the payload
+    uint64_t checksum;     // FNV-1a over the payload
+    double velocity_ref;   // u_ref of the 16-bit forms' stored u_x - u_ref (0 otherwise)
 };
 
 }  // namespace
@@ -209,6 +211,31 @@ int mrlbm_get_bcfluid_table(mrlbm_sim *s, int32_t group, int64_t *node, uint32_t
     return MRLBM_OK;
 }
 
+int mrlbm_set_geometry(mrlbm_sim *s, const uint32_t *node_types, int32_t ngroups, const mrlbm_wall_desc *walls,
+                       const uint32_t *bcf_in256, const uint32_t *bcf_out256) {
+    SIM_CHECK(s);
+    if (s->in_batch) return set_error(MRLBM_ERR_INVALID, "case is stepped by its batch");
+    mrlbm::SimBase &sim = *s->impl;
+    mrlbm::Geometry g = sim.g;
+    std::string err;
+    if (int rc = mrlbm::geometry_from_tables(sim.d, node_types, ngroups, walls, bcf_in256, bcf_out256, g, err))
+        return set_error(rc, err);
+    if (!sim.device_ready) {
+        sim.g = std::move(g);
+        return MRLBM_OK;
+    }
+    mrlbm::Geometry old = std::move(sim.g);
+    sim.g = std::move(g);
+    if (int rc = sim.apply_geometry()) {
+        // the tables of the new geometry failed (a wall stencil outside the grid): restore the old
+        const std::string msg = mrlbm::g_last_error;
+        sim.g = std::move(old);
+        (void)sim.apply_geometry();
+        return set_error(rc, msg);
+    }
+    return MRLBM_OK;
+}
+
 int mrlbm_get_info(mrlbm_sim *s, mrlbm_info *info) {
     SIM_CHECK(s);
     if (!info) return set_error(MRLBM_ERR_INVALID, "null output");
@@ -244,6 +271,12 @@ int mrlbm_time_replays(mrlbm_sim *s, int32_t nsteps, uint32_t flags, double *ms_
     return rc;
 }
 
+int mrlbm_time_memory_twin(mrlbm_sim *s, int32_t launches, double *ms_per_launch) {
+    OWN_CHECK(s);
+    if (!ms_per_launch) return set_error(MRLBM_ERR_INVALID, "null output");
+    return s->impl->time_memory_twin(launches, ms_per_launch);
+}
+
 int mrlbm_checkpoint_write(mrlbm_sim *s, const char *path) {
     DEV_CHECK(s);
     if (!path) return set_error(MRLBM_ERR_INVALID, "null path");
@@ -263,6 +296,7 @@ int mrlbm_checkpoint_write(mrlbm_sim *s, const char *path) {
     h.steps = s->impl->steps;
     h.payload_bytes = payload.size();
     h.checksum = fnv1a(payload.data(), h.payload_bytes);
+    h.velocity_ref = s->impl->velocity_reference();
     FILE *fp = std::fopen(path, "wb");
     if (!fp) return set_error(MRLBM_ERR_IO, std::string("cannot open ") + path);
     bool ok = std::fwrite(&h, sizeof(h), 1, fp) == 1 &&
@@ -282,7 +316,8 @@ int mrlbm_checkpoint_read(mrlbm_sim *s, const char *path) {
     const bool head = std::fread(&h, sizeof(h), 1, fp) == 1;
     bool ok = head && std::memcmp(h.magic, "MRLBMCK1", 8) == 0 && h.version == CHECKPOINT_VERSION &&
               h.precision == uint32_t(s->impl->d.precision) && h.nx == g.nx && h.ny == g.ny && h.nz == g.nz &&
-              h.fields == 10 && h.payload_bytes == payload.size();
+              h.fields == 10 && h.payload_bytes == payload.size() &&
+              h.velocity_ref == s->impl->velocity_reference();
     if (ok) ok = std::fread(payload.data(), 1, h.payload_bytes, fp) == h.payload_bytes;
     if (ok) {
         char extra;
@@ -481,7 +516,7 @@ int mrlbm_write_pressure(mrlbm_sim *s, const char *dir) {
     for (size_t k = 0; k < w.node.size() && ok; ++k) {
         const int64_t i = w.node[k];
         const unsigned x = unsigned(i % g.nx), y = unsigned((i / g.nx) % g.ny), z = unsigned(i / (int64_t(g.nx) * g.ny));
-        const double dx = double(x) - d.xc, dy = double(y) - d.yc;
+        const double dx = double(x) - w.xc, dy = double(y) - w.yc;
         double theta = std::atan2(dy, dx) * 180.0 / pi;
         if (theta < 0) theta += 360.0;
         ok = std::fprintf(fp, "%.12g %u %.12g\n", theta, z, ps[k]) > 0;

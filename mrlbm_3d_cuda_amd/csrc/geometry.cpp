@@ -110,6 +110,97 @@ double hash_uniform(uint64_t seed, uint64_t counter) {
     return double(mix64(seed ^ mix64(counter)) >> 11) * 0x1.0p-53;
 }
 
+// Reference tile activity and the per-node info words of the fused kernel, from the node types,
+// indices and wall lists of g.
+void finish_geometry(const mrlbm_case_desc &d, Geometry &g) {
+    const int NX = g.nx, NY = g.ny, NZ = g.nz;
+    // reference tile activity: 8x8x4 tiles, active if any non-SOLID node; the kernels index the
+    // mask without blockIdx.z (mlbm.cu:87), so the bz = 0 tile decides its whole column
+    const int GX = (NX + 7) / 8, GY = (NY + 7) / 8, GZ = (NZ + 3) / 4;
+    std::vector<uint8_t> col_active(size_t(GX) * GY, 0);
+    for (int y = 0; y < NY; ++y)
+        for (int x = 0; x < NX; ++x)
+            for (int z = 0; z < std::min(4, NZ); ++z)
+                if (g.ref_type[g.idx(x, y, z)] != RT_SOLID) col_active[size_t(x / 8) + size_t(GX) * (y / 8)] = 1;
+    g.total_tiles = GX * GY * GZ;
+    g.active_tiles = 0;
+    for (uint8_t a : col_active) g.active_tiles += int(a) * GZ;
+
+    // per-node info words for the fused kernel
+    g.info.assign(g.n, NC_BULK);
+    for (int z = 0; z < NZ; ++z)
+        for (int y = 0; y < NY; ++y)
+            for (int x = 0; x < NX; ++x) {
+                const int64_t i = g.idx(x, y, z);
+                if (!col_active[size_t(x / 8) + size_t(GX) * (y / 8)]) {
+                    g.info[i] = NC_FROZEN;
+                    continue;
+                }
+                uint32_t code = NC_KEEP, payload = 0;
+                const uint8_t t = g.ref_type[i];
+                switch (t) {
+                case RT_BULK: code = NC_BULK; break;
+                case RT_INNER: code = NC_WALL; payload = g.ref_index[i]; break;
+                case RT_OUTER: code = NC_WALL; payload = uint32_t(g.wall[0].node.size()) + g.ref_index[i]; break;
+                case RT_BCSOLID_INNER:
+                case RT_BCSOLID_OUTER: {
+                    const uint32_t tg8 = g.ref_index[i];
+                    if (tg8 == 7 || tg8 == 11 || tg8 == 13 || tg8 == 14 || tg8 == 112 || tg8 == 176 || tg8 == 208 ||
+                        tg8 == 224) {
+                        code = NC_BCS;
+                        payload = tg8;
+                    }
+                    break;
+                }
+                case RT_BCFLUID_INNER:
+                case RT_BCFLUID_OUTER: {
+                    const uint32_t tg8 = g.ref_index[i];
+                    if (tg8 == 17 || tg8 == 34 || tg8 == 68 || tg8 == 136) {
+                        code = NC_BCF;
+                        payload = tg8;
+                    }
+                    break;
+                }
+                default:
+                    // domain faces have closed forms only in the cylinder case; the annulus
+                    // module leaves them unchanged (annulus_kernels.cuh:54-57)
+                    if (d.kind == MRLBM_CASE_CYLINDER) {
+                        switch (t) {
+                        case RT_WEST: code = NC_WEST; break;
+                        case RT_EAST: code = NC_EAST; break;
+                        case RT_NORTH: code = NC_NORTH; break;
+                        case RT_SOUTH: code = NC_SOUTH; break;
+                        case RT_NORTH_WEST: code = NC_NORTH_WEST; break;
+                        case RT_SOUTH_WEST: code = NC_SOUTH_WEST; break;
+                        case RT_NORTH_EAST: code = NC_NORTH_EAST; break;
+                        case RT_SOUTH_EAST: code = NC_SOUTH_EAST; break;
+                        case RT_FRONT: code = NC_FRONT; break;
+                        case RT_BACK: code = NC_BACK; break;
+                        case RT_NORTH_FRONT: code = NC_NORTH_FRONT; break;
+                        case RT_NORTH_BACK: code = NC_NORTH_BACK; break;
+                        case RT_SOUTH_FRONT: code = NC_SOUTH_FRONT; break;
+                        case RT_SOUTH_BACK: code = NC_SOUTH_BACK; break;
+                        case RT_WEST_FRONT: code = NC_WEST_FRONT; break;
+                        case RT_WEST_BACK: code = NC_WEST_BACK; break;
+                        case RT_EAST_FRONT: code = NC_EAST_FRONT; break;
+                        case RT_EAST_BACK: code = NC_EAST_BACK; break;
+                        case RT_NWF: code = NC_WEST_CORNER; payload = 0; break;
+                        case RT_NWB: code = NC_WEST_CORNER; payload = 1; break;
+                        case RT_SWF: code = NC_WEST_CORNER; payload = 2; break;
+                        case RT_SWB: code = NC_WEST_CORNER; payload = 3; break;
+                        case RT_NEF:
+                        case RT_NEB:
+                        case RT_SEF:
+                        case RT_SEB: code = NC_EAST_CORNER; break;
+                        default: break;
+                        }
+                    }
+                    break;
+                }
+                g.info[i] = code | (payload << INFO_PAYLOAD_SHIFT);
+            }
+}
+
 int build_geometry(const mrlbm_case_desc &d, Geometry &g, std::string &err) {
     if (d.nx < 3 || d.ny < 3 || d.nz < 1 || int64_t(d.nx) * d.ny * d.nz > (int64_t(1) << 31)) {
         err = "grid size out of range";
@@ -210,6 +301,10 @@ int build_geometry(const mrlbm_case_desc &d, Geometry &g, std::string &err) {
                     }
         g.wall[0].radius = r_in;
         g.wall[1].radius = r_out;
+        for (int gi = 0; gi < 2; ++gi) {
+            g.wall[gi].xc = d.xc;
+            g.wall[gi].yc = d.yc;
+        }
         g.wall[1].fluid_dir = -1.0;
         for (int k = 0; k < 3; ++k) {
             g.wall[0].uw[k] = d.u_wall_inner[k];
@@ -311,91 +406,100 @@ int build_geometry(const mrlbm_case_desc &d, Geometry &g, std::string &err) {
         for (int y = 0; y < NY; ++y)
             for (int x = 0; x < NX; ++x) g.ref_type[g.idx(x, y, z)] = tg.at(x, y, z);
 
-    // reference tile activity: 8x8x4 tiles, active if any non-SOLID node; the kernels index the
-    // mask without blockIdx.z (mlbm.cu:87), so the bz = 0 tile decides its whole column
-    const int GX = (NX + 7) / 8, GY = (NY + 7) / 8, GZ = (NZ + 3) / 4;
-    std::vector<uint8_t> col_active(size_t(GX) * GY, 0);
-    for (int y = 0; y < NY; ++y)
-        for (int x = 0; x < NX; ++x)
-            for (int z = 0; z < std::min(4, NZ); ++z)
-                if (g.ref_type[g.idx(x, y, z)] != RT_SOLID) col_active[size_t(x / 8) + size_t(GX) * (y / 8)] = 1;
-    g.total_tiles = GX * GY * GZ;
-    g.active_tiles = 0;
-    for (uint8_t a : col_active) g.active_tiles += int(a) * GZ;
+    finish_geometry(d, g);
+    return MRLBM_OK;
+}
 
-    // per-node info words for the fused kernel
-    g.info.assign(g.n, NC_BULK);
-    for (int z = 0; z < NZ; ++z)
-        for (int y = 0; y < NY; ++y)
-            for (int x = 0; x < NX; ++x) {
-                const int64_t i = g.idx(x, y, z);
-                if (!col_active[size_t(x / 8) + size_t(GX) * (y / 8)]) {
-                    g.info[i] = NC_FROZEN;
-                    continue;
-                }
-                uint32_t code = NC_KEEP, payload = 0;
-                const uint8_t t = g.ref_type[i];
-                switch (t) {
-                case RT_BULK: code = NC_BULK; break;
-                case RT_INNER: code = NC_WALL; payload = g.ref_index[i]; break;
-                case RT_OUTER: code = NC_WALL; payload = uint32_t(g.wall[0].node.size()) + g.ref_index[i]; break;
-                case RT_BCSOLID_INNER:
-                case RT_BCSOLID_OUTER: {
-                    const uint32_t tg8 = g.ref_index[i];
-                    if (tg8 == 7 || tg8 == 11 || tg8 == 13 || tg8 == 14 || tg8 == 112 || tg8 == 176 || tg8 == 208 ||
-                        tg8 == 224) {
-                        code = NC_BCS;
-                        payload = tg8;
-                    }
-                    break;
-                }
-                case RT_BCFLUID_INNER:
-                case RT_BCFLUID_OUTER: {
-                    const uint32_t tg8 = g.ref_index[i];
-                    if (tg8 == 17 || tg8 == 34 || tg8 == 68 || tg8 == 136) {
-                        code = NC_BCF;
-                        payload = tg8;
-                    }
-                    break;
-                }
-                default:
-                    // domain faces have closed forms only in the cylinder case; the annulus
-                    // module leaves them unchanged (annulus_kernels.cuh:54-57)
-                    if (d.kind == MRLBM_CASE_CYLINDER) {
-                        switch (t) {
-                        case RT_WEST: code = NC_WEST; break;
-                        case RT_EAST: code = NC_EAST; break;
-                        case RT_NORTH: code = NC_NORTH; break;
-                        case RT_SOUTH: code = NC_SOUTH; break;
-                        case RT_NORTH_WEST: code = NC_NORTH_WEST; break;
-                        case RT_SOUTH_WEST: code = NC_SOUTH_WEST; break;
-                        case RT_NORTH_EAST: code = NC_NORTH_EAST; break;
-                        case RT_SOUTH_EAST: code = NC_SOUTH_EAST; break;
-                        case RT_FRONT: code = NC_FRONT; break;
-                        case RT_BACK: code = NC_BACK; break;
-                        case RT_NORTH_FRONT: code = NC_NORTH_FRONT; break;
-                        case RT_NORTH_BACK: code = NC_NORTH_BACK; break;
-                        case RT_SOUTH_FRONT: code = NC_SOUTH_FRONT; break;
-                        case RT_SOUTH_BACK: code = NC_SOUTH_BACK; break;
-                        case RT_WEST_FRONT: code = NC_WEST_FRONT; break;
-                        case RT_WEST_BACK: code = NC_WEST_BACK; break;
-                        case RT_EAST_FRONT: code = NC_EAST_FRONT; break;
-                        case RT_EAST_BACK: code = NC_EAST_BACK; break;
-                        case RT_NWF: code = NC_WEST_CORNER; payload = 0; break;
-                        case RT_NWB: code = NC_WEST_CORNER; payload = 1; break;
-                        case RT_SWF: code = NC_WEST_CORNER; payload = 2; break;
-                        case RT_SWB: code = NC_WEST_CORNER; payload = 3; break;
-                        case RT_NEF:
-                        case RT_NEB:
-                        case RT_SEF:
-                        case RT_SEB: code = NC_EAST_CORNER; break;
-                        default: break;
-                        }
-                    }
-                    break;
-                }
-                g.info[i] = code | (payload << INFO_PAYLOAD_SHIFT);
+int geometry_from_tables(const mrlbm_case_desc &d, const uint32_t *types, int ngroups, const mrlbm_wall_desc *walls,
+                         const uint32_t *bcf_in256, const uint32_t *bcf_out256, Geometry &g, std::string &err) {
+    if (!types || ngroups < 0 || ngroups > 2 || (ngroups > 0 && !walls)) {
+        err = "node types missing or bad wall group count";
+        return MRLBM_ERR_INVALID;
+    }
+    const int64_t n = g.n;
+    std::vector<uint8_t> rt(n);
+    std::vector<uint32_t> ri(n);
+    int64_t count[2] = {0, 0};
+    bool has_bcf = false;
+    for (int64_t i = 0; i < n; ++i) {
+        const uint32_t t = types[i] >> 20, k = types[i] & 0xFFFFFu;
+        if (t > RT_SEB) {
+            err = "node " + std::to_string(i) + ": unknown node type " + std::to_string(t);
+            return MRLBM_ERR_GEOMETRY;
+        }
+        rt[i] = uint8_t(t);
+        ri[i] = k;
+        if (t == RT_INNER || t == RT_OUTER) {
+            const int gi = t == RT_OUTER;
+            if (gi >= ngroups) {
+                err = "wall node of a group without a wall table";
+                return MRLBM_ERR_GEOMETRY;
             }
+            ++count[gi];
+        }
+        if ((t == RT_BCFLUID_INNER || t == RT_BCFLUID_OUTER || t == RT_BCSOLID_INNER || t == RT_BCSOLID_OUTER) &&
+            k > 255) {
+            err = "BCFLUID / BCSOLID tag above 255";
+            return MRLBM_ERR_GEOMETRY;
+        }
+        if (t == RT_BCFLUID_INNER || t == RT_BCFLUID_OUTER) {
+            has_bcf = true;
+            if ((t == RT_BCFLUID_OUTER) >= ngroups) {
+                err = "BCFLUID node of a group without a wall table";
+                return MRLBM_ERR_GEOMETRY;
+            }
+        }
+    }
+    if (has_bcf && (!bcf_in256 || !bcf_out256)) {
+        err = "BCFLUID nodes need the per-tag mask tables";
+        return MRLBM_ERR_INVALID;
+    }
+    WallTable wt[2];
+    for (int gi = 0; gi < ngroups; ++gi) {
+        const mrlbm_wall_desc &w = walls[gi];
+        if (w.nb < 0 || int64_t(w.nb) != count[gi] ||
+            (w.nb > 0 && (!w.node || !w.nx || !w.ny || !w.delta || !w.in_mask || !w.out_mask)) ||
+            !(w.radius > 0.0) || (w.fluid_side != 1.0 && w.fluid_side != -1.0)) {
+            err = "wall table " + std::to_string(gi) + ": entry count, arrays, radius or fluid side do not match";
+            return MRLBM_ERR_GEOMETRY;
+        }
+        WallTable &o = wt[gi];
+        for (int32_t k = 0; k < w.nb; ++k) {
+            const int64_t i = w.node[k];
+            if (i < 0 || i >= n || rt[i] != RT_INNER + gi || ri[i] != uint32_t(k)) {
+                err = "wall table " + std::to_string(gi) + " entry " + std::to_string(k) +
+                      ": its node's type is not (wall << 20 | entry)";
+                return MRLBM_ERR_GEOMETRY;
+            }
+            o.node.push_back(i);
+            o.nx.push_back(w.nx[k]);
+            o.ny.push_back(w.ny[k]);
+            o.delta.push_back(w.delta[k]);
+            o.in_mask.push_back(w.in_mask[k]);
+            o.out_mask.push_back(w.out_mask[k]);
+        }
+        o.radius = w.radius;
+        o.xc = w.xc;
+        o.yc = w.yc;
+        o.fluid_dir = w.fluid_side;
+        for (int c = 0; c < 3; ++c) o.uw[c] = w.u_wall[c];
+    }
+    // BCFLUID lists in x-fastest order
+    for (int64_t i = 0; i < n; ++i)
+        if (rt[i] == RT_BCFLUID_INNER || rt[i] == RT_BCFLUID_OUTER) {
+            WallTable &o = wt[rt[i] == RT_BCFLUID_OUTER];
+            o.bcf_node.push_back(i);
+            o.bcf_tag.push_back(ri[i]);
+        }
+    g.ref_type = std::move(rt);
+    g.ref_index = std::move(ri);
+    g.ngroups = ngroups;
+    for (int gi = 0; gi < 2; ++gi) g.wall[gi] = std::move(wt[gi]);
+    for (int t = 0; t < 256; ++t) {
+        g.bcf_in[t] = bcf_in256 ? bcf_in256[t] : 0u;
+        g.bcf_out[t] = bcf_out256 ? bcf_out256[t] : 0u;
+    }
+    finish_geometry(d, g);
     return MRLBM_OK;
 }
 

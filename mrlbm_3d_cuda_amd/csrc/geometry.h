@@ -65,6 +65,7 @@ struct WallTable {
     std::vector<int64_t> bcf_node;
     std::vector<uint32_t> bcf_tag;
     double radius = 0.0;
+    double xc = 0.0, yc = 0.0;  // centre of the wall circle
     double fluid_dir = 1.0;  // +1 inner wall (fluid outside), -1 outer wall (curvedLBM.cu:72-76)
     double uw[3] = {0.0, 0.0, 0.0};
 };
@@ -90,5 +91,15 @@ double hash_uniform(uint64_t seed, uint64_t counter);
 
 // Validates the descriptor and builds the geometry. Returns an mrlbm_status; err gets a message.
 int build_geometry(const mrlbm_case_desc &d, Geometry &g, std::string &err);
+
+// Reference tile activity and the fused kernel's per-node info words from g's node types,
+// indices and wall lists.
+void finish_geometry(const mrlbm_case_desc &d, Geometry &g);
+
+// Replaces g's node types, wall and BCFLUID tables with caller-supplied ones in the reference's
+// encoding (mrlbm_set_geometry); the grid and the inlet profile of g are kept. On error g is
+// unchanged.
+int geometry_from_tables(const mrlbm_case_desc &d, const uint32_t *types, int ngroups, const mrlbm_wall_desc *walls,
+                         const uint32_t *bcf_in256, const uint32_t *bcf_out256, Geometry &g, std::string &err);
 
 }  // namespace mrlbm

@@ -149,6 +149,11 @@ void launch_fused(const FusedArgs<R, S> &a, int bx, int by, int variant, int par
 template <class R, class S>
 bool fused_part_supported(int bx, int by, int variant);
 
+// Memory-only twin of the fused pass over the same work items (no arithmetic): the access
+// pattern's floor on this device. Copies src into dst.
+template <class R, class S>
+void launch_memory_twin(const FusedArgs<R, S> &a, int bx, int by, hipStream_t stream);
+
 // Gives the fused kernel of this configuration the dynamic LDS it needs (once per kernel, before
 // any launch or graph capture); 0 on success, -1 for a configuration that is not instantiated.
 template <class R, class S>
@@ -223,7 +228,7 @@ struct LaunchConfig {
 // Tile shapes (BX x BY) the fused kernel is instantiated for.
 inline bool fused_tile_supported(int bx, int by) {
     return (bx == 32 && by == 8) || (bx == 16 && by == 8) || (bx == 32 && by == 4) || (bx == 16 && by == 16) ||
-           (bx == 64 && by == 4) || (bx == 16 && by == 4);
+           (bx == 64 && by == 4) || (bx == 16 && by == 4) || (bx == 32 && by == 16);
 }
 
 // Bytes of dynamic LDS one fused block uses over the tile and its ring: three planes of the

@@ -218,17 +218,14 @@ __device__ __forceinline__ void boundary_node(const FusedArgs<R, S> &a, R *f, co
 template <class R, class S, int BX, int BY>
 struct BoundaryPath {
     static constexpr int W = BX + 2;
-    S own_cur[10], own_nx[10];
+    RawNode<S> own_cur, own_nx;
     uint32_t info_cur = 0, info_nx = 0;
     NodeTabs<R> tab;
     R Mp[10], Mdp[10];
     uint32_t idxp, didxp;
 
     __device__ void load_info(const FusedArgs<R, S> &a, uint32_t node, uint32_t &out) { out = a.info[node]; }
-    __device__ void copy_own(const S *raw, S *dst) {
-#pragma unroll
-        for (int k = 0; k < 10; ++k) dst[k] = raw[k];
-    }
+    __device__ void copy_own(const RawNode<S> &raw, RawNode<S> &dst) { dst = raw; }
     // The donor snapshot is encoded only in waves that hold a donor node; the others still issue
     // its stores (into the padding slot) so that every path keeps the same memory operations.
     __device__ void store_prev(const FusedArgs<R, S> &a, uint32_t seed) {
@@ -310,14 +307,14 @@ __device__ __forceinline__ void fused_march(const FusedArgs<R, S> &a, R *lds, in
                                : off0;
     const bool west1 = has1 && hx1 == 0 && hy1 >= 1 && hy1 <= BY;  // west ring column
 
-    S m0[10], m1[10];  // stored values of the plane in flight, per position
+    RawNode<S> m0, m1;  // stored values of the plane in flight, per position
     auto load_plane = [&](int z) {
         const uint32_t base = uint32_t(pmod(z, a.nz)) * plane;
         load_raw(a.src, base + off0, m0);
         load_raw(a.src, base + off1, m1);
     };
     // write the populations of a loaded plane (index zp) into the requested group slots
-    auto put = [&](const S *raw, int p, int sp, int s0, bool want0, bool wantm, bool west, int zp) {
+    auto put = [&](const RawNode<S> &raw, int p, int sp, int s0, bool want0, bool wantm, bool west, int zp) {
         R m[10], f[Q];
         decode_node<R, S>(raw, a.scale, a.rho0, m);
         if (SPECIAL && west) {
@@ -462,15 +459,15 @@ __device__ __forceinline__ void window_march(const FusedArgs<R, S> &a, R *lds, i
 
     // stored values of the plane in flight, per position (set B only holds plane z0 in the
     // prologue, which loads it together with plane z0-1)
-    S ra0[10], ra1[10], rb0[10], rb1[10];
-    auto load_plane_to = [&](int z, S *d0, S *d1) {
+    RawNode<S> ra0, ra1, rb0, rb1;
+    auto load_plane_to = [&](int z, RawNode<S> &d0, RawNode<S> &d1) {
         const uint32_t base = uint32_t(pmod(z, a.nz)) * plane;
         load_raw(a.src, base + off0, d0);
         load_raw(a.src, base + off1, d1);
     };
     // windows, per position: c_z=+1 groups of two planes (A, B), c_z=0 group of one plane
     R pa0[9], pb0[9], q0[9], pa1[9], pb1[9], q1[9];
-    auto recon = [&](const S *raw, R *f, bool west, int zp) {
+    auto recon = [&](const RawNode<S> &raw, R *f, bool west, int zp) {
         R m[10];
         decode_node<R, S>(raw, a.scale, a.rho0, m);
         if (SPECIAL && west) {
@@ -528,7 +525,7 @@ __device__ __forceinline__ void window_march(const FusedArgs<R, S> &a, R *lds, i
     uint32_t idxp = a.pad;
 
     // one position: reconstruct plane z+1, write the pulled plane's groups, advance the windows
-    auto put = [&](const S *raw, int p, R *G, R *pold, R *q, bool west, int zp) {
+    auto put = [&](const RawNode<S> &raw, int p, R *G, R *pold, R *q, bool west, int zp) {
         R f[Q];
         recon(raw, f, west, zp);
 #pragma unroll
@@ -594,9 +591,10 @@ __device__ __forceinline__ void window_march(const FusedArgs<R, S> &a, R *lds, i
 // PART: which launch of the fused pass (PART_ALL, PART_INTERIOR, PART_WALL_SIDE). The interior
 // launch has no boundary tile, so the boundary path is not compiled into it; the two parts are
 // separate instantiations so that profiles tell the launches apart.
-// One LDS plane leaves room for three blocks per CU: registers are capped to match.
+// One LDS plane leaves room for three blocks per CU, two planes for two: registers are capped to
+// match (blocks of 512 threads: one per CU).
 template <class R, class S, int BX, int BY, int NBUF, int PART>
-__global__ __launch_bounds__(BX *BY, (NBUF == 1 ? 3 : 2)) void fused_window_kernel(FusedArgs<R, S> a) {
+__global__ __launch_bounds__(BX *BY, (BX * BY > 256 ? 1 : (NBUF == 1 ? 3 : 2))) void fused_window_kernel(FusedArgs<R, S> a) {
     extern __shared__ unsigned char lds_raw[];
     R *lds = reinterpret_cast<R *>(lds_raw);
     int b = blockIdx.x;
@@ -639,6 +637,84 @@ __global__ __launch_bounds__(BX *BY) void fused_step_ring6_kernel(FusedArgs<R, S
         fused_march<R, S, BX, BY, true>(a, lds, tile, z0, z1);
     else
         fused_march<R, S, BX, BY, false>(a, lds, tile, z0, z1);
+}
+
+// The stored words of a node folded into one word (keeps every loaded word alive).
+template <class T>
+__device__ __forceinline__ uint32_t fold_word(const T &v) {
+    struct Words {
+        uint32_t w[sizeof(T) / 4];
+    };
+    const Words x = __builtin_bit_cast(Words, v);
+    uint32_t h = 0;
+#pragma unroll
+    for (size_t k = 0; k < sizeof(T) / 4; ++k) h ^= x.w[k];
+    return h;
+}
+
+template <class S>
+__device__ __forceinline__ uint32_t fold_bits(const RawNode<S> &r) {
+    if constexpr (kQuads<S>) {
+        return fold_word(r.q0) ^ fold_word(r.q1) ^ fold_word(r.p);
+    } else {
+        uint32_t h = 0;
+#pragma unroll
+        for (int j = 0; j < 5; ++j) h ^= fold_word(r.p[j]);
+        return h;
+    }
+}
+
+// Memory-only twin of the fused pass: the same work items and XCD remap, and per plane the same
+// loads of the tile and its one-node ring (planes z0 - 1 .. z1, the next plane in flight while a
+// plane is stored) and the same store of every tile node, without arithmetic, LDS or barriers.
+// Its duration is the floor that this access pattern sets on this device, the ceiling the fused
+// pass is measured against beside HBM peak. It copies the state into the other buffer (which the
+// next step overwrites; frozen nodes hold the same value in both), so it can run on a live
+// simulation. The ring's values are folded into the stored bits under a mask that is zero at run
+// time (a.nx > 0), so no load is dropped.
+template <class R, class S, int BX, int BY>
+__global__ __launch_bounds__(BX *BY) void memory_twin_kernel(FusedArgs<R, S> a) {
+    int b = blockIdx.x;
+    if (b >= a.nblocks) return;
+    if (a.xcd_groups > 1) b = (b % a.xcd_groups) * (a.nblocks / a.xcd_groups) + b / a.xcd_groups;
+    const uint2 item = a.items[b];
+    const int tile = int(item.x & 0x7FFFFFFFu);
+    const int z0 = int(item.y & 0xFFFFu);
+    const int z1 = int(item.y >> 16);
+    if (z0 >= z1) return;
+    using F = Frame<BX, BY>;
+    const int x0 = (tile % a.tiles_x) * BX;
+    const int y0 = (tile / a.tiles_x) * BY;
+    const int tid = threadIdx.x;
+    const int tx = tid % BX, ty = tid / BX;
+    const uint32_t plane = uint32_t(a.nx) * uint32_t(a.ny);
+    const bool has1 = tid < F::NR;
+    int hx1 = 0, hy1 = 0;
+    if (has1) F::ring(tid, hx1, hy1);
+    const uint32_t off0 = uint32_t(pmod(x0 + tx, a.nx)) + uint32_t(a.nx) * uint32_t(pmod(y0 + ty, a.ny));
+    const uint32_t off1 = has1 ? uint32_t(pmod(x0 - 1 + hx1, a.nx)) + uint32_t(a.nx) * uint32_t(pmod(y0 - 1 + hy1, a.ny))
+                               : off0;
+    const bool inside = x0 + tx < a.nx && y0 + ty < a.ny;
+    const uint32_t keep = a.nx > 0 ? 0u : ~0u;
+    RawNode<S> c0, c1, n0, n1;
+    auto load_plane_to = [&](int z, RawNode<S> &d0, RawNode<S> &d1) {
+        const uint32_t base = uint32_t(pmod(z, a.nz)) * plane;
+        load_raw(a.src, base + off0, d0);
+        load_raw(a.src, base + off1, d1);
+    };
+    load_plane_to(z0 - 1, c0, c1);
+    uint32_t fold = fold_bits(c0) ^ fold_bits(c1);
+    load_plane_to(z0, c0, c1);
+    for (int z = z0; z < z1; ++z) {
+        load_plane_to(z + 1, n0, n1);  // in flight while plane z is stored
+        fold ^= fold_bits(c1);
+        RawNode<S> v = c0;
+        if (fold & keep) v = c1;  // never at run time
+        store_raw<S>(a.dst, inside ? off0 + uint32_t(z) * plane : a.pad, v);
+        c0 = n0;
+        c1 = n1;
+    }
+    if ((fold ^ fold_bits(c0) ^ fold_bits(c1)) & keep) store_raw<S>(a.dst, a.pad, c0);
 }
 
 // Fused pass of a batch: work item {case, tile | special << 31, z0 | z1 << 16, 0}; the case's
@@ -694,11 +770,9 @@ __device__ __forceinline__ void interp_rotate(const WallArgs<R, S> &a, uint32_t 
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
 #pragma unroll
-        for (int k = 0; k < 9; ++k) {
-            q[j][k] = R(Conv<S>::to(a.donor[size_t(corner[j]) * 10 + 1 + k]));
-            if constexpr (!Conv<S>::unit_scale) q[j][k] *= a.scale.dec[field_class(k + 1)];
-        }
+        for (int k = 0; k < 9; ++k) q[j][k] = stored_value<R, S>(a.donor[size_t(corner[j]) * 10 + 1 + k], k + 1, a.scale);
         if constexpr (Conv<S>::scaled) {
+            q[j][0] += a.scale.uref;
             // stored second moments are non-equilibrium parts: m = (m - u u) + u u is restored at
             // each corner before the corners are blended
             q[j][3] += q[j][0] * q[j][0];
@@ -947,6 +1021,7 @@ auto fused_kernel_for(int bx, int by, int variant, int part) -> void (*)(FusedAr
     MRLBM_FUSED_TILE(16, 16, PART_ALL)
     MRLBM_FUSED_TILE(64, 4, PART_ALL)
     MRLBM_FUSED_TILE(16, 4, PART_ALL)
+    MRLBM_FUSED_TILE(32, 16, PART_ALL)
 #undef MRLBM_FUSED_TILE
     // the two-stream schedule's launches: tiles 32x8 and 16x8, ring or two-plane window march
 #define MRLBM_FUSED_PART(TX, TY, P)                                                  \
@@ -973,6 +1048,23 @@ int prepare_fused(int bx, int by, int variant, int part) {
                                int(lds)) == hipSuccess
                ? 0
                : -1;
+}
+
+template <class R, class S>
+void launch_memory_twin(const FusedArgs<R, S> &a, int bx, int by, hipStream_t stream) {
+#define MRLBM_TWIN_TILE(TX, TY)                                                                        \
+    if (bx == TX && by == TY) {                                                                        \
+        hipLaunchKernelGGL((memory_twin_kernel<R, S, TX, TY>), dim3(a.nblocks), dim3(TX * TY), 0, stream, a); \
+        return;                                                                                        \
+    }
+    MRLBM_TWIN_TILE(32, 8)
+    MRLBM_TWIN_TILE(16, 8)
+    MRLBM_TWIN_TILE(32, 4)
+    MRLBM_TWIN_TILE(16, 16)
+    MRLBM_TWIN_TILE(64, 4)
+    MRLBM_TWIN_TILE(16, 4)
+    MRLBM_TWIN_TILE(32, 16)
+#undef MRLBM_TWIN_TILE
 }
 
 // The kernel's dynamic LDS limit is set by prepare_fused when the configuration is chosen.
@@ -1058,7 +1150,8 @@ __global__ __launch_bounds__(256) void plane_kernel(PlaneArgs<R, S> a) {
         double u = double(Conv<S>::to(ld_field(a.mom, 1, row + uint32_t(a.nx - 2))));
         if constexpr (Conv<S>::scaled) {
             r *= double(a.scale.dec[0]);
-            u *= double(a.scale.dec[1]);
+            u = double(stored_value<R, S>(ld_field(a.mom, 1, row + uint32_t(a.nx - 2)), 1, a.scale)) +
+                double(a.scale.uref);
         }
         sr += a.rho0 + r;
         su += u;
@@ -1100,6 +1193,7 @@ void launch_decode(const FieldSet<const S> &in, double *out, uint32_t n, const S
 #define MRLBM_INSTANTIATE(R, S)                                                                             \
     template void launch_fused<R, S>(const FusedArgs<R, S> &, int, int, int, int, hipStream_t);             \
     template int prepare_fused<R, S>(int, int, int, int);                                                   \
+    template void launch_memory_twin<R, S>(const FusedArgs<R, S> &, int, int, hipStream_t);                 \
     template bool fused_part_supported<R, S>(int, int, int);                                                \
     template void launch_wall<R, S>(const WallArgs<R, S> &, hipStream_t);                                   \
     template void launch_pressure<R, S>(const PressureArgs<R, S> &, hipStream_t);                           \

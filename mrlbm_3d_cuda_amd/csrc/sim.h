@@ -39,6 +39,13 @@ inline int set_error(int code, const std::string &msg) {
 
 namespace mrlbm {
 
+// Reference velocity (x component) of a case: the mean inflow of the cylinder channel (u_max for
+// a uniform inlet, 2/3 u_max for the parabolic one), 0 otherwise.
+inline double reference_velocity(const mrlbm_case_desc &d) {
+    if (d.kind != MRLBM_CASE_CYLINDER) return 0.0;
+    return d.inlet_profile == MRLBM_INLET_PARABOLIC ? d.u_max * (2.0 / 3.0) : d.u_max;
+}
+
 class SimBase {
   public:
     virtual ~SimBase() = default;
@@ -69,6 +76,13 @@ class SimBase {
     // NaN/Inf watchdog: raised by the fused pass when a node's density is not finite; cleared when
     // a new state is set (init, set_moments, checkpoint reads).
     virtual bool nonfinite() const = 0;
+    // u_ref of the 16-bit forms' stored u_x - u_ref (0 for FP64 and FP32)
+    virtual double velocity_reference() const = 0;
+    // Device tables and work items of a new geometry g, then a fresh initial state.
+    virtual int apply_geometry() = 0;
+    // Mean duration (ms) of one launch of the fused pass's memory-only twin over `launches`
+    // launches (after one untimed); the state is unchanged.
+    virtual int time_memory_twin(int launches, double *ms) = 0;
     int device = 0;  // HIP device the simulation's memory lives on
     int check_finite() const {
         return nonfinite() ? set_error(MRLBM_ERR_NONFINITE, "non-finite moments: the NaN/Inf watchdog tripped")
@@ -119,13 +133,15 @@ class Sim final : public SimBase {
         HIP_TRY(hipMemsetAsync(donor_, 0, 10 * stride_ * sizeof(S), stream_));
         omega_ = R(1.0 / d.tau);
         rho0_ = R(d.rho0);
-        // 16-bit storage scales: deviations of O(1e-2) mapped to O(1)
-        // 16-bit storage scales of the rho, u and m deviations (FP16 only; BF16 stores them as
-        // they are)
+        // 16-bit storage scales of the rho, u and m deviations: FP16 maps deviations of O(1e-2)
+        // to O(1); the BF16 form stores rho and m as they are (and its velocity in FP16 bits)
         scale_.enc[0] = R(Conv<S>::unit_scale ? 1 : 64);
         scale_.enc[1] = R(Conv<S>::unit_scale ? 1 : 16);
         scale_.enc[2] = R(Conv<S>::unit_scale ? 1 : 256);
         for (int k = 0; k < 3; ++k) scale_.dec[k] = R(1) / scale_.enc[k];
+        // the 16-bit forms store u_x - u_ref (MRLBM_UREF=0: u_x itself)
+        const char *ue = std::getenv("MRLBM_UREF");
+        scale_.uref = (Conv<S>::scaled && !(ue && std::atoi(ue) == 0)) ? R(reference_velocity(d)) : R(0);
 
         // ---- launch configuration of the fused pass (defaults, overridden by the environment)
         LaunchConfig lc = default_launch();
@@ -147,7 +163,31 @@ class Sim final : public SimBase {
         if (const char *e = std::getenv("MRLBM_GRAPH_STEPS")) graph_steps_ = std::max(0, std::atoi(e) / 2 * 2);
         if (const char *e = std::getenv("MRLBM_TWO_STREAM")) two_stream_env_ = std::atoi(e) != 0 ? 1 : 0;
 
-        // ---- stencils of the wall nodes and donor marking
+        HIP_TRY(hipMalloc(&d_ring_, size_t(FORCE_RING_STEPS) * 4 * sizeof(double)));
+        HIP_TRY(hipMalloc(&d_plane_ring_, size_t(FORCE_RING_STEPS) * 2 * sizeof(double)));
+        if (int rc = build_tables()) return rc;
+        HIP_TRY(hipMalloc(&d_step_counter_, 2 * sizeof(unsigned long long)));
+        HIP_TRY(hipMemsetAsync(d_step_counter_, 0, 2 * sizeof(unsigned long long), stream_));
+        HIP_TRY(hipMalloc(&d_pending_, sizeof(unsigned long long)));
+        HIP_TRY(hipMemsetAsync(d_pending_, 0, sizeof(unsigned long long), stream_));
+        HIP_TRY(hipMemsetAsync(d_ring_, 0, size_t(FORCE_RING_STEPS) * 4 * sizeof(double), stream_));
+        HIP_TRY(hipMemsetAsync(d_plane_ring_, 0, size_t(FORCE_RING_STEPS) * 2 * sizeof(double), stream_));
+        HIP_TRY(hipStreamSynchronize(stream_));
+        return configure_launch(lc);
+    }
+
+    // Device tables of the geometry g: per-node info words with donor flags, wall records and
+    // stencils, BCFLUID lists and masks, the inlet profile, force partials and the wall-pressure
+    // sums. Tables of an earlier geometry are freed first (mrlbm_set_geometry).
+    int build_tables() {
+        for (void **pp : {(void **)&d_info_, (void **)&d_wconst_, (void **)&d_wnx_, (void **)&d_wny_, (void **)&d_win_,
+                          (void **)&d_wout_, (void **)&d_wnode_, (void **)&d_wcoef_, (void **)&d_stbase_,
+                          (void **)&d_stfrac_, (void **)&d_bcfnode_, (void **)&d_bcftag_, (void **)&d_bcfin_,
+                          (void **)&d_bcfout_, (void **)&d_inlet_, (void **)&d_partials_, (void **)&d_ps_sum_}) {
+            if (*pp) HIP_TRY(hipFree(*pp));
+            *pp = nullptr;
+        }
+        // stencils of the wall nodes and donor marking
         std::vector<uint32_t> info = g.info;
         const int nwall = int(g.wall[0].node.size() + g.wall[1].node.size());
         std::vector<uint32_t> w_node, w_in, w_out, st_base;
@@ -172,7 +212,7 @@ class Sim final : public SimBase {
                 wall_constants(w.nx[k], w.ny[k], w.in_mask[k], kc);
                 for (int j = 0; j < WALL_CONSTS; ++j) w_const.push_back(R(kc[j]));
                 // wall point and the two fluid points along the normal (curvedLBM.cu:78-84)
-                const R xw = R(d.xc) + R(w.radius) * nx, yw = R(d.yc) + R(w.radius) * ny;
+                const R xw = R(w.xc) + R(w.radius) * nx, yw = R(w.yc) + R(w.radius) * ny;
                 const R dir = R(w.fluid_dir);
                 const R pts[2][2] = {{xw + dir * R(1) * nx, yw + dir * R(1) * ny},
                                      {xw + dir * R(2) * R(1) * nx, yw + dir * R(2) * R(1) * ny}};
@@ -263,12 +303,11 @@ class Sim final : public SimBase {
         if ((rc = upload(d_inlet_, inlet))) return rc;
         // the wall pass launches at most wall_blocks(nwall, nbcf) blocks (forces on)
         HIP_TRY(hipMalloc(&d_partials_, size_t(wall_blocks(nwall, nbcf_)) * 4 * sizeof(double)));
-        HIP_TRY(hipMalloc(&d_ring_, size_t(FORCE_RING_STEPS) * 4 * sizeof(double)));
-        HIP_TRY(hipMalloc(&d_plane_ring_, size_t(FORCE_RING_STEPS) * 2 * sizeof(double)));
         // wall-pressure statistics of the inner wall: fluid points 1..3 spacings out must lie inside
         stats_ok_ = g.ngroups > 0 && !g.wall[0].node.empty();
         for (size_t k = 0; stats_ok_ && k < g.wall[0].node.size(); ++k) {
-            const double xw = d.xc + g.wall[0].radius * g.wall[0].nx[k], yw = d.yc + g.wall[0].radius * g.wall[0].ny[k];
+            const double xw = g.wall[0].xc + g.wall[0].radius * g.wall[0].nx[k],
+                         yw = g.wall[0].yc + g.wall[0].radius * g.wall[0].ny[k];
             for (int p = 1; p <= 3; ++p) {
                 const int x0 = int(std::floor(xw + p * g.wall[0].nx[k])), y0 = int(std::floor(yw + p * g.wall[0].ny[k]));
                 if (x0 < 0 || y0 < 0 || x0 + 1 >= g.nx || y0 + 1 >= g.ny) stats_ok_ = false;
@@ -276,14 +315,9 @@ class Sim final : public SimBase {
         }
         HIP_TRY(hipMalloc(&d_ps_sum_, std::max<size_t>(g.wall[0].node.size(), 1) * sizeof(double)));
         HIP_TRY(hipMemsetAsync(d_ps_sum_, 0, std::max<size_t>(g.wall[0].node.size(), 1) * sizeof(double), stream_));
-        HIP_TRY(hipMalloc(&d_step_counter_, 2 * sizeof(unsigned long long)));
-        HIP_TRY(hipMemsetAsync(d_step_counter_, 0, 2 * sizeof(unsigned long long), stream_));
-        HIP_TRY(hipMalloc(&d_pending_, sizeof(unsigned long long)));
-        HIP_TRY(hipMemsetAsync(d_pending_, 0, sizeof(unsigned long long), stream_));
-        HIP_TRY(hipMemsetAsync(d_ring_, 0, size_t(FORCE_RING_STEPS) * 4 * sizeof(double), stream_));
-        HIP_TRY(hipMemsetAsync(d_plane_ring_, 0, size_t(FORCE_RING_STEPS) * 2 * sizeof(double), stream_));
         HIP_TRY(hipStreamSynchronize(stream_));
-        return configure_launch(lc);
+        stats_samples_ = 0;
+        return MRLBM_OK;
     }
 
     // Fused-pass defaults: 16x8 tiles in FP64 and 32x8 in FP32 arithmetic (LDS-bound; 16x8 for
@@ -484,8 +518,8 @@ class Sim final : public SimBase {
         p.n = int(g.wall[0].node.size());
         p.nx = g.nx;
         p.ny = g.ny;
-        p.xc = d.xc;
-        p.yc = d.yc;
+        p.xc = g.wall[0].xc;
+        p.yc = g.wall[0].yc;
         p.radius = g.wall[0].radius;
         p.rho0 = d.rho0;
         p.scale = scale_;
@@ -634,6 +668,34 @@ class Sim final : public SimBase {
         if (stats_on(flags)) ++stats_samples_;
     }
 
+    double velocity_reference() const override { return double(scale_.uref); }
+
+    int time_memory_twin(int launches, double *ms) override {
+        if (launches <= 0 || !ms) return set_error(MRLBM_ERR_INVALID, "launch count must be positive");
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        HIP_TRY(hipEventCreate(&e0));
+        HIP_TRY(hipEventCreate(&e1));
+        FusedArgs<R, S> a = fused_args(cur_);
+        launch_memory_twin<R, S>(a, bx_, by_, stream_);
+        HIP_TRY(hipEventRecord(e0, stream_));
+        for (int k = 0; k < launches; ++k) launch_memory_twin<R, S>(a, bx_, by_, stream_);
+        HIP_TRY(hipEventRecord(e1, stream_));
+        HIP_TRY(hipEventSynchronize(e1));
+        HIP_TRY(hipGetLastError());
+        float t = 0.f;
+        HIP_TRY(hipEventElapsedTime(&t, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        *ms = double(t) / launches;
+        return MRLBM_OK;
+    }
+
+    int apply_geometry() override {
+        HIP_TRY(hipStreamSynchronize(stream_));
+        if (int rc = build_tables()) return rc;
+        if (int rc = configure_launch(launch_)) return rc;
+        return init();
+    }
     bool nonfinite() const override { return h_nonfinite_ && __atomic_load_n(h_nonfinite_, __ATOMIC_ACQUIRE) != 0u; }
     void clear_nonfinite() {
         if (h_nonfinite_) __atomic_store_n(h_nonfinite_, 0u, __ATOMIC_RELEASE);

@@ -1,10 +1,11 @@
 // storage.h - how the 10 moments of a node are kept in HBM.
 //
 // FP64/FP32 store the reference's fields as they are (rho - rho0, u, m; mlbm.cu:154-163).
-// The 16-bit forms store deviations: rho - rho0, u and the non-equilibrium second moments
-// m - u u, decoded in FP32. FP16 scales them by s_rho, s_u, s_m into its normal range; BF16 has
-// float's exponent range and stores them unscaled (the scales are powers of two, so scaling would
-// not change a single rounding).
+// The 16-bit forms store deviations: rho - rho0, u_x - u_ref (u_ref: the case's mean inflow
+// velocity), u_y, u_z and the non-equilibrium second moments m - u u, decoded in FP32. FP16 scales
+// them by s_rho, s_u, s_m into its normal range; the BF16 form keeps rho - rho0 and m - u u in BF16
+// unscaled (float's exponent range; power-of-two scales would not change a single rounding) and
+// the velocity in FP16 bits times 16.
 #pragma once
 
 #include <hip/hip_bf16.h>
@@ -20,6 +21,7 @@ template <class R>
 struct StoreScale {
     R enc[3];  // rho, u, m multipliers (16-bit storage only)
     R dec[3];  // their inverses
+    R uref;    // reference velocity (x): the 16-bit forms store u_x - uref
 };
 
 template <class S>
@@ -51,10 +53,16 @@ typedef __attribute__((__vector_size__(2 * sizeof(_Float16)))) _Float16 f16x2_t;
 // would drop every per-step increment smaller than half an ulp and stall slowly evolving flow.
 // `r` holds 8 random bits; the 8 mantissa bits below them are added as zero, which leaves a
 // bias below 2^-9 ulp.
+// The BF16 form keeps the velocity in FP16 bits (u_x - u_ref, u_y, u_z, times kVelocityScale):
+// BF16's 8-bit significand made the velocity's rounding noise the dominant storage error (drag
+// error on 16 seeded 64-192 x 64-96 x 32-96 channels, mean / max over the cases after 10,000
+// steps: 0.92% / 1.73% with a BF16 velocity, 0.12% / 0.28% with an FP16 one); density and
+// non-equilibrium second moments stay BF16.
 template <>
 struct Conv<__hip_bfloat16> {
     static constexpr bool scaled = true;
     static constexpr bool unit_scale = true;
+    static constexpr bool half_velocity = true;
     __device__ static float to(__hip_bfloat16 v) { return __bfloat162float(v); }
     __device__ static __hip_bfloat16 from(float v) { return __float2bfloat16(v); }
     __device__ static __hip_bfloat16 from_sr(float v, uint32_t r8) {
@@ -67,6 +75,7 @@ template <>
 struct Conv<_Float16> {
     static constexpr bool scaled = true;
     static constexpr bool unit_scale = false;
+    static constexpr bool half_velocity = false;
     __device__ static float to(_Float16 v) { return float(v); }
     __device__ static _Float16 from(float v) { return _Float16(v); }
     // v_cvt_sr_f16_f32 adds the high 13 bits of its random operand below the kept mantissa bits
@@ -285,6 +294,91 @@ __device__ __forceinline__ void load_raw(const FieldSet<const S> &fs, uint32_t i
     }
 }
 
+// Multiplier of the velocity kept in FP16 bits by the BF16 form (a power of two: exact).
+constexpr float kVelocityScale = 16.0f;
+
+// Whether the 16-bit form S keeps its velocity in FP16 bits.
+template <class S>
+__host__ __device__ constexpr bool has_half_velocity() {
+    if constexpr (Conv<S>::scaled) {
+        return Conv<S>::half_velocity;
+    } else {
+        return false;
+    }
+}
+
+// Value of stored field f before the deviation is undone (scaled back for FP16).
+template <class R, class S>
+__device__ __forceinline__ R stored_value(S raw, int f, const StoreScale<R> &sc) {
+    if constexpr (has_half_velocity<S>()) {
+        if (f >= 1 && f <= 3) {
+            const _Float16 h = __builtin_bit_cast(_Float16, __builtin_bit_cast(uint16_t, raw));
+            return R(float(h)) * R(1.0f / kVelocityScale);
+        }
+    }
+    R v = R(Conv<S>::to(raw));
+    if constexpr (!Conv<S>::unit_scale) v *= sc.dec[field_class(f)];
+    return v;
+}
+
+// Stored bits of deviation v of field f (v already scaled for FP16 storage); r8: random bits of
+// the stochastic rounding (sr), unused with round-to-nearest.
+template <class R, class S>
+__device__ __forceinline__ S stored_bits(R v, int f, uint32_t r8, bool sr) {
+    if constexpr (has_half_velocity<S>()) {
+        if (f >= 1 && f <= 3) {
+            const float hv = float(v) * kVelocityScale;
+            const _Float16 h = sr ? Conv<_Float16>::from_sr(hv, r8) : _Float16(hv);
+            return __builtin_bit_cast(S, __builtin_bit_cast(uint16_t, h));
+        }
+    }
+    return sr ? Conv<S>::from_sr(v, r8) : Conv<S>::from(v);
+}
+
+// The stored words of one node as loaded -- three accesses (FP32 and 16-bit storage: fields 0-3,
+// 4-7, 8-9) or five pairs (FP64) -- kept packed until the node is decoded. A 16-bit node then
+// occupies five 32-bit registers instead of ten, which the marches' prefetched planes need.
+template <class S, bool Q = kQuads<S>>
+struct RawNode;
+template <class S>
+struct RawNode<S, true> {
+    quad_t<S> q0, q1;
+    pair_t<S> p;
+};
+template <class S>
+struct RawNode<S, false> {
+    pair_t<S> p[5];
+};
+
+template <class S>
+__device__ __forceinline__ void load_raw(const FieldSet<const S> &fs, uint32_t idx, RawNode<S> &r) {
+    if constexpr (kQuads<S>) {
+        const S *cb = fs.base + chunk_base<S>(idx);
+        const uint32_t l = idx % kChunk<S>;
+        const quad_t<S> *__restrict__ q = reinterpret_cast<const quad_t<S> *>(cb) + l;
+        const pair_t<S> *__restrict__ p = reinterpret_cast<const pair_t<S> *>(cb + 8 * kChunk<S>) + l;
+        r.q0 = q[0];
+        r.q1 = q[kChunk<S>];
+        r.p = p[0];
+    } else {
+        const pair_t<S> *__restrict__ p = pair_ptr(fs.base, idx);
+#pragma unroll
+        for (int j = 0; j < 5; ++j) r.p[j] = p[j * kChunk<S>];
+    }
+}
+
+template <class S>
+__device__ __forceinline__ void unpack_raw(const RawNode<S> &r, S *v) {
+    if constexpr (kQuads<S>) {
+        split_quad<S>(r.q0, v);
+        split_quad<S>(r.q1, v + 4);
+        split_pair<S>(r.p, v[8], v[9]);
+    } else {
+#pragma unroll
+        for (int j = 0; j < 5; ++j) split_pair<S>(r.p[j], v[2 * j], v[2 * j + 1]);
+    }
+}
+
 // Moments from stored values (M[0] absolute density).
 template <class R, class S>
 __device__ __forceinline__ void decode_node(const S *raw, const StoreScale<R> &sc, R rho0, R *M) {
@@ -295,10 +389,10 @@ __device__ __forceinline__ void decode_node(const S *raw, const StoreScale<R> &s
     } else {
 #pragma unroll
         for (int f = 0; f < 10; ++f) {
-            M[f] = R(Conv<S>::to(raw[f]));
-            if constexpr (!Conv<S>::unit_scale) M[f] *= sc.dec[field_class(f)];
+            M[f] = stored_value<R, S>(raw[f], f, sc);
         }
         M[0] += rho0;
+        M[1] += sc.uref;
         M[4] += M[1] * M[1];
         M[5] += M[2] * M[2];
         M[6] += M[3] * M[3];
@@ -306,6 +400,13 @@ __device__ __forceinline__ void decode_node(const S *raw, const StoreScale<R> &s
         M[8] += M[1] * M[3];
         M[9] += M[2] * M[3];
     }
+}
+
+template <class R, class S>
+__device__ __forceinline__ void decode_node(const RawNode<S> &r, const StoreScale<R> &sc, R rho0, R *M) {
+    S raw[10];
+    unpack_raw<S>(r, raw);
+    decode_node<R, S>(raw, sc, rho0, M);
 }
 
 // Decode the moments of node `idx` (M[0] absolute density).
@@ -329,7 +430,7 @@ __device__ __forceinline__ void encode_node(uint32_t idx, const StoreScale<R> &s
     } else {
         R v[10];
         v[0] = M[0] - rho0;
-        v[1] = M[1];
+        v[1] = M[1] - sc.uref;
         v[2] = M[2];
         v[3] = M[3];
         v[4] = M[4] - M[1] * M[1];
@@ -344,34 +445,53 @@ __device__ __forceinline__ void encode_node(uint32_t idx, const StoreScale<R> &s
         }
         if (seed == 0) {
 #pragma unroll
-            for (int f = 0; f < 10; ++f) out[f] = Conv<S>::from(v[f]);
+            for (int f = 0; f < 10; ++f) out[f] = stored_bits<R, S>(v[f], f, 0u, false);
         } else {
             const uint32_t h = sr_hash(idx, seed);
 #pragma unroll
-            for (int f = 0; f < 10; ++f) out[f] = Conv<S>::from_sr(v[f], sr_bits(h, f));
+            for (int f = 0; f < 10; ++f) out[f] = stored_bits<R, S>(v[f], f, sr_bits(h, f), true);
         }
     }
 }
 
-// Store the moments of node `idx` into a field-major buffer.
-template <class R, class S>
-__device__ __forceinline__ void store_node(const FieldSet<S> &fs, uint32_t idx, const StoreScale<R> &sc, R rho0,
-                                           const R *M, uint32_t seed = 0) {
-    S v[10];
-    encode_node<R, S>(idx, sc, rho0, M, seed, v);
+// Store 10 stored values of node `idx` (the layout of load_raw).
+template <class S>
+__device__ __forceinline__ void store_raw(const FieldSet<S> &fs, uint32_t idx, const S *v) {
     if constexpr (kQuads<S>) {
         S *cb = fs.base + chunk_base<S>(idx);
         const uint32_t l = idx % kChunk<S>;
         quad_t<S> *__restrict__ q = reinterpret_cast<quad_t<S> *>(cb) + l;
         pair_t<S> *__restrict__ p = reinterpret_cast<pair_t<S> *>(cb + 8 * kChunk<S>) + l;
+#ifdef MRLBM_NT_STORES
+        __builtin_nontemporal_store(join_quad<S>(v), q);
+        __builtin_nontemporal_store(join_quad<S>(v + 4), q + kChunk<S>);
+        __builtin_nontemporal_store(join_pair<S>(v[8], v[9]), p);
+#else
         q[0] = join_quad<S>(v);
         q[kChunk<S>] = join_quad<S>(v + 4);
         p[0] = join_pair<S>(v[8], v[9]);
+#endif
     } else {
         pair_t<S> *__restrict__ p = pair_ptr(fs.base, idx);
 #pragma unroll
         for (int j = 0; j < 5; ++j) p[j * kChunk<S>] = join_pair<S>(v[2 * j], v[2 * j + 1]);
     }
+}
+
+template <class S>
+__device__ __forceinline__ void store_raw(const FieldSet<S> &fs, uint32_t idx, const RawNode<S> &r) {
+    S v[10];
+    unpack_raw<S>(r, v);
+    store_raw<S>(fs, idx, v);
+}
+
+// Store the moments of node `idx`.
+template <class R, class S>
+__device__ __forceinline__ void store_node(const FieldSet<S> &fs, uint32_t idx, const StoreScale<R> &sc, R rho0,
+                                           const R *M, uint32_t seed = 0) {
+    S v[10];
+    encode_node<R, S>(idx, sc, rho0, M, seed, v);
+    store_raw<S>(fs, idx, v);
 }
 
 // Node-major buffers (the 10 values of node i at base[10 i .. 10 i + 9]): the donor snapshot,

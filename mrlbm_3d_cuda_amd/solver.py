@@ -128,6 +128,12 @@ class Simulation:
                                            C.byref(med), C.byref(reps)))
         return total.value, med.value, reps.value
 
+    def time_memory_twin(self, launches=20):
+        """Mean ms of one launch of the fused pass's memory-only twin (the access pattern's floor)."""
+        ms = C.c_double()
+        _check(self.lib.mrlbm_time_memory_twin(self.h, launches, C.byref(ms)))
+        return ms.value
+
     # -- state
     def field(self, f):
         fi = _abi.FIELDS.index(f) if isinstance(f, str) else int(f)
@@ -215,6 +221,39 @@ class Simulation:
         i = Info()
         _check(self.lib.mrlbm_get_info(self.h, C.byref(i)))
         return i
+
+    def set_geometry(self, node_types, walls=(), bcf_in=None, bcf_out=None):
+        """Replace the geometry (mrlbm_set_geometry): node types in the reference encoding
+        (type << 20 | index, x fastest), one dict per wall group with the keys of wall_table()
+        plus xc, yc, radius, fluid_side (+1 fluid outside the circle, -1 inside) and u_wall (3 values),
+        and the 256-entry BCFLUID mask tables. A device simulation restarts from init()."""
+        types = np.ascontiguousarray(node_types, dtype=np.uint32).reshape(-1)
+        assert types.size == self.n
+        keep = [types]
+        arr = (_abi.WallDesc * max(len(walls), 1))()
+        for g, w in enumerate(walls):
+            cols = {}
+            for key, ct, dt in (("node", C.c_int64, np.int64), ("nx", C.c_double, np.float64),
+                                ("ny", C.c_double, np.float64), ("delta", C.c_double, np.float64),
+                                ("in_mask", C.c_uint32, np.uint32), ("out_mask", C.c_uint32, np.uint32)):
+                a = np.ascontiguousarray(w[key], dtype=dt)
+                keep.append(a)
+                cols[key] = as_ptr(a, ct)
+            arr[g] = _abi.WallDesc(len(w["node"]), 0, cols["node"], cols["nx"], cols["ny"], cols["delta"],
+                                   cols["in_mask"], cols["out_mask"], float(w["xc"]), float(w["yc"]),
+                                   float(w["radius"]), float(w["fluid_side"]),
+                                   (C.c_double * 3)(*w["u_wall"]))
+        masks = []
+        for m in (bcf_in, bcf_out):
+            if m is None:
+                masks.append(None)
+            else:
+                a = np.ascontiguousarray(m, dtype=np.uint32)
+                assert a.size == 256
+                keep.append(a)
+                masks.append(as_ptr(a, C.c_uint32))
+        _check(self.lib.mrlbm_set_geometry(self.h, as_ptr(types, C.c_uint32), len(walls), arr, masks[0], masks[1]))
+        return self
 
     def node_types(self):
         out = np.empty(self.n, dtype=np.uint32)

@@ -93,8 +93,9 @@ def test_checkpoint_round_trip_bitwise(tmp_path, precision):
 
 @pytest.mark.parametrize("precision", [_abi.FP32_BF16, _abi.FP32_FP16])
 def test_stored_deviation_layout(tmp_path, precision):
-    """The checkpoint payload holds the stored 16-bit values: BF16 keeps rho - rho0 and u as they
-    are, FP16 scales them by 64 and 16 (exact powers of two); the decoded fields equal them."""
+    """The checkpoint payload holds the stored 16-bit values: rho - rho0 (BF16 as it is, FP16 times
+    64), then u_x - u_ref, u_y, u_z as FP16 times 16 in both forms (u_ref = 2/3 u_max, the parabolic
+    inlet's mean, in the header); the decoded fields equal them, in FP32 arithmetic."""
     from mrlbm_3d_cuda_amd.solver import Simulation
     d = channel(precision)
     with Simulation(d) as s:
@@ -103,15 +104,20 @@ def test_stored_deviation_layout(tmp_path, precision):
         s.checkpoint_write(path)
         F = s.fields()
     n = d.nx * d.ny * d.nz
-    raw = np.frombuffer(open(path, "rb").read()[-20 * n:], dtype=np.uint16).reshape(10, n)
+    data = open(path, "rb").read()
+    uref = np.frombuffer(data[-20 * n - 8:-20 * n], dtype=np.float64)[0]
+    assert uref == np.float32(d.u_max * 2.0 / 3.0)
+    raw = np.frombuffer(data[-20 * n:], dtype=np.uint16).reshape(10, n)
     if precision == _abi.FP32_BF16:
-        stored = (raw.astype(np.uint32) << 16).view(np.float32).astype(np.float64)
-        scales = [1.0, 1.0, 1.0, 1.0]
+        rho = (raw[0].astype(np.uint32) << 16).view(np.float32)
     else:
-        stored = raw.view(np.float16).astype(np.float64)
-        scales = [64.0, 16.0, 16.0, 16.0]
-    for f in range(4):
-        assert np.array_equal(stored[f] / scales[f], F[f].ravel()), f
+        rho = raw[0].view(np.float16).astype(np.float32) * np.float32(1.0 / 64.0)
+    assert np.array_equal(rho.astype(np.float64), F[0].ravel())
+    for f in range(1, 4):
+        u = raw[f].view(np.float16).astype(np.float32) * np.float32(1.0 / 16.0)
+        if f == 1:
+            u = u + np.float32(uref)
+        assert np.array_equal(u.astype(np.float64), F[f].ravel()), f
 
 
 @pytest.mark.parametrize("precision", [_abi.FP64, _abi.FP32])
