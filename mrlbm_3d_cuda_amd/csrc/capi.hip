@@ -511,7 +511,6 @@ int mrlbm_write_pressure(mrlbm_sim *s, const char *dir) {
     FILE *fp = std::fopen(path.c_str(), "w");
     if (!fp) return set_error(MRLBM_ERR_IO, "cannot write " + path);
     const double pi = 3.14159265358979323846;
-    const mrlbm_case_desc &d = s->impl->d;
     bool ok = std::fputs("theta z Cp\n", fp) >= 0;
     for (size_t k = 0; k < w.node.size() && ok; ++k) {
         const int64_t i = w.node[k];

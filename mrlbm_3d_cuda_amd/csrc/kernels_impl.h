@@ -110,8 +110,8 @@ __device__ __forceinline__ void watch_finite(const FusedArgs<R, S> &a, R rho) {
         if (a.nonfinite) *a.nonfinite = 1u;
 }
 
-// Per-node table values of a boundary node, loaded one iteration before its update with
-// in-range indices whatever the node code (so every thread issues the same loads).
+// Per-node table values of a boundary node, loaded before the plane's prefetch with in-range
+// indices whatever the node code (so every thread issues the same loads).
 template <class R>
 struct NodeTabs {
     R wnx, wny, inlet;
@@ -133,15 +133,15 @@ __device__ __forceinline__ void load_tabs(const FusedArgs<R, S> &a, uint32_t inf
 // LDS form of reconstruct_lds), its moments at the start of the step (old) and those of its x-1
 // neighbour (oxm), with the node boundary condition (the per-node part of
 // streaming_and_evaluate_Mom and collision_halo_update, solver/mlbm.cu:83-164, 186-242). Returns
-// in M the value to store (collided unless the wall pass finishes the node) and in Md the
-// pre-collision snapshot for donor nodes; store is false for frozen nodes.
+// in M the pre-collision moments (the snapshot of donor nodes), in do_collide whether the node
+// collides now (not when the wall pass finishes it), and store = false for frozen nodes.
 template <class R, class S>
 __device__ __forceinline__ void boundary_node(const FusedArgs<R, S> &a, R *f, const R *old, const R *oxm,
-                                              uint32_t info, const NodeTabs<R> &t, R *M, R *Md, bool &store,
-                                              bool &donor) {
+                                              uint32_t info, const NodeTabs<R> &t, R *M, bool &store, bool &donor,
+                                              bool &do_collide) {
     const uint32_t code = info & INFO_CODE_MASK;
     const uint32_t payload = info >> INFO_PAYLOAD_SHIFT;
-    bool do_collide = true;
+    do_collide = true;
     store = true;
     if (code == NC_BULK) {
         moments_lds(f, M);
@@ -204,41 +204,24 @@ __device__ __forceinline__ void boundary_node(const FusedArgs<R, S> &a, R *f, co
         }
     }
     donor = store && (info & INFO_DONOR);
-#pragma unroll
-    for (int k = 0; k < 10; ++k) Md[k] = M[k];
-    if (do_collide) collide(a.omega, M);
 }
 
 // State and steps of the boundary-tile path shared by both marches. Per thread: the stored
 // values of its own node for the plane being updated and the next one, the node codes of those
-// planes, the table values of the plane being updated, and the previous plane's results (stored
-// at the start of the next iteration, like the bulk path's). The moments of the west ring column
-// go through LDS (wold, three planes) so that a node in the tile's first column can read its x-1
-// neighbour.
+// planes and the table values of the plane being updated. The results of a plane are stored as
+// soon as they are computed (the bulk path keeps its results in registers until the next
+// iteration; here that would hold 20 more values through the boundary closed forms, which then
+// spill). The moments of the west ring column go through LDS (wold, three planes) so that a node
+// in the tile's first column can read its x-1 neighbour.
 template <class R, class S, int BX, int BY>
 struct BoundaryPath {
     static constexpr int W = BX + 2;
     RawNode<S> own_cur, own_nx;
     uint32_t info_cur = 0, info_nx = 0;
     NodeTabs<R> tab;
-    R Mp[10], Mdp[10];
-    uint32_t idxp, didxp;
 
     __device__ void load_info(const FusedArgs<R, S> &a, uint32_t node, uint32_t &out) { out = a.info[node]; }
     __device__ void copy_own(const RawNode<S> &raw, RawNode<S> &dst) { dst = raw; }
-    // The donor snapshot is encoded only in waves that hold a donor node; the others still issue
-    // its stores (into the padding slot) so that every path keeps the same memory operations.
-    __device__ void store_prev(const FusedArgs<R, S> &a, uint32_t seed) {
-        store_node<R, S>(a.dst, idxp, a.scale, a.rho0, Mp, seed);
-        S v[10];
-        if (__any(didxp != a.pad)) {
-            encode_node<R, S>(didxp, a.scale, a.rho0, Mdp, seed ^ 0x5A5A5A5Au, v);
-        } else {
-#pragma unroll
-            for (int k = 0; k < 10; ++k) v[k] = Conv<S>::from(R(0));
-        }
-        store_raw_nm(a.donor, didxp, v);
-    }
     // the x-1 neighbour's moments at the plane being updated: lane - 1 inside the tile row, the
     // west ring column (LDS) for the first column
     __device__ void neighbour_xm1(const R *old, int tx, int ty, const R *wold, R *oxm) {
@@ -248,8 +231,12 @@ struct BoundaryPath {
             oxm[k] = tx > 0 ? v : wold[k * BY + ty];
         }
     }
+    // Updates the thread's node of the plane and stores it, with the donor snapshot (the
+    // pre-collision moments) first. The snapshot is encoded only in waves that hold a donor node;
+    // the others still issue its stores (into the padding slot), so every path keeps the same
+    // memory operations.
     __device__ void update(const FusedArgs<R, S> &a, R *f, int tx, int ty, bool inside, bool east_tile, uint32_t node,
-                           const R *wold) {
+                           const R *wold, uint32_t seed) {
         // the node's own and x-1 moments are only read by boundary nodes: waves made of bulk
         // nodes skip their decode
         R old[10], oxm[10];
@@ -262,10 +249,22 @@ struct BoundaryPath {
 #pragma unroll
             for (int k = 0; k < 10; ++k) old[k] = oxm[k] = R(0);
         }
-        bool store, donor;
-        boundary_node(a, f, old, oxm, info_cur, tab, Mp, Mdp, store, donor);
-        idxp = (inside && store) ? node : a.pad;
-        didxp = (inside && donor) ? node : a.pad;
+        R M[10];
+        bool store, donor, do_collide;
+        boundary_node(a, f, old, oxm, info_cur, tab, M, store, donor, do_collide);
+        watch_finite(a, M[0]);
+        const uint32_t idx = (inside && store) ? node : a.pad;
+        const uint32_t didx = (inside && donor) ? node : a.pad;
+        S v[10];
+        if (__any(didx != a.pad)) {
+            encode_node<R, S>(didx, a.scale, a.rho0, M, seed ^ 0x5A5A5A5Au, v);
+        } else {
+#pragma unroll
+            for (int k = 0; k < 10; ++k) v[k] = Conv<S>::from(R(0));
+        }
+        store_raw_nm(a.donor, didx, v);
+        if (do_collide) collide(a.omega, M);
+        store_node<R, S>(a.dst, idx, a.scale, a.rho0, M, seed);
     }
 };
 
@@ -274,12 +273,13 @@ struct BoundaryPath {
 // are in flight in registers while a plane is updated.
 //
 // Vector-memory loads and stores share one counter, counted in issue order. Every path of an
-// iteration issues the same loads and stores: results of a plane are kept in registers and
+// iteration issues the same loads and stores: bulk results of a plane are kept in registers and
 // stored at the start of the next iteration (after the wait for the prefetched plane, before the
-// next prefetch), threads outside the grid, frozen nodes and non-donors store into the padding
-// slot after the last node, second-position loads of threads without one repeat the first, and
-// the prefetch past the item wraps around in z. A wait for a prefetched plane therefore never
-// waits for stores or loads issued after it.
+// next prefetch), boundary-tile results right after the update (after the prefetch), threads
+// outside the grid, frozen nodes and non-donors store into the padding slot after the last node,
+// second-position loads of threads without one repeat the first, and the prefetch past the item
+// wraps around in z. A wait for a prefetched plane therefore never waits for stores or loads
+// issued after it.
 template <class R, class S, int BX, int BY, bool SPECIAL>
 __device__ __forceinline__ void fused_march(const FusedArgs<R, S> &a, R *lds, int tile, int z0, int z1) {
     using F = Frame<BX, BY>;
@@ -355,7 +355,6 @@ __device__ __forceinline__ void fused_march(const FusedArgs<R, S> &a, R *lds, in
     if constexpr (SPECIAL) {
         bp.copy_own(m0, bp.own_cur);
         bp.load_info(a, off0 + uint32_t(z0) * plane, bp.info_cur);
-        bp.idxp = bp.didxp = a.pad;
     }
     load_plane(z0 + 1);
 
@@ -390,11 +389,10 @@ __device__ __forceinline__ void fused_march(const FusedArgs<R, S> &a, R *lds, in
     };
 
     for (int z = z0; z < z1; ++z) {
-        if constexpr (SPECIAL) load_tabs(a, bp.info_cur, uint32_t(y) + uint32_t(a.ny) * uint32_t(z), bp.tab);
         put_plane(sp_next, s0_cur ^ 1, true, true, z + 1);  // plane z+1
         if constexpr (SPECIAL) {
             bp.copy_own(m0, bp.own_nx);
-            bp.store_prev(a, seed);
+            load_tabs(a, bp.info_cur, uint32_t(y) + uint32_t(a.ny) * uint32_t(z), bp.tab);
             bp.load_info(a, off0 + uint32_t(pmod(z + 1, a.nz)) * plane, bp.info_nx);
         } else {
             store_node<R, S>(a.dst, idxp, a.scale, a.rho0, Mp, seed);
@@ -405,8 +403,7 @@ __device__ __forceinline__ void fused_march(const FusedArgs<R, S> &a, R *lds, in
         pull(f);
         const uint32_t node = off0 + uint32_t(z) * plane;
         if constexpr (SPECIAL) {
-            bp.update(a, f, tx, ty, inside, east_tile, node, wold + west_slot(z) * 10 * BY);
-            watch_finite(a, bp.Mp[0]);
+            bp.update(a, f, tx, ty, inside, east_tile, node, wold + west_slot(z) * 10 * BY, seed);
         } else {
             moments_lds(f, Mp);
             collide(a.omega, Mp);
@@ -420,10 +417,7 @@ __device__ __forceinline__ void fused_march(const FusedArgs<R, S> &a, R *lds, in
             bp.info_cur = bp.info_nx;
         }
     }
-    if constexpr (SPECIAL)
-        bp.store_prev(a, seed);
-    else
-        store_node<R, S>(a.dst, idxp, a.scale, a.rho0, Mp, seed);
+    if constexpr (!SPECIAL) store_node<R, S>(a.dst, idxp, a.scale, a.rho0, Mp, seed);
 }
 
 // Register-window march: LDS holds NBUF planes of the 27 populations pulled in one iteration;
@@ -510,7 +504,6 @@ __device__ __forceinline__ void window_march(const FusedArgs<R, S> &a, R *lds, i
         if constexpr (SPECIAL) {
             bp.copy_own(rb0, bp.own_cur);
             bp.load_info(a, off0 + uint32_t(z0) * plane, bp.info_cur);
-            bp.idxp = bp.didxp = a.pad;
         }
     }
 
@@ -542,12 +535,11 @@ __device__ __forceinline__ void window_march(const FusedArgs<R, S> &a, R *lds, i
     };
     auto iteration = [&](int z, R *G, R *w0, R *w1) {
         if (NBUF == 1 && z > z0) lds_barrier();  // previous plane's pulls are done
-        if constexpr (SPECIAL) load_tabs(a, bp.info_cur, uint32_t(y) + uint32_t(a.ny) * uint32_t(z), bp.tab);
         put(ra0, h, G, w0, q0, false, z + 1);
         if (has1) put(ra1, p1, G, w1, q1, west1, z + 1);
         if constexpr (SPECIAL) {
             bp.copy_own(ra0, bp.own_nx);
-            bp.store_prev(a, seed);
+            load_tabs(a, bp.info_cur, uint32_t(y) + uint32_t(a.ny) * uint32_t(z), bp.tab);
             bp.load_info(a, off0 + uint32_t(pmod(z + 1, a.nz)) * plane, bp.info_nx);
         } else {
             store_node<R, S>(a.dst, idxp, a.scale, a.rho0, Mp, seed);
@@ -559,8 +551,7 @@ __device__ __forceinline__ void window_march(const FusedArgs<R, S> &a, R *lds, i
         for (int q = 0; q < Q; ++q) f[q] = G[q * NH + h - LATTICE.cx[q] - LATTICE.cy[q] * W];
         const uint32_t node = off0 + uint32_t(z) * plane;
         if constexpr (SPECIAL) {
-            bp.update(a, f, tx, ty, inside, east_tile, node, wold + west_slot(z) * 10 * BY);
-            watch_finite(a, bp.Mp[0]);
+            bp.update(a, f, tx, ty, inside, east_tile, node, wold + west_slot(z) * 10 * BY, seed);
             bp.copy_own(bp.own_nx, bp.own_cur);
             bp.info_cur = bp.info_nx;
         } else {
@@ -581,10 +572,7 @@ __device__ __forceinline__ void window_march(const FusedArgs<R, S> &a, R *lds, i
         iteration(z, G0, pa0, pa1);
         ++z;
     }
-    if constexpr (SPECIAL)
-        bp.store_prev(a, seed);
-    else
-        store_node<R, S>(a.dst, idxp, a.scale, a.rho0, Mp, seed);
+    if constexpr (!SPECIAL) store_node<R, S>(a.dst, idxp, a.scale, a.rho0, Mp, seed);
     if (NBUF == 1) lds_barrier();  // the block's LDS is free again for the next item
 }
 
