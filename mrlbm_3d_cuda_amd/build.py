@@ -20,11 +20,14 @@ ARCH = os.environ.get("MRLBM_OFFLOAD_ARCH", "gfx950")
 # reciprocal and square-root approximations (about 1 ulp, like the bulk path's reciprocal of rho)
 # instead of the correctly rounded sequences, which dominated the boundary-node closed forms, and
 # flush FP32 denormals (the moments are O(1) and their deviations far above 1e-38), which removes
-# the operand scaling the approximations otherwise need to keep denormal results
+# the operand scaling the approximations otherwise need to keep denormal results; they fuse
+# multiply-adds only within one source expression, so a function rounds alike in every kernel it
+# is inlined into (the batched and the single-case fused kernels give the same bits)
 SOURCES = {
     "geometry.cpp": [],
     "kernels_f64.hip": ["-ffp-contract=off"],
-    "kernels_f32.hip": ["-fno-hip-fp32-correctly-rounded-divide-sqrt", "-fgpu-flush-denormals-to-zero"],
+    "kernels_f32.hip": ["-fno-hip-fp32-correctly-rounded-divide-sqrt", "-fgpu-flush-denormals-to-zero",
+                        "-ffp-contract=on"],
     "capi.hip": [],
     "batch.hip": [],
     "output.hip": ["-ffp-contract=off"],

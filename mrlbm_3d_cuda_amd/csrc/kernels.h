@@ -222,7 +222,9 @@ struct LaunchConfig {
     int variant = 1;        // 1 ring march, 3 / 4 register-window march with 1 / 2 LDS planes
     int lz = 32;            // z chunk of a work item
     int lz_special = 0;     // z chunk of boundary tiles (0: lz)
-    bool xcd_remap = true;  // one z slab per XCD
+    bool xcd_remap = true;  // one slab per XCD
+    int slab_axis = 2;      // axis cut into the XCD slabs: 2 = z, 1 = y (rows of tiles, z chunks in rounds)
+    bool special_first = true;  // boundary tiles first (in a z slab, or in a z round of a y slab)
 };
 
 // Tile shapes (BX x BY) the fused kernel is instantiated for.

@@ -159,6 +159,8 @@ class Sim final : public SimBase {
         if (const char *e = std::getenv("MRLBM_LZ")) lc.lz = std::max(1, std::atoi(e));
         if (const char *e = std::getenv("MRLBM_LZ_SPECIAL")) lc.lz_special = std::max(1, std::atoi(e));
         if (const char *e = std::getenv("MRLBM_XCD_REMAP")) lc.xcd_remap = std::atoi(e) != 0;
+        if (const char *e = std::getenv("MRLBM_SLAB_AXIS")) lc.slab_axis = (e[0] == 'y' || e[0] == '1') ? 1 : 2;
+        if (const char *e = std::getenv("MRLBM_SPECIAL_FIRST")) lc.special_first = std::atoi(e) != 0;
         if (const char *e = std::getenv("MRLBM_STOCHASTIC_ROUNDING")) stochastic_ = Conv<S>::scaled && std::atoi(e) != 0;
         if (const char *e = std::getenv("MRLBM_GRAPH_STEPS")) graph_steps_ = std::max(0, std::atoi(e) / 2 * 2);
         if (const char *e = std::getenv("MRLBM_TWO_STREAM")) two_stream_env_ = std::atoi(e) != 0 ? 1 : 0;
@@ -347,10 +349,11 @@ class Sim final : public SimBase {
         return lc;
     }
 
-    // Tile shape, march variant and work items of the fused pass. The z axis is cut into one
-    // slab per XCD (blocks are dispatched round-robin over the XCDs, so slab j runs on XCD j and
-    // its tiles share that L2); inside a slab every tile is cut into z chunks of lz planes
-    // (lz_special for tiles that hold boundary or donor nodes), and the boundary tiles go first.
+    // Tile shape, march variant and work items of the fused pass. The grid is cut into one slab
+    // per XCD (blocks are dispatched round-robin over the XCDs, so slab j runs on XCD j and its
+    // tiles share that L2): z slabs, inside which every tile is cut into z chunks of lz planes
+    // (lz_special for tiles that hold boundary or donor nodes), or y slabs marched in z rounds.
+    // The boundary tiles go first.
     int configure_launch(const LaunchConfig &want) {
         LaunchConfig lc = want;
         if (!fused_tile_supported(lc.bx, lc.by)) return set_error(MRLBM_ERR_INVALID, "unsupported fused tile");
@@ -387,28 +390,44 @@ class Sim final : public SimBase {
                         tile_wall[xx / lc.bx + tiles_x * (yy / lc.by)] = 1;
                     }
             }
-        const int groups = (lc.xcd_remap && g.nz >= 32) ? 8 : 1;
-        const int slab = (g.nz + groups - 1) / groups;
+        // y slabs: each XCD owns whole rows of tiles over the full z range and runs them in rounds
+        // of one z chunk, so that a chunk's halo planes are still in that XCD's L2 from the round
+        // before (one z chunk for every tile: lz_special does not apply)
+        const bool yslab = lc.xcd_remap && lc.slab_axis == 1 && tiles_y >= 8;
+        const int groups = yslab ? 8 : ((lc.xcd_remap && g.nz >= 32) ? 8 : 1);
+        const int slab = yslab ? g.nz : (g.nz + groups - 1) / groups;
         const int lz = std::max(1, std::min(slab, lc.lz));
-        const int lzs = lc.lz_special > 0 ? std::max(1, std::min(slab, lc.lz_special)) : lz;
+        const int lzs = yslab ? lz : (lc.lz_special > 0 ? std::max(1, std::min(slab, lc.lz_special)) : lz);
         // lists[0]: every item; lists[1]: interior tiles; lists[2]: wall-side tiles
         std::vector<std::vector<uint2>> lists[3];
         for (auto &l : lists) l.resize(groups);
+        auto add = [&](int gi, int t, int z, int z1) {
+            const bool sp = tile_special[t];
+            const int part = (sp || tile_wall[t]) ? 2 : 1;
+            const uint2 it = make_uint2(uint32_t(t) | (sp ? 0x80000000u : 0u), uint32_t(z) | (uint32_t(z1) << 16));
+            lists[0][gi].push_back(it);
+            lists[part][gi].push_back(it);
+        };
         for (int gi = 0; gi < groups; ++gi) {
+            if (yslab) {
+                const int ya = gi * tiles_y / groups, yb = (gi + 1) * tiles_y / groups;
+                for (int z = 0; z < g.nz; z += lz)
+                    for (int pass = 0; pass < 2; ++pass)
+                        for (int t = ya * tiles_x; t < yb * tiles_x; ++t) {
+                            const bool first = lc.special_first ? bool(tile_special[t]) : true;
+                            if (first != (pass == 0) || !tile_live[t]) continue;
+                            add(gi, t, z, std::min(g.nz, z + lz));
+                        }
+                continue;
+            }
             const int za = gi * slab, zb = std::min(g.nz, za + slab);
             for (int pass = 0; pass < 2; ++pass)
                 for (int t = 0; t < ntiles; ++t) {
                     const bool sp = tile_special[t];
-                    if (sp != (pass == 0) || !tile_live[t]) continue;
+                    const bool first = lc.special_first ? sp : true;
+                    if (first != (pass == 0) || !tile_live[t]) continue;
                     const int step = sp ? lzs : lz;
-                    const int part = (sp || tile_wall[t]) ? 2 : 1;
-                    for (int z = za; z < zb; z += step) {
-                        const int z1 = std::min(zb, z + step);
-                        const uint2 it = make_uint2(uint32_t(t) | (sp ? 0x80000000u : 0u),
-                                                    uint32_t(z) | (uint32_t(z1) << 16));
-                        lists[0][gi].push_back(it);
-                        lists[part][gi].push_back(it);
-                    }
+                    for (int z = za; z < zb; z += step) add(gi, t, z, std::min(zb, z + step));
                 }
         }
         std::vector<uint2> items[3];
