@@ -45,7 +45,7 @@
 extern "C" {
 #endif
 
-#define MRLBM_ABI_VERSION 1
+#define MRLBM_ABI_VERSION 2
 
 /* ---- status codes (replace exit(EXIT_FAILURE) in the reference) ---- */
 enum mrlbm_status {
@@ -381,8 +381,10 @@ int mrlbm_sim_info(mrlbm_sim *sim, char *buf, int64_t cap, int64_t *len);
  * mrlbm_batch_plan does the same admission on the host only (explicit budget).
  * mrlbm_batch_case returns a handle owned by the batch: get/set moments, forces, tables, info and
  * checkpoint_write work on it; init, step, time_steps, checkpoint_read and destroy are refused.
- * mrlbm_batch_autotune times trial_steps steps of each launch configuration (tile, march
- * variant, z chunk), keeps the fastest and re-initialises every case.
+ * mrlbm_batch_autotune times trial_steps steps of each launch configuration -- tile, march
+ * variant and z chunk, then for the fastest of those the cases' assignment to the CUs (blocks
+ * per CU of a persistent launch, cases cut into XCD slabs or kept whole, slab axis) -- keeps the
+ * fastest and re-initialises every case.
  */
 typedef struct mrlbm_batch mrlbm_batch;
 
@@ -391,9 +393,13 @@ typedef struct mrlbm_tune_result {
     int32_t variant;        /* 1 ring march, 3 / 4 register-window march with 1 / 2 LDS planes */
     int32_t lz;             /* z chunk */
     int32_t candidates;     /* configurations timed */
-    int32_t reserved;
+    int32_t blocks_per_cu;  /* persistent fused launch: blocks resident per CU, each pulling work
+                               items from its XCD's queue (0: one block per work item) */
     double ms_per_step;         /* chosen configuration */
     double ms_per_step_default; /* default configuration */
+    int32_t packing;        /* cases on the XCDs: 0 every case of 32 planes or more cut into one z
+                               slab per XCD, 1 whole cases on the least loaded XCD, largest first */
+    int32_t slab_axis;      /* XCD slabs of a cut case: 2 z slabs, 1 y slabs (rows of tiles) */
 } mrlbm_tune_result;
 
 int mrlbm_batch_plan(const mrlbm_case_desc *descs, int32_t n, uint64_t hbm_budget_bytes, int32_t *admitted,

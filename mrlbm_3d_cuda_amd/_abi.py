@@ -5,7 +5,7 @@ sizes and offsets against the compiled library's own sizeof reports.
 """
 import ctypes as C
 
-ABI_VERSION = 1
+ABI_VERSION = 2
 
 # status codes
 OK = 0
@@ -129,9 +129,11 @@ class TuneResult(C.Structure):
         ("variant", C.c_int32),
         ("lz", C.c_int32),
         ("candidates", C.c_int32),
-        ("reserved", C.c_int32),
+        ("blocks_per_cu", C.c_int32),
         ("ms_per_step", C.c_double),
         ("ms_per_step_default", C.c_double),
+        ("packing", C.c_int32),
+        ("slab_axis", C.c_int32),
     ]
 
 

@@ -60,30 +60,64 @@ class Batch final : public BatchBase {
             int rc = upload(d_wblocks_[f], order);
             if (rc) return rc;
         }
-        return configure(sim(0).launch_config());
+        // the first case's launch configuration, with the batch's own placement from the environment
+        LaunchConfig lc = sim(0).launch_config();
+        if (const char *e = std::getenv("MRLBM_BATCH_BLOCKS_PER_CU")) lc.blocks_per_cu = std::max(0, std::atoi(e));
+        if (const char *e = std::getenv("MRLBM_BATCH_PACKING")) lc.packing = std::atoi(e) == 1 ? 1 : 0;
+        return configure(lc);
     }
 
-    // Applies one launch configuration to every case and merges their work items: XCD group g
-    // of the launch takes group g of every case that is cut into XCD slabs, and the whole item
-    // list of case c otherwise goes to group c mod 8; boundary-tile items lead each group.
+    // Applies one launch configuration to every case and merges their work items into one list
+    // per XCD group. Packing 0: group g of the launch takes group g of every case that is cut into
+    // XCD slabs, and the whole item list of case c otherwise goes to group c mod 8. Packing 1:
+    // every case stays whole (one item list) on the XCD with the least work so far, largest case
+    // first, so that a case's ring loads share one L2. Boundary-tile items lead each group.
     int configure(const LaunchConfig &lc) override {
         if (!batch_tile_supported(lc.bx, lc.by)) return set_error(MRLBM_ERR_INVALID, "unsupported batch tile");
         HIP_TRY(hipStreamSynchronize(stream_));
         if (prepare_fused_batch<R, S>(lc.bx, lc.by, lc.variant) != 0)
             return set_error(MRLBM_ERR_UNSUPPORTED, "batched fused kernel cannot get its LDS");
+        LaunchConfig per_case = lc;
+        if (lc.packing == 1) per_case.xcd_remap = false;
         for (size_t c = 0; c < cases.size(); ++c)
-            if (int rc = sim(c).configure_launch(lc)) return rc;
+            if (int rc = sim(c).configure_launch(per_case)) return rc;
         const int groups = lc.xcd_remap ? 8 : 1;
+        // destination group of every (case, case group)
+        std::vector<std::vector<int>> dst(cases.size());
+        if (lc.packing == 1 && groups > 1) {
+            // work of a case: nodes of its items (boundary tiles count 1.5 times)
+            std::vector<std::pair<double, size_t>> work;
+            for (size_t c = 0; c < cases.size(); ++c) {
+                const Sim<R, S> &sc = sim(c);
+                double w = 0.0;
+                for (const auto &l : sc.item_lists())
+                    for (const uint2 &it : l)
+                        w += double(int(it.y >> 16) - int(it.y & 0xFFFFu)) * lc.bx * lc.by * ((it.x >> 31) ? 1.5 : 1.0);
+                work.emplace_back(w, c);
+            }
+            std::stable_sort(work.begin(), work.end(), [](const auto &p, const auto &q) { return p.first > q.first; });
+            std::vector<double> load(groups, 0.0);
+            for (const auto &w : work) {
+                const int gmin = int(std::min_element(load.begin(), load.end()) - load.begin());
+                load[gmin] += w.first;
+                dst[w.second].assign(sim(w.second).item_lists().size(), gmin);
+            }
+        } else {
+            for (size_t c = 0; c < cases.size(); ++c) {
+                const size_t ng = sim(c).item_lists().size();
+                dst[c].resize(ng);
+                for (size_t gi = 0; gi < ng; ++gi) dst[c][gi] = ng == 1 ? int(c % groups) : int(gi % groups);
+            }
+        }
         std::vector<std::vector<uint4>> lists(groups);
         for (int pass = 0; pass < 2; ++pass)
             for (size_t c = 0; c < cases.size(); ++c) {
                 const auto &cl = sim(c).item_lists();
                 for (size_t gi = 0; gi < cl.size(); ++gi) {
-                    const int dst = cl.size() == 1 ? int(c % groups) : int(gi % groups);
                     for (const uint2 &it : cl[gi]) {
                         const bool sp = (it.x >> 31) != 0;
                         if ((it.y & 0xFFFFu) >= (it.y >> 16) || sp != (pass == 0)) continue;
-                        lists[dst].push_back(make_uint4(uint32_t(c), it.x, it.y, 0u));
+                        lists[dst[c][gi]].push_back(make_uint4(uint32_t(c), it.x, it.y, 0u));
                     }
                 }
             }
@@ -107,6 +141,17 @@ class Batch final : public BatchBase {
         }
         nitems_ = int(items.size());
         groups_ = groups;
+        // persistent launch: blocks_per_cu blocks on every CU, a multiple of the groups, each group
+        // counted down by its own counters (both parities start at zero)
+        pblocks_ = 0;
+        if (lc.blocks_per_cu > 0) {
+            int dev = 0, ncu = 0;
+            HIP_TRY(hipGetDevice(&dev));
+            HIP_TRY(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+            pblocks_ = (std::max(1, ncu * lc.blocks_per_cu) + groups - 1) / groups * groups;
+        }
+        if (!d_counters_) HIP_TRY(hipMalloc(&d_counters_, 2 * 8 * sizeof(unsigned)));
+        HIP_TRY(hipMemset(d_counters_, 0, 2 * 8 * sizeof(unsigned)));
         launch = lc;
         drop_graphs();
         return MRLBM_OK;
@@ -178,13 +223,45 @@ class Batch final : public BatchBase {
         return rc;
     }
 
-    // Times trial_steps steps (with forces) of every candidate configuration -- tile 32x8, 32x4 or
-    // 16x8 (where its LDS fits), ring or register-window march, z chunks of 4, 8, 16, 32 planes --
-    // and keeps the fastest. The cases' states are scrambled by the trial
-    // steps: the tuner re-initialises them at the end.
+    // Times trial_steps steps (with forces) of every candidate configuration and keeps the
+    // fastest, in two stages: tile 32x8, 32x4 or 16x8 (where its LDS fits) x ring or
+    // register-window march x z chunks of 4, 8, 16, 32 planes; then, around the fastest of those,
+    // the assignment of the cases to the CUs -- one block per work item or a persistent launch with
+    // 1, 2 or 3 blocks per CU, cases cut into XCD slabs (z or y) or kept whole on the least loaded
+    // XCD. The cases' states are scrambled by the trial steps: the tuner re-initialises them at
+    // the end.
     int autotune(int trial_steps, mrlbm_tune_result *out) override {
         if (trial_steps <= 0) return set_error(MRLBM_ERR_INVALID, "trial steps must be positive");
         const LaunchConfig def = launch;  // the configuration in use is timed first
+        int dev = 0, max_lds = 0;
+        HIP_TRY(hipGetDevice(&dev));
+        HIP_TRY(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+        auto same = [](const LaunchConfig &p, const LaunchConfig &q) {
+            return p.bx == q.bx && p.by == q.by && p.variant == q.variant && p.lz == q.lz &&
+                   p.lz_special == q.lz_special && p.blocks_per_cu == q.blocks_per_cu && p.packing == q.packing &&
+                   p.slab_axis == q.slab_axis && p.xcd_remap == q.xcd_remap;
+        };
+        if (int rc = init()) return rc;
+        const int graph_steps = graph_steps_;
+        graph_steps_ = 0;  // trials run eagerly: no graph capture per candidate
+        double best = 1e300, at_default = 0.0;
+        LaunchConfig best_lc = def;
+        int tried = 0;
+        auto trial = [&](const LaunchConfig &lc) -> int {
+            if (int rc = configure(lc)) return rc;
+            double ms = 0.0;
+            int rc = step(2, MRLBM_STEP_FORCES);
+            if (!rc) rc = time_steps(trial_steps, MRLBM_STEP_FORCES, &ms);
+            if (rc) return rc;
+            ms /= trial_steps;
+            ++tried;
+            if (tried == 1) at_default = ms;
+            if (ms < best) {
+                best = ms;
+                best_lc = lc;
+            }
+            return MRLBM_OK;
+        };
         std::vector<LaunchConfig> cands{def};
         const int tiles[3][2] = {{32, 8}, {32, 4}, {16, 8}};
         for (const auto &t : tiles)
@@ -197,41 +274,30 @@ class Batch final : public BatchBase {
                     lc.variant = variant;
                     lc.lz = lz;
                     lc.lz_special = 0;
-                    int dev = 0, max_lds = 0;
-                    HIP_TRY(hipGetDevice(&dev));
-                    HIP_TRY(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
                     if (fused_lds_bytes(lc.bx, lc.by, sizeof(R), lc.variant) > size_t(max_lds)) continue;
-                    if (lc.bx == def.bx && lc.by == def.by && lc.variant == def.variant && lc.lz == def.lz &&
-                        lc.lz_special == def.lz_special)
-                        continue;
-                    cands.push_back(lc);
+                    if (!same(lc, def)) cands.push_back(lc);
                 }
-        if (int rc = init()) return rc;
-        const int graph_steps = graph_steps_;
-        graph_steps_ = 0;  // trials run eagerly: no graph capture per candidate
-        double best = 1e300, at_default = 0.0;
-        LaunchConfig best_lc = def;
-        int tried = 0;
-        for (const LaunchConfig &lc : cands) {
-            if (int rc = configure(lc)) {
+        for (const LaunchConfig &lc : cands)
+            if (int rc = trial(lc)) {
                 graph_steps_ = graph_steps;
                 return rc;
             }
-            double ms = 0.0;
-            int rc = step(2, MRLBM_STEP_FORCES);
-            if (!rc) rc = time_steps(trial_steps, MRLBM_STEP_FORCES, &ms);
-            if (rc) {
-                graph_steps_ = graph_steps;
-                return rc;
-            }
-            ms /= trial_steps;
-            ++tried;
-            if (tried == 1) at_default = ms;
-            if (ms < best) {
-                best = ms;
-                best_lc = lc;
-            }
-        }
+        // second stage: the cases' assignment to the CUs around the fastest configuration
+        const LaunchConfig base = best_lc;
+        for (int packing : {0, 1})
+            for (int axis : {2, 1})
+                for (int bpc : {0, 1, 2, 3}) {
+                    if (packing == 1 && axis == 1) continue;  // whole cases are not cut into slabs
+                    LaunchConfig lc = base;
+                    lc.packing = packing;
+                    lc.slab_axis = axis;
+                    lc.blocks_per_cu = bpc;
+                    if (same(lc, base)) continue;
+                    if (int rc = trial(lc)) {
+                        graph_steps_ = graph_steps;
+                        return rc;
+                    }
+                }
         graph_steps_ = graph_steps;
         if (int rc = configure(best_lc)) return rc;
         if (int rc = init()) return rc;
@@ -241,8 +307,11 @@ class Batch final : public BatchBase {
             out->variant = best_lc.variant;
             out->lz = best_lc.lz;
             out->candidates = tried;
+            out->blocks_per_cu = best_lc.blocks_per_cu;
             out->ms_per_step = best;
             out->ms_per_step_default = at_default;
+            out->packing = best_lc.packing;
+            out->slab_axis = best_lc.slab_axis;
         }
         return MRLBM_OK;
     }
@@ -256,8 +325,9 @@ class Batch final : public BatchBase {
 
     // One step from buffer parity `cur` (the caller advances the parity).
     void queue_step(uint32_t flags, int cur) {
+        const BatchQueue q{pblocks_, d_counters_, cur};
         launch_fused_batch<R, S>(d_fargs_[cur], int(cases.size()), d_items_, nitems_, groups_, launch.bx, launch.by,
-                                 launch.variant, stream_);
+                                 launch.variant, q, stream_);
         const int f = (flags & MRLBM_STEP_FORCES) ? 1 : 0;
         if (nwblocks_[f] > 0)
             launch_wall_batch<R, S>(d_wargs_[cur][f], d_wblocks_[f], nwblocks_[f], stream_);
@@ -304,7 +374,7 @@ class Batch final : public BatchBase {
         if (stream_) (void)hipStreamSynchronize(stream_);
         drop_graphs();
         void *ptrs[] = {d_fargs_[0], d_fargs_[1], d_wargs_[0][0], d_wargs_[0][1], d_wargs_[1][0], d_wargs_[1][1],
-                        d_wblocks_[0], d_wblocks_[1], d_items_};
+                        d_wblocks_[0], d_wblocks_[1], d_items_, d_counters_};
         for (void *p : ptrs)
             if (p) (void)hipFree(p);
         for (auto &c : cases) c->impl->batch_stream = nullptr;
@@ -321,6 +391,8 @@ class Batch final : public BatchBase {
     int nwblocks_[2] = {0, 0};
     uint4 *d_items_ = nullptr;
     int nitems_ = 0, groups_ = 1;
+    int pblocks_ = 0;                  // persistent fused launch: resident blocks (0: one per item)
+    unsigned *d_counters_ = nullptr;   // its work-queue counters, 2 parities x 8 groups
     hipGraphExec_t graphs_[8] = {};
 };
 
@@ -454,6 +526,9 @@ int mrlbm_batch_get_launch(mrlbm_batch *b, mrlbm_tune_result *out) {
     out->by = b->impl->launch.by;
     out->variant = b->impl->launch.variant;
     out->lz = b->impl->launch.lz;
+    out->blocks_per_cu = b->impl->launch.blocks_per_cu;
+    out->packing = b->impl->launch.packing;
+    out->slab_axis = b->impl->launch.slab_axis;
     return MRLBM_OK;
 }
 

@@ -175,9 +175,19 @@ void launch_record(const RecordArgs &r, hipStream_t stream);
 // special << 31, z0 | z1 << 16, 0}; tiles BX x BY in {32x8, 16x8, 32x4}) plus one block per case
 // that files the case's pending record, and one wall pass whose block b runs block blocks[b].y
 // of case blocks[b].x. Argument arrays live on the device.
+// Work queues of a persistent batched launch: pblocks resident blocks (a multiple of the XCD
+// groups; block b serves group b mod groups, on XCD b mod 8) pull their group's work items in
+// order through one counter per group. A launch uses the counters of its buffer parity and zeroes
+// the other parity's, which the next step uses. pblocks = 0: one block per work item.
+struct BatchQueue {
+    int pblocks;
+    unsigned *counters;  // 2 x groups
+    int parity;
+};
+
 template <class R, class S>
 void launch_fused_batch(const FusedArgs<R, S> *cases, int ncases, const uint4 *items, int nitems, int groups, int bx,
-                        int by, int variant, hipStream_t stream);
+                        int by, int variant, const BatchQueue &queue, hipStream_t stream);
 
 // Files the pending records of the cases (one block per case).
 template <class R, class S>
@@ -223,6 +233,8 @@ struct LaunchConfig {
     int lz = 32;            // z chunk of a work item
     int lz_special = 0;     // z chunk of boundary tiles (0: lz)
     bool xcd_remap = true;  // one slab per XCD
+    int blocks_per_cu = 0;  // batches: persistent fused launch with this many blocks per CU (0: one block per item)
+    int packing = 0;        // batches: 0 deep cases cut into XCD slabs, 1 whole cases on the least loaded XCD
     int slab_axis = 2;      // axis cut into the XCD slabs: 2 = z, 1 = y (rows of tiles, z chunks in rounds)
     bool special_first = true;  // boundary tiles first (in a z slab, or in a z round of a y slab)
 };

@@ -705,21 +705,10 @@ __global__ __launch_bounds__(BX *BY) void memory_twin_kernel(FusedArgs<R, S> a) 
     if ((fold ^ fold_bits(c0) ^ fold_bits(c1)) & keep) store_raw<S>(a.dst, a.pad, c0);
 }
 
-// Fused pass of a batch: work item {case, tile | special << 31, z0 | z1 << 16, 0}; the case's
-// arguments come from a device array. VARIANT as in launch_fused.
+// One work item of a batch {case, tile | special << 31, z0 | z1 << 16, 0}; the case's arguments
+// come from a device array. VARIANT as in launch_fused.
 template <class R, class S, int BX, int BY, int VARIANT>
-__global__ __launch_bounds__(BX *BY, (VARIANT == 3 ? 3 : 2)) void fused_batch_kernel(
-    const FusedArgs<R, S> *__restrict__ cases, const uint4 *__restrict__ items, int nitems, int groups) {
-    extern __shared__ unsigned char lds_raw[];
-    R *lds = reinterpret_cast<R *>(lds_raw);
-    int b = blockIdx.x;
-    if (b >= nitems) {  // blocks after the work items file the cases' previous records
-        const RecordArgs r = cases[b - nitems].rec;
-        if (r.pending && threadIdx.x < 64) file_record(r);
-        return;
-    }
-    if (groups > 1) b = (b % groups) * (nitems / groups) + b / groups;
-    const uint4 item = items[b];
+__device__ __forceinline__ void batch_item(const FusedArgs<R, S> *__restrict__ cases, const uint4 &item, R *lds) {
     const int z0 = int(item.z & 0xFFFFu);
     const int z1 = int(item.z >> 16);
     if (z0 >= z1) return;  // padding item (uniform per block)
@@ -737,6 +726,44 @@ __global__ __launch_bounds__(BX *BY, (VARIANT == 3 ? 3 : 2)) void fused_batch_ke
             window_march<R, S, BX, BY, NBUF, true>(a, lds, tile, z0, z1);
         else
             window_march<R, S, BX, BY, NBUF, false>(a, lds, tile, z0, z1);
+    }
+}
+
+// Fused pass of a batch: one block per work item, or q.pblocks persistent blocks that pull the
+// items of their XCD group through the group's counter until the group's list is exhausted (every
+// block stops at the end of the list; no block waits for another). Then one block per case that
+// files the case's previous record.
+template <class R, class S, int BX, int BY, int VARIANT>
+__global__ __launch_bounds__(BX *BY, (VARIANT == 3 ? 3 : 2)) void fused_batch_kernel(
+    const FusedArgs<R, S> *__restrict__ cases, const uint4 *__restrict__ items, int nitems, int groups, BatchQueue q) {
+    extern __shared__ unsigned char lds_raw[];
+    R *lds = reinterpret_cast<R *>(lds_raw);
+    int b = blockIdx.x;
+    const int nwork = q.pblocks > 0 ? q.pblocks : nitems;
+    if (b >= nwork) {  // blocks after the work blocks file the cases' previous records
+        const RecordArgs r = cases[b - nwork].rec;
+        if (r.pending && threadIdx.x < 64) file_record(r);
+        return;
+    }
+    if (q.pblocks == 0) {
+        if (groups > 1) b = (b % groups) * (nitems / groups) + b / groups;
+        batch_item<R, S, BX, BY, VARIANT>(cases, items[b], lds);
+        return;
+    }
+    // the next step (the other buffer parity) starts from zeroed counters
+    if (b == 0 && int(threadIdx.x) < groups) q.counters[(q.parity ^ 1) * groups + int(threadIdx.x)] = 0u;
+    const int g = b % groups;
+    const int len = nitems / groups;
+    unsigned *const counter = q.counters + q.parity * groups + g;
+    __shared__ int next;
+    for (;;) {
+        if (threadIdx.x == 0) next = int(atomicAdd(counter, 1u));
+        __syncthreads();
+        const int k = next;
+        __syncthreads();  // every thread has read the index before thread 0 takes the next one
+        if (k >= len) break;
+        batch_item<R, S, BX, BY, VARIANT>(cases, items[g * len + k], lds);
+        __syncthreads();  // the item's LDS reads are done before the next item's writes
     }
 }
 
@@ -1066,7 +1093,8 @@ void launch_fused(const FusedArgs<R, S> &a, int bx, int by, int variant, int par
 
 // Kernel of the batched fused pass for a tile and variant (null if not instantiated).
 template <class R, class S>
-auto fused_batch_kernel_for(int bx, int by, int variant) -> void (*)(const FusedArgs<R, S> *, const uint4 *, int, int) {
+auto fused_batch_kernel_for(int bx, int by, int variant)
+    -> void (*)(const FusedArgs<R, S> *, const uint4 *, int, int, BatchQueue) {
 #define MRLBM_BATCH_TILE(TX, TY)                                                     \
     if (bx == TX && by == TY) {                                                      \
         if (variant == 3) return fused_batch_kernel<R, S, TX, TY, 3>;                \
@@ -1094,11 +1122,13 @@ int prepare_fused_batch(int bx, int by, int variant) {
 
 template <class R, class S>
 void launch_fused_batch(const FusedArgs<R, S> *cases, int ncases, const uint4 *items, int nitems, int groups, int bx,
-                        int by, int variant, hipStream_t stream) {
+                        int by, int variant, const BatchQueue &queue, hipStream_t stream) {
     const size_t lds = fused_lds_bytes(bx, by, sizeof(R), variant);
     auto kernel = fused_batch_kernel_for<R, S>(bx, by, variant);
+    const int nwork = queue.pblocks > 0 ? queue.pblocks : nitems;
     if (kernel)
-        hipLaunchKernelGGL(kernel, dim3(nitems + ncases), dim3(bx * by), lds, stream, cases, items, nitems, groups);
+        hipLaunchKernelGGL(kernel, dim3(nwork + ncases), dim3(bx * by), lds, stream, cases, items, nitems, groups,
+                           queue);
 }
 
 template <class R, class S>
@@ -1187,7 +1217,7 @@ void launch_decode(const FieldSet<const S> &in, double *out, uint32_t n, const S
     template void launch_pressure<R, S>(const PressureArgs<R, S> &, hipStream_t);                           \
     template void launch_planes<R, S>(const PlaneArgs<R, S> &, hipStream_t);                                 \
     template void launch_fused_batch<R, S>(const FusedArgs<R, S> *, int, const uint4 *, int, int, int, int, int, \
-                                           hipStream_t);                                                    \
+                                           const BatchQueue &, hipStream_t);                                \
     template void launch_record_batch<R, S>(const FusedArgs<R, S> *, int, hipStream_t);                      \
     template int prepare_fused_batch<R, S>(int, int, int);                                                  \
     template void launch_wall_batch<R, S>(const WallArgs<R, S> *, const int2 *, int, hipStream_t);            \

@@ -351,12 +351,13 @@ class Batch:
     def autotune(self, trial_steps=10):
         r = TuneResult()
         _check(self.lib.mrlbm_batch_autotune(self.h, trial_steps, C.byref(r)))
-        return {k: getattr(r, k) for k, _ in r._fields_ if k != "reserved"}
+        return {k: getattr(r, k) for k, _ in r._fields_}
 
     def launch(self):
         r = TuneResult()
         _check(self.lib.mrlbm_batch_get_launch(self.h, C.byref(r)))
-        return {"bx": r.bx, "by": r.by, "variant": r.variant, "lz": r.lz}
+        return {"bx": r.bx, "by": r.by, "variant": r.variant, "lz": r.lz, "blocks_per_cu": r.blocks_per_cu,
+                "packing": r.packing, "slab_axis": r.slab_axis}
 
 
 def algorithmic_bytes_per_update(precision):

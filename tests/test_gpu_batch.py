@@ -70,6 +70,32 @@ def test_batch_equals_standalone_runs(precision):
             check_same(got[i][3], pm, f"case {i} plane means")
 
 
+@pytest.mark.parametrize("precision", [_abi.FP64, _abi.FP32_BF16])
+@pytest.mark.parametrize("blocks_per_cu,packing", [(1, 0), (2, 0), (3, 1), (0, 1)])
+def test_batch_placement_equals_standalone_runs(monkeypatch, blocks_per_cu, packing, precision):
+    """Persistent launches (blocks pulling work items from their XCD's queue) and whole-case
+    packing on the least loaded XCD give every case the state and force records of its
+    standalone run, bit for bit, over graph replays and eager steps."""
+    from mrlbm_3d_cuda_amd.solver import Batch
+    descs = mixed_cases(precision)
+    steps = 37
+    monkeypatch.setenv("MRLBM_BATCH_BLOCKS_PER_CU", str(blocks_per_cu))
+    monkeypatch.setenv("MRLBM_BATCH_PACKING", str(packing))
+    print(f"[gpu batch] {_abi.PRECISION_NAMES[precision]}: blocks per CU {blocks_per_cu}, packing {packing}",
+          flush=True)
+    with Batch(descs) as b:
+        launch = b.launch()
+        assert launch["blocks_per_cu"] == blocks_per_cu and launch["packing"] == packing
+        b.init().step(steps, forces=True)
+        got = [(c.fields(), c.forces(0, steps)) for c in b.cases]
+    monkeypatch.delenv("MRLBM_BATCH_BLOCKS_PER_CU")
+    monkeypatch.delenv("MRLBM_BATCH_PACKING")
+    for i, d in enumerate(descs):
+        F, f, _, _ = standalone(d, steps, stats=False)
+        check_same(got[i][0], F, f"case {i} fields")
+        check_same(got[i][1], f, f"case {i} forces")
+
+
 def test_batch_autotune_keeps_results(monkeypatch):
     """The tuner times every candidate launch configuration, re-initialises the cases and keeps
     the fastest; results after tuning equal standalone runs with the same launch configuration
@@ -84,7 +110,7 @@ def test_batch_autotune_keeps_results(monkeypatch):
         assert r["candidates"] >= 24
         assert (r["bx"], r["by"]) in [(32, 8), (32, 4), (16, 8)] and r["variant"] in (1, 3, 4)
         assert 0 < r["ms_per_step"] <= r["ms_per_step_default"] * 1.0001
-        assert b.launch() == {k: r[k] for k in ("bx", "by", "variant", "lz")}
+        assert b.launch() == {k: r[k] for k in ("bx", "by", "variant", "lz", "blocks_per_cu", "packing", "slab_axis")}
         assert all(c.info().steps_done == 0 for c in b.cases)
         b.step(20, forces=True)
         got = [(c.fields(), c.forces(0, 20)) for c in b.cases]
