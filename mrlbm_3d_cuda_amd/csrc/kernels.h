@@ -236,7 +236,7 @@ struct LaunchConfig {
     int blocks_per_cu = 0;  // batches: persistent fused launch with this many blocks per CU (0: one block per item)
     int packing = 0;        // batches: 0 deep cases cut into XCD slabs, 1 whole cases on the least loaded XCD
     int slab_axis = 2;      // axis cut into the XCD slabs: 2 = z, 1 = y (rows of tiles, z chunks in rounds)
-    bool special_first = true;  // boundary tiles first (in a z slab, or in a z round of a y slab)
+    int special_order = 0;  // boundary tiles: 0 first, 1 in tile order, 2 last (in a z slab; in a y slab per z round, or after every round)
 };
 
 // Tile shapes (BX x BY) the fused kernel is instantiated for.

@@ -160,7 +160,7 @@ class Sim final : public SimBase {
         if (const char *e = std::getenv("MRLBM_LZ_SPECIAL")) lc.lz_special = std::max(1, std::atoi(e));
         if (const char *e = std::getenv("MRLBM_XCD_REMAP")) lc.xcd_remap = std::atoi(e) != 0;
         if (const char *e = std::getenv("MRLBM_SLAB_AXIS")) lc.slab_axis = (e[0] == 'y' || e[0] == '1') ? 1 : 2;
-        if (const char *e = std::getenv("MRLBM_SPECIAL_FIRST")) lc.special_first = std::atoi(e) != 0;
+        if (const char *e = std::getenv("MRLBM_SPECIAL_ORDER")) lc.special_order = std::max(0, std::min(2, std::atoi(e)));
         if (const char *e = std::getenv("MRLBM_STOCHASTIC_ROUNDING")) stochastic_ = Conv<S>::scaled && std::atoi(e) != 0;
         if (const char *e = std::getenv("MRLBM_GRAPH_STEPS")) graph_steps_ = std::max(0, std::atoi(e) / 2 * 2);
         if (const char *e = std::getenv("MRLBM_TWO_STREAM")) two_stream_env_ = std::atoi(e) != 0 ? 1 : 0;
@@ -408,25 +408,26 @@ class Sim final : public SimBase {
             lists[0][gi].push_back(it);
             lists[part][gi].push_back(it);
         };
+        // pass of a tile in its slab (or z round): boundary tiles in pass 0 (first), in pass 1 with
+        // the others (tile order), or in pass 2 (last; in a y slab after every round)
+        auto pass_of = [&](int t) { return tile_special[t] ? lc.special_order : 1; };
         for (int gi = 0; gi < groups; ++gi) {
             if (yslab) {
-                const int ya = gi * tiles_y / groups, yb = (gi + 1) * tiles_y / groups;
-                for (int z = 0; z < g.nz; z += lz)
+                const int ta = (gi * tiles_y / groups) * tiles_x, tb = ((gi + 1) * tiles_y / groups) * tiles_x;
+                for (int z = 0; z < g.nz; z += lz)  // the rounds: passes 0 and 1
                     for (int pass = 0; pass < 2; ++pass)
-                        for (int t = ya * tiles_x; t < yb * tiles_x; ++t) {
-                            const bool first = lc.special_first ? bool(tile_special[t]) : true;
-                            if (first != (pass == 0) || !tile_live[t]) continue;
-                            add(gi, t, z, std::min(g.nz, z + lz));
-                        }
+                        for (int t = ta; t < tb; ++t)
+                            if (pass_of(t) == pass && tile_live[t]) add(gi, t, z, std::min(g.nz, z + lz));
+                for (int z = 0; z < g.nz; z += lz)  // then pass 2, after every round
+                    for (int t = ta; t < tb; ++t)
+                        if (pass_of(t) == 2 && tile_live[t]) add(gi, t, z, std::min(g.nz, z + lz));
                 continue;
             }
             const int za = gi * slab, zb = std::min(g.nz, za + slab);
-            for (int pass = 0; pass < 2; ++pass)
+            for (int pass = 0; pass < 3; ++pass)
                 for (int t = 0; t < ntiles; ++t) {
-                    const bool sp = tile_special[t];
-                    const bool first = lc.special_first ? sp : true;
-                    if (first != (pass == 0) || !tile_live[t]) continue;
-                    const int step = sp ? lzs : lz;
+                    if (pass_of(t) != pass || !tile_live[t]) continue;
+                    const int step = tile_special[t] ? lzs : lz;
                     for (int z = za; z < zb; z += step) add(gi, t, z, std::min(zb, z + step));
                 }
         }

@@ -289,7 +289,7 @@ def test_two_stream_schedule_is_bitwise_neutral(kind, precision):
 @pytest.mark.parametrize("kind", ["channel", "annulus", "closed_channel"])
 def test_slab_axis_is_bitwise_neutral(kind, precision):
     """Tile work items cut into y slabs (rows of tiles marched in z rounds) instead of z slabs,
-    boundary tiles first or not: the same state, force records, plane means and wall pressure,
+    boundary tiles first, in tile order or last: the same state, force records, plane means and wall pressure,
     bit for bit, over eager steps, graph replays and statistics steps; FP64 also equals the
     oracle."""
     from mrlbm_3d_cuda_amd.solver import Simulation
@@ -303,16 +303,16 @@ def test_slab_axis_is_bitwise_neutral(kind, precision):
         if kind == "closed_channel":
             d = closed_z(d)
     out = []
-    for axis, lz, first in (("z", None, "1"), ("y", "8", "1"), ("y", "64", "0"), ("z", None, "0")):
-        print(f"[gpu parity] {kind} slab axis {axis} lz {lz} boundary tiles first {first}", flush=True)
+    for axis, lz, order in (("z", None, "0"), ("y", "8", "0"), ("y", "64", "1"), ("z", None, "2"), ("y", "16", "2")):
+        print(f"[gpu parity] {kind} slab axis {axis} lz {lz} boundary tile order {order}", flush=True)
         os.environ["MRLBM_SLAB_AXIS"] = axis
-        os.environ["MRLBM_SPECIAL_FIRST"] = first
+        os.environ["MRLBM_SPECIAL_ORDER"] = order
         if lz:
             os.environ["MRLBM_LZ"] = lz
         try:
             s = Simulation(d).init()
         finally:
-            del os.environ["MRLBM_SLAB_AXIS"], os.environ["MRLBM_SPECIAL_FIRST"]
+            del os.environ["MRLBM_SLAB_AXIS"], os.environ["MRLBM_SPECIAL_ORDER"]
             os.environ.pop("MRLBM_LZ", None)
         s.step(3, forces=True).step(32, forces=True).step(17, forces=True, stats=True).step(8)
         r = [s.fields(), s.forces(0, 52), s.surface_pressure()[0]]
