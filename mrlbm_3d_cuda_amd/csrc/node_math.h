@@ -226,36 +226,64 @@ __device__ __forceinline__ void reconstruct_lds(const R *M, R *g) {
     }
 }
 
-// Moments of the populations held in LDS (eval_moments.cuh:5-22).
+// Direction q of velocity (cx, cy, cz), indexed [cx + 1][cy + 1][cz + 1].
+struct QIndex {
+    int q[3][3][3];
+};
+constexpr QIndex make_qindex() {
+    QIndex t{};
+    for (int q = 0; q < Q; ++q) t.q[LATTICE.cx[q] + 1][LATTICE.cy[q] + 1][LATTICE.cz[q] + 1] = q;
+    return t;
+}
+constexpr QIndex QOF = make_qindex();
+
+// Moments of the populations held in LDS (eval_moments.cuh:5-22). FP32: the D3Q27 weights are
+// products of the 1-D weights w1(0) = 2/3, w1(+-1) = 1/6, so the weighted sums factor into three
+// passes of 1-D sums (x, then y, then z), each of which keeps, per line, the zeroth, first and
+// second moment times 6: (g- + g+) + 4 g0, g+ - g-, g+ + g-. Pruned to the ten moments needed
+// (order <= 2 in every pair of axes) that is 66 additions; the common factor 6^3 = 216 cancels in
+// u and m and is divided out of rho.
 template <class R>
 __device__ __forceinline__ void moments_lds(const R *g, R *M) {
     if constexpr (kExactOrder<R>) {
         moments(g, M);
     } else {
-        R s[13], d[13];
+        // x pass: per (cy, cz) line
+        R X0[3][3], X1[3][3], X2[3][3];
 #pragma unroll
-        for (int k = 0; k < 13; ++k) {
-            s[k] = g[2 * k + 1] + g[2 * k + 2];
-            d[k] = g[2 * k + 1] - g[2 * k + 2];
+        for (int b = 0; b < 3; ++b)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const R gm = g[QOF.q[0][b][c]], g0 = g[QOF.q[1][b][c]], gp = g[QOF.q[2][b][c]];
+                X2[b][c] = gp + gm;
+                X1[b][c] = gp - gm;
+                X0[b][c] = R(4) * g0 + X2[b][c];
+            }
+        // y pass: per cz, the (x, y) orders 00, 01, 02, 10, 11, 20
+        R Y00[3], Y01[3], Y02[3], Y10[3], Y11[3], Y20[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            Y02[c] = X0[2][c] + X0[0][c];
+            Y00[c] = R(4) * X0[1][c] + Y02[c];
+            Y01[c] = X0[2][c] - X0[0][c];
+            Y10[c] = R(4) * X1[1][c] + (X1[2][c] + X1[0][c]);
+            Y11[c] = X1[2][c] - X1[0][c];
+            Y20[c] = R(4) * X2[1][c] + (X2[2][c] + X2[0][c]);
         }
-        // pair k: 0 x, 1 y, 2 z, 3 (1,1,0), 4 (1,0,1), 5 (0,1,1), 6 (1,-1,0), 7 (1,0,-1),
-        // 8 (0,1,-1), 9 (1,1,1), 10 (1,1,-1), 11 (1,-1,1), 12 (-1,1,1)
-        const R w0 = R(W_REST), w1 = R(W_AXIS), w2 = R(W_FACE), w3 = R(W_CORNER);
-        const R C = (s[9] + s[10]) + (s[11] + s[12]);
-        const R rho = w0 * g[0] + w1 * ((s[0] + s[1]) + s[2]) +
-                      w2 * (((s[3] + s[4]) + (s[5] + s[6])) + (s[7] + s[8])) + w3 * C;
-        const R jx = w1 * d[0] + w2 * ((d[3] + d[4]) + (d[6] + d[7])) + w3 * ((d[9] + d[10]) + (d[11] - d[12]));
-        const R jy = w1 * d[1] + w2 * ((d[3] + d[5]) + (d[8] - d[6])) + w3 * ((d[9] + d[10]) + (d[12] - d[11]));
-        const R jz = w1 * d[2] + w2 * ((d[4] + d[5]) - (d[7] + d[8])) + w3 * ((d[9] + d[11]) + (d[12] - d[10]));
-        const R pxx = w1 * s[0] + w2 * ((s[3] + s[4]) + (s[6] + s[7])) + w3 * C;
-        const R pyy = w1 * s[1] + w2 * ((s[3] + s[5]) + (s[6] + s[8])) + w3 * C;
-        const R pzz = w1 * s[2] + w2 * ((s[4] + s[5]) + (s[7] + s[8])) + w3 * C;
-        const R pxy = w2 * (s[3] - s[6]) + w3 * ((s[9] + s[10]) - (s[11] + s[12]));
-        const R pxz = w2 * (s[4] - s[7]) + w3 * ((s[9] + s[11]) - (s[10] + s[12]));
-        const R pyz = w2 * (s[5] - s[8]) + w3 * ((s[9] + s[12]) - (s[10] + s[11]));
+        // z pass: the ten moments times 216
+        const R pzz = Y00[2] + Y00[0];
+        const R rho = R(4) * Y00[1] + pzz;
+        const R jz = Y00[2] - Y00[0];
+        const R jy = R(4) * Y01[1] + (Y01[2] + Y01[0]);
+        const R pyz = Y01[2] - Y01[0];
+        const R pyy = R(4) * Y02[1] + (Y02[2] + Y02[0]);
+        const R jx = R(4) * Y10[1] + (Y10[2] + Y10[0]);
+        const R pxz = Y10[2] - Y10[0];
+        const R pxy = R(4) * Y11[1] + (Y11[2] + Y11[0]);
+        const R pxx = R(4) * Y20[1] + (Y20[2] + Y20[0]);
         const R inv = fast_rcp(rho);
         const R cs2 = R(1) / R(3);
-        M[0] = rho;
+        M[0] = rho * R(1.0 / 216.0);
         M[1] = jx * inv;
         M[2] = jy * inv;
         M[3] = jz * inv;
