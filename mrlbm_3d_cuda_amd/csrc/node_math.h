@@ -190,6 +190,21 @@ __device__ __forceinline__ void moments(const R *f, R *M) {
 // products are dropped from the reconstruction and applied once per lattice class (rest, axis,
 // face diagonal, corner) when the moments are summed.
 
+// Direction q of velocity (cx, cy, cz), indexed [cx + 1][cy + 1][cz + 1].
+struct QIndex {
+    int q[3][3][3];
+};
+constexpr QIndex make_qindex() {
+    QIndex t{};
+    for (int q = 0; q < Q; ++q) t.q[LATTICE.cx[q] + 1][LATTICE.cy[q] + 1][LATTICE.cz[q] + 1] = q;
+    return t;
+}
+constexpr QIndex QOF = make_qindex();
+
+// Regularised populations in the LDS form, g = f / w (core/D3Q27/second_order/reconstruction.cuh:
+// 5-60 as a Hermite sum). FP32: for fixed (cy, cz), g is a quadratic in cx, g = A + B cx + Dx cx^2,
+// and A (a quadratic in cy and cz) and B (linear in cy and cz) are built the same way, so the 27
+// values take 3 operations per x line after 22 for the coefficients.
 template <class R>
 __device__ __forceinline__ void reconstruct_lds(const R *M, R *g) {
     if constexpr (kExactOrder<R>) {
@@ -201,41 +216,26 @@ __device__ __forceinline__ void reconstruct_lds(const R *M, R *g) {
         const R Ux = r3 * M[1], Uy = r3 * M[2], Uz = r3 * M[3];
         const R Dx = r45 * M[4], Dy = r45 * M[5], Dz = r45 * M[6];
         const R Oxy = r9 * M[7], Oxz = r9 * M[8], Oyz = r9 * M[9];
-        g[0] = P;
-        auto pair = [&](int q, R S, R V) {
-            g[q] = S + V;
-            g[q + 1] = S - V;
-        };
-        const R Px = P + Dx;
-        pair(1, Px, Ux);
-        pair(3, P + Dy, Uy);
-        pair(5, P + Dz, Uz);
-        const R Pxy = Px + Dy, Pxz = Px + Dz, Pyz = (P + Dy) + Dz, Pxyz = Pxy + Dz;
-        const R Uxpy = Ux + Uy, Uxmy = Ux - Uy;
-        pair(7, Pxy + Oxy, Uxpy);
-        pair(9, Pxz + Oxz, Ux + Uz);
-        pair(11, Pyz + Oyz, Uy + Uz);
-        pair(13, Pxy - Oxy, Uxmy);
-        pair(15, Pxz - Oxz, Ux - Uz);
-        pair(17, Pyz - Oyz, Uy - Uz);
-        const R a = Pxyz + Oxy, b = Pxyz - Oxy, o1 = Oxz + Oyz, o2 = Oxz - Oyz;
-        pair(19, a + o1, Uxpy + Uz);
-        pair(21, a - o1, Uxpy - Uz);
-        pair(23, b + o2, Uxmy + Uz);
-        pair(25, b - o2, (Uy + Uz) - Ux);
+        // A(cy, cz) = A0(cz) + cy Byz(cz) + cy^2 Dy; B(cy, cz) = Bx(cz) + cy Oxy
+        const R Pz = P + Dz;
+        const R A0[3] = {Pz - Uz, P, Pz + Uz};
+        const R Byz[3] = {Uy - Oyz, Uy, Uy + Oyz};
+        const R Bx[3] = {Ux - Oxz, Ux, Ux + Oxz};
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const R Ay = A0[c] + Dy;
+            const R A[3] = {Ay - Byz[c], A0[c], Ay + Byz[c]};
+            const R B[3] = {Bx[c] - Oxy, Bx[c], Bx[c] + Oxy};
+#pragma unroll
+            for (int b = 0; b < 3; ++b) {
+                const R t = A[b] + Dx;
+                g[QOF.q[0][b][c]] = t - B[b];
+                g[QOF.q[1][b][c]] = A[b];
+                g[QOF.q[2][b][c]] = t + B[b];
+            }
+        }
     }
 }
-
-// Direction q of velocity (cx, cy, cz), indexed [cx + 1][cy + 1][cz + 1].
-struct QIndex {
-    int q[3][3][3];
-};
-constexpr QIndex make_qindex() {
-    QIndex t{};
-    for (int q = 0; q < Q; ++q) t.q[LATTICE.cx[q] + 1][LATTICE.cy[q] + 1][LATTICE.cz[q] + 1] = q;
-    return t;
-}
-constexpr QIndex QOF = make_qindex();
 
 // Moments of the populations held in LDS (eval_moments.cuh:5-22). FP32: the D3Q27 weights are
 // products of the 1-D weights w1(0) = 2/3, w1(+-1) = 1/6, so the weighted sums factor into three
