@@ -22,6 +22,15 @@ struct RecordArgs {
     int nblocks;                               // wall blocks of a step with forces
 };
 
+// Work item tile field: tile index | face kind << 28 | special << 31. A special tile (one that
+// holds boundary or donor nodes) whose only boundary nodes form one domain face -- the inlet or
+// outlet column or a y-wall row, without donors -- has a face kind, and runs the plain march with
+// that face's closed form on its face lanes.
+constexpr uint32_t kItemSpecial = 0x80000000u;
+constexpr uint32_t kItemTileMask = 0x0FFFFFFFu;
+constexpr int kItemFaceShift = 28;
+constexpr int FACE_NONE = 0, FACE_WEST = 1, FACE_EAST = 2, FACE_NORTH = 3, FACE_SOUTH = 4;
+
 // Arguments of the fused stream/collide pass over one ping-pong buffer pair.
 template <class R, class S>
 struct FusedArgs {
@@ -33,8 +42,9 @@ struct FusedArgs {
     int tiles_x;     // tiles along x
     int nblocks;     // work items, one block each
     int xcd_groups;  // blocks are spread round-robin over this many XCDs (1: no remap)
-    // work item b: x = tile index | (special << 31), y = z0 | (z1 << 16); special items (tiles
-    // with boundary nodes, which need info[]) come first and march shorter z ranges
+    // work item b: x = tile field (tile index, face kind, special flag), y = z0 | (z1 << 16);
+    // special items (tiles with boundary nodes, which need info[]) come first and may march
+    // shorter z ranges
     const uint2 *__restrict__ items;
     const uint32_t *__restrict__ info;  // per-node NodeCode | flags | payload
     const R *__restrict__ wall_nx;      // wall_count entries (at least 1)
@@ -236,6 +246,7 @@ struct LaunchConfig {
     int blocks_per_cu = 0;  // batches: persistent fused launch with this many blocks per CU (0: one block per item)
     int packing = 0;        // batches: 0 deep cases cut into XCD slabs, 1 whole cases on the least loaded XCD
     int slab_axis = 2;      // axis cut into the XCD slabs: 2 = z, 1 = y (rows of tiles, z chunks in rounds)
+    bool face_tiles = true;  // face marches for tiles whose only boundary nodes form one face
     int special_order = 0;  // boundary tiles: 0 first, 1 in tile order, 2 last (in a z slab; in a y slab per z round, or after every round)
 };
 

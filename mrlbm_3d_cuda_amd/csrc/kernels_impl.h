@@ -428,8 +428,13 @@ __device__ __forceinline__ void fused_march(const FusedArgs<R, S> &a, R *lds, in
 // unrolled by two so the two c_z=+1 windows swap roles without register moves. Memory
 // operations are ordered and made unconditional as in fused_march. The prefetch past the item's
 // last halo plane z1 re-reads plane z1 (an L2 hit) instead of fetching a plane nobody uses.
-template <class R, class S, int BX, int BY, int NBUF, bool SPECIAL>
+// FACE (with SPECIAL false): a tile whose only boundary nodes form one domain face runs the plain
+// march, and its face lanes replace the moments of their pulled populations with the face's
+// closed form before the collision (the density and velocity of the node and, at the outlet, of its
+// x-1 neighbour are kept from the decode of the plane, the x-1 values by lane shuffle).
+template <class R, class S, int BX, int BY, int NBUF, bool SPECIAL, int FACE = FACE_NONE>
 __device__ __forceinline__ void window_march(const FusedArgs<R, S> &a, R *lds, int tile, int z0, int z1) {
+    static_assert(!SPECIAL || FACE == FACE_NONE, "a face march runs the plain path");
     using F = Frame<BX, BY>;
     constexpr int W = F::W;
     constexpr int NH = F::NH;
@@ -461,9 +466,17 @@ __device__ __forceinline__ void window_march(const FusedArgs<R, S> &a, R *lds, i
     };
     // windows, per position: c_z=+1 groups of two planes (A, B), c_z=0 group of one plane
     R pa0[9], pb0[9], q0[9], pa1[9], pb1[9], q1[9];
-    auto recon = [&](const RawNode<S> &raw, R *f, bool west, int zp) {
+    // face march: density and velocity of the own node of the plane being updated and of the next
+    R fo_cur[4] = {R(0), R(0), R(0), R(0)}, fo_nx[4] = {R(0), R(0), R(0), R(0)};
+    auto recon = [&](const RawNode<S> &raw, R *f, bool west, int zp, R *keep = nullptr) {
         R m[10];
         decode_node<R, S>(raw, a.scale, a.rho0, m);
+        if (FACE == FACE_WEST || FACE == FACE_EAST) {
+            if (keep) {
+#pragma unroll
+                for (int k = 0; k < 4; ++k) keep[k] = m[k];
+            }
+        }
         if (SPECIAL && west) {
 #pragma unroll
             for (int k = 0; k < 10; ++k) wold[(west_slot(zp) * 10 + k) * BY + (hy1 - 1)] = m[k];
@@ -487,7 +500,7 @@ __device__ __forceinline__ void window_march(const FusedArgs<R, S> &a, R *lds, i
             for (int i = 0; i < 9; ++i) pa1[i] = f[GROUP_ZP[i]];
         }
         load_plane_to(z0 + 1, ra0, ra1);
-        recon(rb0, f, false, z0);
+        recon(rb0, f, false, z0, fo_cur);
 #pragma unroll
         for (int i = 0; i < 9; ++i) {
             pb0[i] = f[GROUP_ZP[i]];
@@ -517,10 +530,17 @@ __device__ __forceinline__ void window_march(const FusedArgs<R, S> &a, R *lds, i
     R Mp[10];
     uint32_t idxp = a.pad;
 
+    // face lanes of a face march: fixed per thread (the outlet's x-1 neighbour is lane - 1, which
+    // the host guarantees: the outlet column is not a tile's first column)
+    const bool face_lane = FACE != FACE_NONE && inside &&
+                           (FACE == FACE_WEST    ? x == 0
+                            : FACE == FACE_EAST  ? x == a.nx - 1
+                            : FACE == FACE_NORTH ? y == a.ny - 1
+                                                 : y == 0);
     // one position: reconstruct plane z+1, write the pulled plane's groups, advance the windows
-    auto put = [&](const RawNode<S> &raw, int p, R *G, R *pold, R *q, bool west, int zp) {
+    auto put = [&](const RawNode<S> &raw, int p, R *G, R *pold, R *q, bool west, int zp, R *keep = nullptr) {
         R f[Q];
-        recon(raw, f, west, zp);
+        recon(raw, f, west, zp, keep);
 #pragma unroll
         for (int i = 0; i < 9; ++i) {
             G[GROUP_ZP[i] * NH + p] = pold[i];  // plane z-1
@@ -535,7 +555,7 @@ __device__ __forceinline__ void window_march(const FusedArgs<R, S> &a, R *lds, i
     };
     auto iteration = [&](int z, R *G, R *w0, R *w1) {
         if (NBUF == 1 && z > z0) lds_barrier();  // previous plane's pulls are done
-        put(ra0, h, G, w0, q0, false, z + 1);
+        put(ra0, h, G, w0, q0, false, z + 1, fo_nx);
         if (has1) put(ra1, p1, G, w1, q1, west1, z + 1);
         if constexpr (SPECIAL) {
             bp.copy_own(ra0, bp.own_nx);
@@ -544,6 +564,10 @@ __device__ __forceinline__ void window_march(const FusedArgs<R, S> &a, R *lds, i
         } else {
             store_node<R, S>(a.dst, idxp, a.scale, a.rho0, Mp, seed);
         }
+        // the inlet velocity of the plane (every thread loads an in-range entry)
+        R u_in = R(0);
+        if constexpr (FACE == FACE_WEST)
+            u_in = a.inlet[min(uint32_t(y) + uint32_t(a.ny) * uint32_t(z), uint32_t(a.inlet_count - 1))];
         load_plane_to(min(z + 2, z1), ra0, ra1);  // consumed in the next iteration
         lds_barrier();
         R f[Q];
@@ -556,9 +580,36 @@ __device__ __forceinline__ void window_march(const FusedArgs<R, S> &a, R *lds, i
             bp.info_cur = bp.info_nx;
         } else {
             moments_lds(f, Mp);
+            if constexpr (FACE != FACE_NONE) {
+                if (__any(face_lane)) {  // the face's closed form (cases/cylinder/boundaries.cuh:201-315)
+                    R g[Q], Mf[10], old[10], oxm[10];
+#pragma unroll
+                    for (int q = 0; q < Q; ++q) g[q] = f[q];
+                    lds_to_pop(g);
+#pragma unroll
+                    for (int k = 0; k < 10; ++k) old[k] = oxm[k] = R(0);
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) {
+                        old[k] = fo_cur[k];
+                        oxm[k] = __shfl(fo_cur[k], int(threadIdx.x) - 1, 64);
+                    }
+                    if constexpr (FACE == FACE_WEST) bc_west(g, old, u_in, Mf);
+                    else if constexpr (FACE == FACE_EAST) bc_east(g, old, oxm, Mf);
+                    else if constexpr (FACE == FACE_NORTH) bc_ywall<face::K_N>(g, Mf);
+                    else bc_ywall<face::K_S>(g, Mf);
+                    if (face_lane) {
+#pragma unroll
+                        for (int k = 0; k < 10; ++k) Mp[k] = Mf[k];
+                    }
+                }
+            }
             collide(a.omega, Mp);
             watch_finite(a, Mp[0]);
             idxp = inside ? node : a.pad;
+        }
+        if constexpr (FACE == FACE_WEST || FACE == FACE_EAST) {
+#pragma unroll
+            for (int k = 0; k < 4; ++k) fo_cur[k] = fo_nx[k];
         }
     };
     R *const G0 = lds;
@@ -574,6 +625,26 @@ __device__ __forceinline__ void window_march(const FusedArgs<R, S> &a, R *lds, i
     }
     if constexpr (!SPECIAL) store_node<R, S>(a.dst, idxp, a.scale, a.rho0, Mp, seed);
     if (NBUF == 1) lds_barrier();  // the block's LDS is free again for the next item
+}
+
+// One work item of the register-window march from its tile field: the boundary path for special
+// tiles (when BOUNDARY), a face march for face tiles, the plain march otherwise.
+template <class R, class S, int BX, int BY, int NBUF, bool BOUNDARY>
+__device__ __forceinline__ void window_item(const FusedArgs<R, S> &a, R *lds, uint32_t field, int z0, int z1) {
+    const int tile = int(field & kItemTileMask);
+    const int face = int((field >> kItemFaceShift) & 7u);
+    if (!BOUNDARY || !(field & kItemSpecial))
+        window_march<R, S, BX, BY, NBUF, false>(a, lds, tile, z0, z1);
+    else if (face == FACE_WEST)
+        window_march<R, S, BX, BY, NBUF, false, FACE_WEST>(a, lds, tile, z0, z1);
+    else if (face == FACE_EAST)
+        window_march<R, S, BX, BY, NBUF, false, FACE_EAST>(a, lds, tile, z0, z1);
+    else if (face == FACE_NORTH)
+        window_march<R, S, BX, BY, NBUF, false, FACE_NORTH>(a, lds, tile, z0, z1);
+    else if (face == FACE_SOUTH)
+        window_march<R, S, BX, BY, NBUF, false, FACE_SOUTH>(a, lds, tile, z0, z1);
+    else
+        window_march<R, S, BX, BY, NBUF, true>(a, lds, tile, z0, z1);
 }
 
 // PART: which launch of the fused pass (PART_ALL, PART_INTERIOR, PART_WALL_SIDE). The interior
@@ -592,15 +663,10 @@ __global__ __launch_bounds__(BX *BY, (BX * BY > 256 ? 1 : (NBUF == 1 ? 3 : 2))) 
     }
     if (a.xcd_groups > 1) b = (b % a.xcd_groups) * (a.nblocks / a.xcd_groups) + b / a.xcd_groups;
     const uint2 item = a.items[b];
-    const int tile = int(item.x & 0x7FFFFFFFu);
-    const bool special = (item.x >> 31) != 0;
     const int z0 = int(item.y & 0xFFFFu);
     const int z1 = int(item.y >> 16);
     if (z0 >= z1) return;  // padding item (uniform per block)
-    if (PART != PART_INTERIOR && special)
-        window_march<R, S, BX, BY, NBUF, true>(a, lds, tile, z0, z1);
-    else
-        window_march<R, S, BX, BY, NBUF, false>(a, lds, tile, z0, z1);
+    window_item<R, S, BX, BY, NBUF, PART != PART_INTERIOR>(a, lds, item.x, z0, z1);
 }
 
 template <class R, class S, int BX, int BY, int PART>
@@ -616,8 +682,8 @@ __global__ __launch_bounds__(BX *BY) void fused_step_ring6_kernel(FusedArgs<R, S
     }
     if (a.xcd_groups > 1) b = (b % a.xcd_groups) * (a.nblocks / a.xcd_groups) + b / a.xcd_groups;
     const uint2 item = a.items[b];
-    const int tile = int(item.x & 0x7FFFFFFFu);
-    const bool special = (item.x >> 31) != 0;
+    const int tile = int(item.x & kItemTileMask);
+    const bool special = (item.x & kItemSpecial) != 0;
     const int z0 = int(item.y & 0xFFFFu);
     const int z1 = int(item.y >> 16);
     if (z0 >= z1) return;  // padding item (uniform per block)
@@ -666,7 +732,7 @@ __global__ __launch_bounds__(BX *BY) void memory_twin_kernel(FusedArgs<R, S> a) 
     if (b >= a.nblocks) return;
     if (a.xcd_groups > 1) b = (b % a.xcd_groups) * (a.nblocks / a.xcd_groups) + b / a.xcd_groups;
     const uint2 item = a.items[b];
-    const int tile = int(item.x & 0x7FFFFFFFu);
+    const int tile = int(item.x & kItemTileMask);
     const int z0 = int(item.y & 0xFFFFu);
     const int z1 = int(item.y >> 16);
     if (z0 >= z1) return;
@@ -713,19 +779,14 @@ __device__ __forceinline__ void batch_item(const FusedArgs<R, S> *__restrict__ c
     const int z1 = int(item.z >> 16);
     if (z0 >= z1) return;  // padding item (uniform per block)
     const FusedArgs<R, S> a = cases[item.x];
-    const int tile = int(item.y & 0x7FFFFFFFu);
-    const bool special = (item.y >> 31) != 0;
     if constexpr (VARIANT == 1) {
-        if (special)
+        const int tile = int(item.y & kItemTileMask);
+        if (item.y & kItemSpecial)
             fused_march<R, S, BX, BY, true>(a, lds, tile, z0, z1);
         else
             fused_march<R, S, BX, BY, false>(a, lds, tile, z0, z1);
     } else {
-        constexpr int NBUF = VARIANT == 4 ? 2 : 1;
-        if (special)
-            window_march<R, S, BX, BY, NBUF, true>(a, lds, tile, z0, z1);
-        else
-            window_march<R, S, BX, BY, NBUF, false>(a, lds, tile, z0, z1);
+        window_item<R, S, BX, BY, VARIANT == 4 ? 2 : 1, true>(a, lds, item.y, z0, z1);
     }
 }
 

@@ -160,6 +160,7 @@ class Sim final : public SimBase {
         if (const char *e = std::getenv("MRLBM_LZ_SPECIAL")) lc.lz_special = std::max(1, std::atoi(e));
         if (const char *e = std::getenv("MRLBM_XCD_REMAP")) lc.xcd_remap = std::atoi(e) != 0;
         if (const char *e = std::getenv("MRLBM_SLAB_AXIS")) lc.slab_axis = (e[0] == 'y' || e[0] == '1') ? 1 : 2;
+        if (const char *e = std::getenv("MRLBM_FACE_TILES")) lc.face_tiles = std::atoi(e) != 0;
         if (const char *e = std::getenv("MRLBM_SPECIAL_ORDER")) lc.special_order = std::max(0, std::min(2, std::atoi(e)));
         if (const char *e = std::getenv("MRLBM_STOCHASTIC_ROUNDING")) stochastic_ = Conv<S>::scaled && std::atoi(e) != 0;
         if (const char *e = std::getenv("MRLBM_GRAPH_STEPS")) graph_steps_ = std::max(0, std::atoi(e) / 2 * 2);
@@ -249,18 +250,23 @@ class Sim final : public SimBase {
         // columns (x, y) that hold a boundary or donor node at some z make a tile special;
         // columns frozen at every z (inactive tiles of the reference) are never updated: both
         // buffers hold their values, and tiles made only of them get no work items
+        // (per special column: the set of node codes it holds, bit 31 for a donor node)
         {
             const size_t ncol = size_t(g.nx) * g.ny;
-            std::vector<uint8_t> col(ncol, 0);
+            std::vector<uint32_t> col(ncol, 0);
             frozen_cols_.assign(ncol, 1);
             for (int64_t i = 0; i < g.n; ++i) {
                 const size_t c = size_t(i % int64_t(ncol));
-                if (info[i] != NC_BULK) col[c] = 1;
+                if (info[i] != NC_BULK) col[c] |= (1u << (info[i] & INFO_CODE_MASK)) | ((info[i] & INFO_DONOR) ? 0x80000000u : 0u);
                 if ((info[i] & INFO_CODE_MASK) != NC_FROZEN) frozen_cols_[c] = 0;
             }
             special_cols_.clear();
+            special_col_codes_.clear();
             for (size_t c = 0; c < ncol; ++c)
-                if (col[c]) special_cols_.push_back(uint32_t(c));
+                if (col[c]) {
+                    special_cols_.push_back(uint32_t(c));
+                    special_col_codes_.push_back(col[c]);
+                }
         }
 
         std::vector<R> wall_nx_all(std::max(nwall, 1)), wall_ny_all(std::max(nwall, 1));
@@ -367,11 +373,27 @@ class Sim final : public SimBase {
         const int tiles_x = (g.nx + lc.bx - 1) / lc.bx;
         const int tiles_y = (g.ny + lc.by - 1) / lc.by;
         const int ntiles = tiles_x * tiles_y;
+        if (uint32_t(ntiles) > kItemTileMask) return set_error(MRLBM_ERR_UNSUPPORTED, "too many tiles in a z plane");
         std::vector<uint8_t> tile_special(ntiles, 0), tile_live(ntiles, 0);
-        for (uint32_t c : special_cols_) {
+        std::vector<uint32_t> tile_codes(ntiles, 0);
+        for (size_t k = 0; k < special_cols_.size(); ++k) {
+            const uint32_t c = special_cols_[k];
             const int x = int(c % uint32_t(g.nx)), y = int(c / uint32_t(g.nx));
             tile_special[x / lc.bx + tiles_x * (y / lc.by)] = 1;
+            tile_codes[x / lc.bx + tiles_x * (y / lc.by)] |= special_col_codes_[k];
         }
+        // face kind of a special tile: its only boundary nodes (no donor) are inlet nodes, outlet
+        // nodes (not in the tile's first column, whose x-1 neighbour is in the ring), north-wall or
+        // south-wall nodes; the register-window march runs such tiles as face marches
+        auto face_kind = [&](int t) -> uint32_t {
+            if (lc.variant == 1 || !lc.face_tiles) return uint32_t(FACE_NONE);
+            const uint32_t m = tile_codes[t];
+            if (m == (1u << NC_WEST)) return uint32_t(FACE_WEST);
+            if (m == (1u << NC_EAST) && (g.nx - 1) % lc.bx != 0) return uint32_t(FACE_EAST);
+            if (m == (1u << NC_NORTH)) return uint32_t(FACE_NORTH);
+            if (m == (1u << NC_SOUTH)) return uint32_t(FACE_SOUTH);
+            return uint32_t(FACE_NONE);
+        };
         for (size_t c = 0; c < frozen_cols_.size(); ++c)
             if (!frozen_cols_[c]) {
                 const int x = int(c % size_t(g.nx)), y = int(c / size_t(g.nx));
@@ -404,7 +426,8 @@ class Sim final : public SimBase {
         auto add = [&](int gi, int t, int z, int z1) {
             const bool sp = tile_special[t];
             const int part = (sp || tile_wall[t]) ? 2 : 1;
-            const uint2 it = make_uint2(uint32_t(t) | (sp ? 0x80000000u : 0u), uint32_t(z) | (uint32_t(z1) << 16));
+            const uint32_t field = uint32_t(t) | (sp ? kItemSpecial | (face_kind(t) << kItemFaceShift) : 0u);
+            const uint2 it = make_uint2(field, uint32_t(z) | (uint32_t(z1) << 16));
             lists[0][gi].push_back(it);
             lists[part][gi].push_back(it);
         };
@@ -1104,6 +1127,7 @@ class Sim final : public SimBase {
     int bx_ = 32, by_ = 8, lz_ = 16, lz_special_ = 16, tiles_x_ = 1, tiles_y_ = 1, ntiles_ = 1, xcd_ = 1;
     int nitems_[3] = {1, 0, 0};  // work items (blocks) of the three item lists
     std::vector<uint32_t> special_cols_;  // x + nx y of columns holding non-bulk nodes
+    std::vector<uint32_t> special_col_codes_;  // their node-code sets (bit 31: a donor node)
     std::vector<uint8_t> frozen_cols_;    // per column: frozen at every z
     std::vector<std::vector<uint2>> lists_;
     LaunchConfig launch_;

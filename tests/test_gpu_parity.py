@@ -329,6 +329,30 @@ def test_slab_axis_is_bitwise_neutral(kind, precision):
         assert_fp64_parity(out[0][0], o.fields())
 
 
+@pytest.mark.parametrize("precision", [_abi.FP32, _abi.FP32_BF16, _abi.FP32_FP16])
+def test_face_marches_are_bitwise_neutral(precision):
+    """Tiles whose only boundary nodes are inlet, outlet, north-wall or south-wall nodes run the
+    plain march with the face's closed form on their face lanes: the same state and force records
+    as the boundary path, bit for bit (a channel whose inlet, outlet and wall tiles include face
+    tiles of all four kinds)."""
+    from mrlbm_3d_cuda_amd.solver import Simulation
+
+    d = configs.cylinder(nx=160, ny=48, nz=12, diameter=10.0, xc=40.0, yc=24.0, u_max=0.05, re=60.0,
+                         precision=precision)
+    out = []
+    for on in ("0", "1"):
+        os.environ["MRLBM_FACE_TILES"] = on
+        try:
+            s = Simulation(d).init()
+        finally:
+            del os.environ["MRLBM_FACE_TILES"]
+        s.step(5, forces=True).step(32, forces=True)
+        out.append((s.fields(), s.forces(0, 37)))
+        s.close()
+    assert np.array_equal(out[0][0], out[1][0])
+    assert np.array_equal(out[0][1], out[1][1])
+
+
 def test_deterministic_repeat():
     from mrlbm_3d_cuda_amd.solver import Simulation
 
