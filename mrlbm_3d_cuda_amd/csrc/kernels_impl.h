@@ -110,6 +110,34 @@ __device__ __forceinline__ void watch_finite(const FusedArgs<R, S> &a, R rho) {
         if (a.nonfinite) *a.nonfinite = 1u;
 }
 
+// Collision of a node that is stored next (do_collide false: stored as it is). The 16-bit forms
+// store m - u u, which the collision scales by 1 - omega: m* - u u = (1 - omega)(m - u u). They
+// keep that non-equilibrium part in M[4..9] (two operations per moment instead of the collision's
+// two and the encoder's subtraction), which store_collided then encodes as it is.
+template <class R, class S>
+__device__ __forceinline__ void collide_for_store(R omega, bool do_collide, R *M) {
+    if constexpr (Conv<S>::scaled) {
+        const R fac = do_collide ? R(1) - omega : R(1);
+        M[4] = fac * (M[4] - M[1] * M[1]);
+        M[5] = fac * (M[5] - M[2] * M[2]);
+        M[6] = fac * (M[6] - M[3] * M[3]);
+        M[7] = fac * (M[7] - M[1] * M[2]);
+        M[8] = fac * (M[8] - M[1] * M[3]);
+        M[9] = fac * (M[9] - M[2] * M[3]);
+    } else {
+        if (do_collide) collide(omega, M);
+    }
+}
+
+// Store of a node finished by collide_for_store.
+template <class R, class S>
+__device__ __forceinline__ void store_collided(const FieldSet<S> &fs, uint32_t idx, const StoreScale<R> &sc, R rho0,
+                                               const R *M, uint32_t seed) {
+    S v[10];
+    encode_node<R, S, Conv<S>::scaled>(idx, sc, rho0, M, seed, v);
+    store_raw<S>(fs, idx, v);
+}
+
 // Per-node table values of a boundary node, loaded before the plane's prefetch with in-range
 // indices whatever the node code (so every thread issues the same loads).
 template <class R>
@@ -263,8 +291,8 @@ struct BoundaryPath {
             for (int k = 0; k < 10; ++k) v[k] = Conv<S>::from(R(0));
         }
         store_raw_nm(a.donor, didx, v);
-        if (do_collide) collide(a.omega, M);
-        store_node<R, S>(a.dst, idx, a.scale, a.rho0, M, seed);
+        collide_for_store<R, S>(a.omega, do_collide, M);
+        store_collided<R, S>(a.dst, idx, a.scale, a.rho0, M, seed);
     }
 };
 
@@ -395,7 +423,7 @@ __device__ __forceinline__ void fused_march(const FusedArgs<R, S> &a, R *lds, in
             load_tabs(a, bp.info_cur, uint32_t(y) + uint32_t(a.ny) * uint32_t(z), bp.tab);
             bp.load_info(a, off0 + uint32_t(pmod(z + 1, a.nz)) * plane, bp.info_nx);
         } else {
-            store_node<R, S>(a.dst, idxp, a.scale, a.rho0, Mp, seed);
+            store_collided<R, S>(a.dst, idxp, a.scale, a.rho0, Mp, seed);
         }
         load_plane(z + 2);  // prefetch, consumed next iteration (wraps past the item)
         lds_barrier();
@@ -406,7 +434,7 @@ __device__ __forceinline__ void fused_march(const FusedArgs<R, S> &a, R *lds, in
             bp.update(a, f, tx, ty, inside, east_tile, node, wold + west_slot(z) * 10 * BY, seed);
         } else {
             moments_lds(f, Mp);
-            collide(a.omega, Mp);
+            collide_for_store<R, S>(a.omega, true, Mp);
             watch_finite(a, Mp[0]);
             idxp = inside ? node : a.pad;
         }
@@ -417,7 +445,7 @@ __device__ __forceinline__ void fused_march(const FusedArgs<R, S> &a, R *lds, in
             bp.info_cur = bp.info_nx;
         }
     }
-    if constexpr (!SPECIAL) store_node<R, S>(a.dst, idxp, a.scale, a.rho0, Mp, seed);
+    if constexpr (!SPECIAL) store_collided<R, S>(a.dst, idxp, a.scale, a.rho0, Mp, seed);
 }
 
 // Register-window march: LDS holds NBUF planes of the 27 populations pulled in one iteration;
@@ -455,6 +483,17 @@ __device__ __forceinline__ void window_march(const FusedArgs<R, S> &a, R *lds, i
     const uint32_t off1 = has1 ? uint32_t(pmod(x0 - 1 + hx1, a.nx)) + uint32_t(a.nx) * uint32_t(pmod(y0 - 1 + hy1, a.ny))
                                : off0;
     const bool west1 = has1 && hx1 == 0 && hy1 >= 1 && hy1 <= BY;
+    // FP32 arithmetic: a ring position reconstructs only the nine populations that stream from it
+    // into the tile -- c_y = +1 from the south row, -1 from the north row (corners included), c_x =
+    // +1 / -1 from the west / east column -- and keeps windows of three per c_z group. roff[j]: its
+    // LDS offsets of the c_z = -1 populations with c_b = j - 1 (lds_slot order; + 9 NH per c_z step).
+    constexpr bool RING9 = !kExactOrder<R>;
+    constexpr int RW = RING9 ? 3 : 9;
+    const bool ring_row = hy1 == 0 || hy1 == BY + 1;
+    const int ring_si = ring_row ? (hy1 == 0 ? 1 : -1) : (hx1 == 0 ? 1 : -1);
+    int roff[3];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) roff[j] = (ring_row ? 3 * (ring_si + 1) + j : 3 * j + (ring_si + 1)) * NH + p1;
 
     // stored values of the plane in flight, per position (set B only holds plane z0 in the
     // prologue, which loads it together with plane z0-1)
@@ -465,11 +504,10 @@ __device__ __forceinline__ void window_march(const FusedArgs<R, S> &a, R *lds, i
         load_raw(a.src, base + off1, d1);
     };
     // windows, per position: c_z=+1 groups of two planes (A, B), c_z=0 group of one plane
-    R pa0[9], pb0[9], q0[9], pa1[9], pb1[9], q1[9];
+    R pa0[9], pb0[9], q0[9], pa1[RW], pb1[RW], q1[RW];
     // face march: density and velocity of the own node of the plane being updated and of the next
     R fo_cur[4] = {R(0), R(0), R(0), R(0)}, fo_nx[4] = {R(0), R(0), R(0), R(0)};
-    auto recon = [&](const RawNode<S> &raw, R *f, bool west, int zp, R *keep = nullptr) {
-        R m[10];
+    auto decode_pos = [&](const RawNode<S> &raw, R *m, bool west, int zp, R *keep) {
         decode_node<R, S>(raw, a.scale, a.rho0, m);
         if (FACE == FACE_WEST || FACE == FACE_EAST) {
             if (keep) {
@@ -481,7 +519,35 @@ __device__ __forceinline__ void window_march(const FusedArgs<R, S> &a, R *lds, i
 #pragma unroll
             for (int k = 0; k < 10; ++k) wold[(west_slot(zp) * 10 + k) * BY + (hy1 - 1)] = m[k];
         }
+    };
+    auto recon = [&](const RawNode<S> &raw, R *f, bool west, int zp, R *keep = nullptr) {
+        R m[10];
+        decode_pos(raw, m, west, zp, keep);
         reconstruct_lds(m, f);
+    };
+    // a ring position's populations of plane zp by c_z group (+1, 0, -1; RW values each; null: not
+    // wanted)
+    auto recon_ring = [&](const RawNode<S> &raw, bool west, int zp, R *gzp, R *gz0, R *gzm) {
+        if constexpr (RING9) {
+            R m[10], g[9];
+            decode_pos(raw, m, west, zp, nullptr);
+            reconstruct_ring(m, ring_row, R(ring_si), g);
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                gzp[j] = g[6 + j];
+                if (gz0) gz0[j] = g[3 + j];
+                if (gzm) gzm[j] = g[j];
+            }
+        } else {
+            R f[Q];
+            recon(raw, f, west, zp);
+#pragma unroll
+            for (int i = 0; i < 9; ++i) {
+                gzp[i] = f[GROUP_ZP[i]];
+                if (gz0) gz0[i] = f[GROUP_Z0[i]];
+                if (gzm) gzm[i] = f[GROUP_ZM[i]];
+            }
+        }
     };
 
     BoundaryPath<R, S, BX, BY> bp;
@@ -494,11 +560,7 @@ __device__ __forceinline__ void window_march(const FusedArgs<R, S> &a, R *lds, i
         recon(ra0, f, false, z0 - 1);
 #pragma unroll
         for (int i = 0; i < 9; ++i) pa0[i] = f[GROUP_ZP[i]];
-        if (has1) {
-            recon(ra1, f, false, z0 - 1);
-#pragma unroll
-            for (int i = 0; i < 9; ++i) pa1[i] = f[GROUP_ZP[i]];
-        }
+        if (has1) recon_ring(ra1, false, z0 - 1, pa1, nullptr, nullptr);
         load_plane_to(z0 + 1, ra0, ra1);
         recon(rb0, f, false, z0, fo_cur);
 #pragma unroll
@@ -506,14 +568,7 @@ __device__ __forceinline__ void window_march(const FusedArgs<R, S> &a, R *lds, i
             pb0[i] = f[GROUP_ZP[i]];
             q0[i] = f[GROUP_Z0[i]];
         }
-        if (has1) {
-            recon(rb1, f, west1, z0);
-#pragma unroll
-            for (int i = 0; i < 9; ++i) {
-                pb1[i] = f[GROUP_ZP[i]];
-                q1[i] = f[GROUP_Z0[i]];
-            }
-        }
+        if (has1) recon_ring(rb1, west1, z0, pb1, q1, nullptr);
         if constexpr (SPECIAL) {
             bp.copy_own(rb0, bp.own_cur);
             bp.load_info(a, off0 + uint32_t(z0) * plane, bp.info_cur);
@@ -543,9 +598,9 @@ __device__ __forceinline__ void window_march(const FusedArgs<R, S> &a, R *lds, i
         recon(raw, f, west, zp, keep);
 #pragma unroll
         for (int i = 0; i < 9; ++i) {
-            G[GROUP_ZP[i] * NH + p] = pold[i];  // plane z-1
-            G[GROUP_Z0[i] * NH + p] = q[i];     // plane z
-            G[GROUP_ZM[i] * NH + p] = f[GROUP_ZM[i]];  // plane z+1
+            G[lds_slot(GROUP_ZP[i]) * NH + p] = pold[i];  // plane z-1
+            G[lds_slot(GROUP_Z0[i]) * NH + p] = q[i];     // plane z
+            G[lds_slot(GROUP_ZM[i]) * NH + p] = f[GROUP_ZM[i]];  // plane z+1
         }
 #pragma unroll
         for (int i = 0; i < 9; ++i) {
@@ -553,16 +608,36 @@ __device__ __forceinline__ void window_march(const FusedArgs<R, S> &a, R *lds, i
             q[i] = f[GROUP_Z0[i]];
         }
     };
+    // the same for the thread's ring position
+    auto put_ring = [&](const RawNode<S> &raw, R *G, R *pold, R *q, bool west, int zp) {
+        if constexpr (RING9) {
+            R gzp[3], gz0[3], gzm[3];
+            recon_ring(raw, west, zp, gzp, gz0, gzm);
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                G[roff[j] + 18 * NH] = pold[j];
+                G[roff[j] + 9 * NH] = q[j];
+                G[roff[j]] = gzm[j];
+            }
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                pold[j] = gzp[j];
+                q[j] = gz0[j];
+            }
+        } else {
+            put(raw, p1, G, pold, q, west, zp);
+        }
+    };
     auto iteration = [&](int z, R *G, R *w0, R *w1) {
         if (NBUF == 1 && z > z0) lds_barrier();  // previous plane's pulls are done
         put(ra0, h, G, w0, q0, false, z + 1, fo_nx);
-        if (has1) put(ra1, p1, G, w1, q1, west1, z + 1);
+        if (has1) put_ring(ra1, G, w1, q1, west1, z + 1);
         if constexpr (SPECIAL) {
             bp.copy_own(ra0, bp.own_nx);
             load_tabs(a, bp.info_cur, uint32_t(y) + uint32_t(a.ny) * uint32_t(z), bp.tab);
             bp.load_info(a, off0 + uint32_t(pmod(z + 1, a.nz)) * plane, bp.info_nx);
         } else {
-            store_node<R, S>(a.dst, idxp, a.scale, a.rho0, Mp, seed);
+            store_collided<R, S>(a.dst, idxp, a.scale, a.rho0, Mp, seed);
         }
         // the inlet velocity of the plane (every thread loads an in-range entry)
         R u_in = R(0);
@@ -572,7 +647,7 @@ __device__ __forceinline__ void window_march(const FusedArgs<R, S> &a, R *lds, i
         lds_barrier();
         R f[Q];
 #pragma unroll
-        for (int q = 0; q < Q; ++q) f[q] = G[q * NH + h - LATTICE.cx[q] - LATTICE.cy[q] * W];
+        for (int q = 0; q < Q; ++q) f[q] = G[lds_slot(q) * NH + h - LATTICE.cx[q] - LATTICE.cy[q] * W];
         const uint32_t node = off0 + uint32_t(z) * plane;
         if constexpr (SPECIAL) {
             bp.update(a, f, tx, ty, inside, east_tile, node, wold + west_slot(z) * 10 * BY, seed);
@@ -603,7 +678,7 @@ __device__ __forceinline__ void window_march(const FusedArgs<R, S> &a, R *lds, i
                     }
                 }
             }
-            collide(a.omega, Mp);
+            collide_for_store<R, S>(a.omega, true, Mp);
             watch_finite(a, Mp[0]);
             idxp = inside ? node : a.pad;
         }
@@ -623,7 +698,7 @@ __device__ __forceinline__ void window_march(const FusedArgs<R, S> &a, R *lds, i
         iteration(z, G0, pa0, pa1);
         ++z;
     }
-    if constexpr (!SPECIAL) store_node<R, S>(a.dst, idxp, a.scale, a.rho0, Mp, seed);
+    if constexpr (!SPECIAL) store_collided<R, S>(a.dst, idxp, a.scale, a.rho0, Mp, seed);
     if (NBUF == 1) lds_barrier();  // the block's LDS is free again for the next item
 }
 

@@ -237,6 +237,37 @@ __device__ __forceinline__ void reconstruct_lds(const R *M, R *g) {
     }
 }
 
+// The nine populations in the LDS form (FP32 arithmetic) with one in-plane velocity component
+// fixed, c_a = s (a = y when `along_y`, else x; s = +-1), the ones a ring position of a tile hands
+// to the tile. g[3 k + j] has c_b = j - 1 (b the other in-plane axis) and c_z = k - 1. With c_a
+// fixed the Hermite sum of reconstruct_lds is a quadratic in c_b and c_z:
+// g = K + Lz cz + Dz cz^2 + (Lb + Obz cz) cb + Db cb^2, K = P + Da + s Ua, Lb = Ub + s Oab,
+// Lz = Uz + s Oaz.
+template <class R>
+__device__ __forceinline__ void reconstruct_ring(const R *M, bool along_y, R s, R *g) {
+    const R rho = M[0];
+    const R P = rho * (R(1) - R(1.5) * (M[4] + M[5] + M[6]));
+    const R r3 = R(3) * rho, r45 = R(4.5) * rho, r9 = R(9) * rho;
+    const R sr3 = s * r3, sr9 = s * r9;
+    const R ua = along_y ? M[2] : M[1], ub = along_y ? M[1] : M[2];
+    const R maa = along_y ? M[5] : M[4], mbb = along_y ? M[4] : M[5];
+    const R maz = along_y ? M[9] : M[8], mbz = along_y ? M[8] : M[9];
+    const R K = P + r45 * maa + sr3 * ua;
+    const R Lb = r3 * ub + sr9 * M[7];
+    const R Lz = r3 * M[3] + sr9 * maz;
+    const R Dz = r45 * M[6], Db = r45 * mbb, Obz = r9 * mbz;
+    const R Kz = K + Dz;
+    const R A[3] = {Kz - Lz, K, Kz + Lz};
+    const R B[3] = {Lb - Obz, Lb, Lb + Obz};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const R t = A[k] + Db;
+        g[3 * k + 0] = t - B[k];
+        g[3 * k + 1] = A[k];
+        g[3 * k + 2] = t + B[k];
+    }
+}
+
 // Moments of the populations held in LDS (eval_moments.cuh:5-22). FP32: the D3Q27 weights are
 // products of the 1-D weights w1(0) = 2/3, w1(+-1) = 1/6, so the weighted sums factor into three
 // passes of 1-D sums (x, then y, then z), each of which keeps, per line, the zeroth, first and

@@ -51,8 +51,7 @@ typedef __attribute__((__vector_size__(2 * sizeof(_Float16)))) _Float16 f16x2_t;
 // 16-bit forms round stochastically when given random bits: a uniform integer is added below
 // the kept mantissa bits before truncation, so the stored value is unbiased. Round-to-nearest
 // would drop every per-step increment smaller than half an ulp and stall slowly evolving flow.
-// `r` holds 8 random bits; the 8 mantissa bits below them are added as zero, which leaves a
-// bias below 2^-9 ulp.
+// BF16 takes 16 random bits (the low half of `r`, the rest zero), all the bits it drops.
 // The BF16 form keeps the velocity in FP16 bits (u_x - u_ref, u_y, u_z, times kVelocityScale):
 // BF16's 8-bit significand made the velocity's rounding noise the dominant storage error (drag
 // error on 16 seeded 64-192 x 64-96 x 32-96 channels, mean / max over the cases after 10,000
@@ -65,8 +64,8 @@ struct Conv<__hip_bfloat16> {
     static constexpr bool half_velocity = true;
     __device__ static float to(__hip_bfloat16 v) { return __bfloat162float(v); }
     __device__ static __hip_bfloat16 from(float v) { return __float2bfloat16(v); }
-    __device__ static __hip_bfloat16 from_sr(float v, uint32_t r8) {
-        const uint32_t b = __float_as_uint(v) + (r8 << 8);
+    __device__ static __hip_bfloat16 from_sr(float v, uint32_t r16) {
+        const uint32_t b = __float_as_uint(v) + r16;
         return __ushort_as_bfloat16(static_cast<unsigned short>(b >> 16));
     }
 };
@@ -79,18 +78,18 @@ struct Conv<_Float16> {
     __device__ static float to(_Float16 v) { return float(v); }
     __device__ static _Float16 from(float v) { return _Float16(v); }
     // v_cvt_sr_f16_f32 adds the high 13 bits of its random operand below the kept mantissa bits
-    // and truncates; with r8 in the top byte that is v + (r8 << 5) on the float's bits, truncated
-    // to half (13 float mantissa bits fall below the half mantissa in the normal range)
-    __device__ static _Float16 from_sr(float v, uint32_t r8) {
+    // and truncates (13 float mantissa bits fall below the half mantissa in the normal range):
+    // `r` is a whole random word
+    __device__ static _Float16 from_sr(float v, uint32_t r) {
         f16x2_t w = {};
-        w = __builtin_amdgcn_cvt_sr_f16_f32(w, v, r8 << 24, false);
+        w = __builtin_amdgcn_cvt_sr_f16_f32(w, v, r, false);
         return w[0];
     }
 };
 
-// Random bits for stochastic rounding: one 32-bit hash of (node, step seed) per node; field f
-// takes the 8 bits starting at bit (8 f) mod 25 (overlapping windows inside the word: the
-// rounding of each field is unbiased, correlations between the fields of one node do not matter).
+// Random bits for stochastic rounding: one 32-bit hash of (node, step seed) per node, from which
+// every field takes a window (overlapping windows inside the word: the rounding of each field is
+// unbiased, correlations between the fields of one node do not matter).
 __device__ __forceinline__ uint32_t sr_hash(uint32_t idx, uint32_t seed) {
     uint32_t h = (idx ^ 0x61C88647u) * 0x9E3779B1u + seed;
     h ^= h >> 15;
@@ -99,7 +98,6 @@ __device__ __forceinline__ uint32_t sr_hash(uint32_t idx, uint32_t seed) {
     return h;
 }
 
-__device__ __forceinline__ uint32_t sr_bits(uint32_t h, int f) { return (h >> ((8 * f) % 25)) & 0xFFu; }
 
 // Field scale class: 0 for rho, 1 for u, 2 for the second moments.
 __device__ __host__ constexpr int field_class(int f) { return f == 0 ? 0 : (f < 4 ? 1 : 2); }
@@ -307,6 +305,22 @@ __host__ __device__ constexpr bool has_half_velocity() {
     }
 }
 
+// Random operand of field f's stochastic rounding from the node's hash h: a whole word for the
+// FP16 conversions (its high 13 bits are used; the word rotated by 3 f, or 7 f for the BF16 form's
+// velocity), 16 bits for a BF16 field (the halves of h and of h rotated by 5 and 10 bits, so a
+// field costs one rotation per two fields and an add).
+template <class S>
+__device__ __forceinline__ uint32_t sr_rand(uint32_t h, int f) {
+    if constexpr (has_half_velocity<S>()) {
+        if (f >= 1 && f <= 3) return __builtin_rotateright32(h, 7 * f);
+        const int k = f == 0 ? 0 : f - 3;  // BF16 fields 0, 4..9 -> 0..6
+        const uint32_t w = k < 2 ? h : __builtin_rotateright32(h, 5 * (k / 2));
+        return (k & 1) ? (w >> 16) : (w & 0xFFFFu);
+    } else {
+        return __builtin_rotateright32(h, 3 * f);
+    }
+}
+
 // Value of stored field f before the deviation is undone (scaled back for FP16).
 template <class R, class S>
 __device__ __forceinline__ R stored_value(S raw, int f, const StoreScale<R> &sc) {
@@ -321,18 +335,18 @@ __device__ __forceinline__ R stored_value(S raw, int f, const StoreScale<R> &sc)
     return v;
 }
 
-// Stored bits of deviation v of field f (v already scaled for FP16 storage); r8: random bits of
-// the stochastic rounding (sr), unused with round-to-nearest.
+// Stored bits of deviation v of field f (v already scaled for FP16 storage); r: random operand of
+// the stochastic rounding (sr, from sr_rand), unused with round-to-nearest.
 template <class R, class S>
-__device__ __forceinline__ S stored_bits(R v, int f, uint32_t r8, bool sr) {
+__device__ __forceinline__ S stored_bits(R v, int f, uint32_t r, bool sr) {
     if constexpr (has_half_velocity<S>()) {
         if (f >= 1 && f <= 3) {
             const float hv = float(v) * kVelocityScale;
-            const _Float16 h = sr ? Conv<_Float16>::from_sr(hv, r8) : _Float16(hv);
+            const _Float16 h = sr ? Conv<_Float16>::from_sr(hv, r) : _Float16(hv);
             return __builtin_bit_cast(S, __builtin_bit_cast(uint16_t, h));
         }
     }
-    return sr ? Conv<S>::from_sr(v, r8) : Conv<S>::from(v);
+    return sr ? Conv<S>::from_sr(v, r) : Conv<S>::from(v);
 }
 
 // The stored words of one node as loaded -- three accesses (FP32 and 16-bit storage: fields 0-3,
@@ -419,8 +433,9 @@ __device__ __forceinline__ void load_node(const FieldSet<const S> &fs, uint32_t 
 }
 
 // The 10 stored values of the moments M of node `idx`. For the 16-bit forms `seed` selects
-// stochastic rounding (seed != 0) or round-to-nearest (seed == 0).
-template <class R, class S>
+// stochastic rounding (seed != 0) or round-to-nearest (seed == 0), and NEQ says that M[4..9]
+// already hold the non-equilibrium second moments m - u u.
+template <class R, class S, bool NEQ = false>
 __device__ __forceinline__ void encode_node(uint32_t idx, const StoreScale<R> &sc, R rho0, const R *M, uint32_t seed,
                                             S *out) {
     if constexpr (!Conv<S>::scaled) {
@@ -433,12 +448,12 @@ __device__ __forceinline__ void encode_node(uint32_t idx, const StoreScale<R> &s
         v[1] = M[1] - sc.uref;
         v[2] = M[2];
         v[3] = M[3];
-        v[4] = M[4] - M[1] * M[1];
-        v[5] = M[5] - M[2] * M[2];
-        v[6] = M[6] - M[3] * M[3];
-        v[7] = M[7] - M[1] * M[2];
-        v[8] = M[8] - M[1] * M[3];
-        v[9] = M[9] - M[2] * M[3];
+        v[4] = NEQ ? M[4] : M[4] - M[1] * M[1];
+        v[5] = NEQ ? M[5] : M[5] - M[2] * M[2];
+        v[6] = NEQ ? M[6] : M[6] - M[3] * M[3];
+        v[7] = NEQ ? M[7] : M[7] - M[1] * M[2];
+        v[8] = NEQ ? M[8] : M[8] - M[1] * M[3];
+        v[9] = NEQ ? M[9] : M[9] - M[2] * M[3];
         if constexpr (!Conv<S>::unit_scale) {
 #pragma unroll
             for (int f = 0; f < 10; ++f) v[f] *= sc.enc[field_class(f)];
@@ -449,7 +464,7 @@ __device__ __forceinline__ void encode_node(uint32_t idx, const StoreScale<R> &s
         } else {
             const uint32_t h = sr_hash(idx, seed);
 #pragma unroll
-            for (int f = 0; f < 10; ++f) out[f] = stored_bits<R, S>(v[f], f, sr_bits(h, f), true);
+            for (int f = 0; f < 10; ++f) out[f] = stored_bits<R, S>(v[f], f, sr_rand<S>(h, f), true);
         }
     }
 }
