@@ -39,6 +39,7 @@ struct FusedArgs {
     S *__restrict__ donor;  // pre-collision snapshot of donor nodes, node-major
     uint32_t pad;           // index of a padding slot after the last node (discarded stores)
     int nx, ny, nz;
+    int plane_aligned;  // nx ny is a multiple of the chunk: a node's offsets repeat from plane to plane
     int tiles_x;     // tiles along x
     int nblocks;     // work items, one block each
     int xcd_groups;  // blocks are spread round-robin over this many XCDs (1: no remap)

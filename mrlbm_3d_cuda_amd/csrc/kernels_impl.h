@@ -461,7 +461,7 @@ __device__ __forceinline__ void fused_march(const FusedArgs<R, S> &a, R *lds, in
 // closed form before the collision (the density and velocity of the node and, at the outlet, of its
 // x-1 neighbour are kept from the decode of the plane, the x-1 values by lane shuffle).
 template <class R, class S, int BX, int BY, int NBUF, bool SPECIAL, int FACE = FACE_NONE>
-__device__ __forceinline__ void window_march(const FusedArgs<R, S> &a, R *lds, int tile, int z0, int z1) {
+__device__ __forceinline__ void window_march(const FusedArgs<R, S> &a, R *lds, int tile, int z0, int z1, int rs) {
     static_assert(!SPECIAL || FACE == FACE_NONE, "a face march runs the plain path");
     using F = Frame<BX, BY>;
     constexpr int W = F::W;
@@ -475,9 +475,10 @@ __device__ __forceinline__ void window_march(const FusedArgs<R, S> &a, R *lds, i
     const uint32_t plane = uint32_t(a.nx) * uint32_t(a.ny);
     const int h = (ty + 1) * W + (tx + 1);
 
-    const bool has1 = tid < F::NR;
+    // ring positions are held by threads rs .. rs + NR - 1
+    const bool has1 = tid >= rs && tid < rs + F::NR;
     int hx1 = 0, hy1 = 0;
-    if (has1) F::ring(tid, hx1, hy1);
+    if (has1) F::ring(tid - rs, hx1, hy1);
     const int p1 = hy1 * W + hx1;
     const uint32_t off0 = uint32_t(pmod(x0 + tx, a.nx)) + uint32_t(a.nx) * uint32_t(pmod(y0 + ty, a.ny));
     const uint32_t off1 = has1 ? uint32_t(pmod(x0 - 1 + hx1, a.nx)) + uint32_t(a.nx) * uint32_t(pmod(y0 - 1 + hy1, a.ny))
@@ -498,10 +499,24 @@ __device__ __forceinline__ void window_march(const FusedArgs<R, S> &a, R *lds, i
     // stored values of the plane in flight, per position (set B only holds plane z0 in the
     // prologue, which loads it together with plane z0-1)
     RawNode<S> ra0, ra1, rb0, rb1;
+    // planes of whole chunks and a grid of whole tiles (every thread's node inside): loads and the
+    // plain march's stores address a position by its fixed offsets from the plane's first element
+    const bool fast = a.plane_aligned && a.nx % BX == 0 && a.ny % BY == 0;
+    const size_t plane_elems = size_t(plane) * 10;
+    const LaneOffsets lo0 = lane_offsets<S>(off0), lo1 = lane_offsets<S>(off1);
     auto load_plane_to = [&](int z, RawNode<S> &d0, RawNode<S> &d1) {
-        const uint32_t base = uint32_t(pmod(z, a.nz)) * plane;
-        load_raw(a.src, base + off0, d0);
-        load_raw(a.src, base + off1, d1);
+        const uint32_t zz = uint32_t(pmod(z, a.nz));
+        NodePtrs q0, q1;
+        if (fast) {
+            const S *pz = a.src.base + size_t(zz) * plane_elems;
+            q0 = node_ptrs_at(pz, lo0);
+            q1 = node_ptrs_at(pz, lo1);
+        } else {
+            q0 = node_ptrs(a.src.base, zz * plane + off0);
+            q1 = node_ptrs(a.src.base, zz * plane + off1);
+        }
+        load_raw_p(q0, d0);
+        load_raw_p(q1, d1);
     };
     // windows, per position: c_z=+1 groups of two planes (A, B), c_z=0 group of one plane
     R pa0[9], pb0[9], q0[9], pa1[RW], pb1[RW], q1[RW];
@@ -584,6 +599,16 @@ __device__ __forceinline__ void window_march(const FusedArgs<R, S> &a, R *lds, i
     }
     R Mp[10];
     uint32_t idxp = a.pad;
+    int zprev = z0;  // plane of Mp
+    // the plain march's store of the previous plane's results (none before the first plane)
+    auto store_prev = [&](bool any) {
+        if (fast && !any) return;
+        NodePtrs q = fast ? node_ptrs_at(static_cast<const S *>(a.dst.base + size_t(zprev) * plane_elems), lo0)
+                          : node_ptrs(static_cast<const S *>(a.dst.base), idxp);
+        S v[10];
+        encode_node<R, S, Conv<S>::scaled>(idxp, a.scale, a.rho0, Mp, seed, v);
+        store_raw_p<S>(q, v);
+    };
 
     // face lanes of a face march: fixed per thread (the outlet's x-1 neighbour is lane - 1, which
     // the host guarantees: the outlet column is not a tile's first column)
@@ -628,7 +653,7 @@ __device__ __forceinline__ void window_march(const FusedArgs<R, S> &a, R *lds, i
             put(raw, p1, G, pold, q, west, zp);
         }
     };
-    auto iteration = [&](int z, R *G, R *w0, R *w1) {
+    auto iteration = [&](int z, R *G, R *w0, R *w1, bool stored_before) {
         if (NBUF == 1 && z > z0) lds_barrier();  // previous plane's pulls are done
         put(ra0, h, G, w0, q0, false, z + 1, fo_nx);
         if (has1) put_ring(ra1, G, w1, q1, west1, z + 1);
@@ -637,7 +662,7 @@ __device__ __forceinline__ void window_march(const FusedArgs<R, S> &a, R *lds, i
             load_tabs(a, bp.info_cur, uint32_t(y) + uint32_t(a.ny) * uint32_t(z), bp.tab);
             bp.load_info(a, off0 + uint32_t(pmod(z + 1, a.nz)) * plane, bp.info_nx);
         } else {
-            store_collided<R, S>(a.dst, idxp, a.scale, a.rho0, Mp, seed);
+            store_prev(stored_before);
         }
         // the inlet velocity of the plane (every thread loads an in-range entry)
         R u_in = R(0);
@@ -681,6 +706,7 @@ __device__ __forceinline__ void window_march(const FusedArgs<R, S> &a, R *lds, i
             collide_for_store<R, S>(a.omega, true, Mp);
             watch_finite(a, Mp[0]);
             idxp = inside ? node : a.pad;
+            zprev = z;
         }
         if constexpr (FACE == FACE_WEST || FACE == FACE_EAST) {
 #pragma unroll
@@ -690,36 +716,46 @@ __device__ __forceinline__ void window_march(const FusedArgs<R, S> &a, R *lds, i
     R *const G0 = lds;
     R *const G1 = NBUF == 2 ? lds + 27 * NH : lds;
     int z = z0;
-    iteration(z, G0, pa0, pa1);
+    iteration(z, G0, pa0, pa1, false);
     ++z;
     while (z < z1) {
-        iteration(z, G1, pb0, pb1);
+        iteration(z, G1, pb0, pb1, true);
         if (++z >= z1) break;
-        iteration(z, G0, pa0, pa1);
+        iteration(z, G0, pa0, pa1, true);
         ++z;
     }
-    if constexpr (!SPECIAL) store_collided<R, S>(a.dst, idxp, a.scale, a.rho0, Mp, seed);
+    if constexpr (!SPECIAL) store_prev(true);
     if (NBUF == 1) lds_barrier();  // the block's LDS is free again for the next item
 }
 
 // One work item of the register-window march from its tile field: the boundary path for special
 // tiles (when BOUNDARY), a face march for face tiles, the plain march otherwise.
+// The ring positions (more work per thread) go to the first waves of the block, or, for odd tiles,
+// to its last waves: blocks resident on one CU then spread that work over the SIMDs instead of
+// loading the SIMDs of the first waves with all of it.
+template <int BX, int BY>
+constexpr int ring_alt_start() {
+    constexpr int nt = BX * BY, nr = Frame<BX, BY>::NR, nrw = (nr + 63) / 64 * 64;
+    return nt - nrw >= 64 ? nt - nrw : 0;
+}
+
 template <class R, class S, int BX, int BY, int NBUF, bool BOUNDARY>
 __device__ __forceinline__ void window_item(const FusedArgs<R, S> &a, R *lds, uint32_t field, int z0, int z1) {
     const int tile = int(field & kItemTileMask);
     const int face = int((field >> kItemFaceShift) & 7u);
+    const int rs = (tile & 1) ? ring_alt_start<BX, BY>() : 0;
     if (!BOUNDARY || !(field & kItemSpecial))
-        window_march<R, S, BX, BY, NBUF, false>(a, lds, tile, z0, z1);
+        window_march<R, S, BX, BY, NBUF, false>(a, lds, tile, z0, z1, rs);
     else if (face == FACE_WEST)
-        window_march<R, S, BX, BY, NBUF, false, FACE_WEST>(a, lds, tile, z0, z1);
+        window_march<R, S, BX, BY, NBUF, false, FACE_WEST>(a, lds, tile, z0, z1, rs);
     else if (face == FACE_EAST)
-        window_march<R, S, BX, BY, NBUF, false, FACE_EAST>(a, lds, tile, z0, z1);
+        window_march<R, S, BX, BY, NBUF, false, FACE_EAST>(a, lds, tile, z0, z1, rs);
     else if (face == FACE_NORTH)
-        window_march<R, S, BX, BY, NBUF, false, FACE_NORTH>(a, lds, tile, z0, z1);
+        window_march<R, S, BX, BY, NBUF, false, FACE_NORTH>(a, lds, tile, z0, z1, rs);
     else if (face == FACE_SOUTH)
-        window_march<R, S, BX, BY, NBUF, false, FACE_SOUTH>(a, lds, tile, z0, z1);
+        window_march<R, S, BX, BY, NBUF, false, FACE_SOUTH>(a, lds, tile, z0, z1, rs);
     else
-        window_march<R, S, BX, BY, NBUF, true>(a, lds, tile, z0, z1);
+        window_march<R, S, BX, BY, NBUF, true>(a, lds, tile, z0, z1, rs);
 }
 
 // PART: which launch of the fused pass (PART_ALL, PART_INTERIOR, PART_WALL_SIDE). The interior
@@ -825,11 +861,24 @@ __global__ __launch_bounds__(BX *BY) void memory_twin_kernel(FusedArgs<R, S> a) 
                                : off0;
     const bool inside = x0 + tx < a.nx && y0 + ty < a.ny;
     const uint32_t keep = a.nx > 0 ? 0u : ~0u;
+    // the fused march's addressing of whole-chunk planes of a grid of whole tiles
+    const bool fast = a.plane_aligned && a.nx % BX == 0 && a.ny % BY == 0;
+    const size_t plane_elems = size_t(plane) * 10;
+    const LaneOffsets lo0 = lane_offsets<S>(off0), lo1 = lane_offsets<S>(off1);
     RawNode<S> c0, c1, n0, n1;
     auto load_plane_to = [&](int z, RawNode<S> &d0, RawNode<S> &d1) {
-        const uint32_t base = uint32_t(pmod(z, a.nz)) * plane;
-        load_raw(a.src, base + off0, d0);
-        load_raw(a.src, base + off1, d1);
+        const uint32_t zz = uint32_t(pmod(z, a.nz));
+        NodePtrs q0, q1;
+        if (fast) {
+            const S *pz = a.src.base + size_t(zz) * plane_elems;
+            q0 = node_ptrs_at(pz, lo0);
+            q1 = node_ptrs_at(pz, lo1);
+        } else {
+            q0 = node_ptrs(a.src.base, zz * plane + off0);
+            q1 = node_ptrs(a.src.base, zz * plane + off1);
+        }
+        load_raw_p(q0, d0);
+        load_raw_p(q1, d1);
     };
     load_plane_to(z0 - 1, c0, c1);
     uint32_t fold = fold_bits(c0) ^ fold_bits(c1);
@@ -839,7 +888,9 @@ __global__ __launch_bounds__(BX *BY) void memory_twin_kernel(FusedArgs<R, S> a) 
         fold ^= fold_bits(c1);
         RawNode<S> v = c0;
         if (fold & keep) v = c1;  // never at run time
-        store_raw<S>(a.dst, inside ? off0 + uint32_t(z) * plane : a.pad, v);
+        store_raw_p<S>(fast ? node_ptrs_at(static_cast<const S *>(a.dst.base + size_t(z) * plane_elems), lo0)
+                            : node_ptrs(static_cast<const S *>(a.dst.base), inside ? off0 + uint32_t(z) * plane : a.pad),
+                       v);
         c0 = n0;
         c1 = n1;
     }

@@ -339,13 +339,20 @@ class Sim final : public SimBase {
         // tiles run faster (256x128x64 cylinder +3.7%); grids that stream from HBM keep 32-wide
         // rows (512x256x256 wake: 16x8 is 5% slower)
         constexpr size_t kLastLevelCacheBytes = size_t(256) << 20;
-        if (sizeof(R) == 4 && !Conv<S>::scaled && 2 * field_buffer_elems<S>(g.n) * sizeof(S) <= kLastLevelCacheBytes)
-            lc.bx = 16;
-        // register-window march for FP32 arithmetic: with FP32 storage two LDS planes (one
-        // barrier per plane, no register cap), with 16-bit storage one LDS plane, whose three
-        // waves per SIMD hide the decode and encode work better than the two-plane form's two
-        // (it spills a few registers at that bound, and still runs 10-17% faster)
-        lc.variant = sizeof(R) == 8 ? 1 : (Conv<S>::scaled ? 3 : 4);
+        const bool cache_resident = 2 * field_buffer_elems<S>(g.n) * sizeof(S) <= kLastLevelCacheBytes;
+        if (sizeof(R) == 4 && !Conv<S>::scaled && cache_resident) lc.bx = 16;
+        // FP32 storage streaming from HBM is bound by its traffic: its boundary tiles keep their
+        // place in tile order, so an inlet or outlet column runs beside its neighbours and they
+        // share ring lines in L2 (512x256x256 wake: 9% fewer L2 read requests, +2.5% MLUPS); the
+        // arithmetic-bound forms run the slower boundary tiles first, so that they form no tail
+        // (16-bit storage on the same wake: first +2% over tile order)
+        if (sizeof(R) == 4 && !Conv<S>::scaled && !cache_resident) lc.special_order = 1;
+        // register-window march for FP32 arithmetic: with 16-bit storage one LDS plane, whose
+        // four blocks per CU hide the decode and encode work better than the two-plane form's two;
+        // with FP32 storage two LDS planes (one barrier per plane) on grids that stream from HBM
+        // (512x256x256 wake: 1% faster than one plane), one plane on cache-resident grids, whose
+        // boundary tiles bound them (256x128x64 cylinder: 37.6 against 40.8 us per fused pass)
+        lc.variant = sizeof(R) == 8 ? 1 : ((Conv<S>::scaled || cache_resident) ? 3 : 4);
         // z chunks short enough for about 1024 work items (a block's plane-by-plane march is
         // latency-bound, so small grids need many short chunks), at most 32 planes
         const int64_t tiles = int64_t((g.nx + lc.bx - 1) / lc.bx) * ((g.ny + lc.by - 1) / lc.by);
@@ -624,6 +631,7 @@ class Sim final : public SimBase {
         a.nx = g.nx;
         a.ny = g.ny;
         a.nz = g.nz;
+        a.plane_aligned = (int64_t(g.nx) * g.ny) % int64_t(kChunk<S>) == 0 ? 1 : 0;
         a.tiles_x = tiles_x_;
         a.nblocks = nitems_[part];
         a.xcd_groups = xcd_;

@@ -509,6 +509,79 @@ __device__ __forceinline__ void store_node(const FieldSet<S> &fs, uint32_t idx, 
     store_raw<S>(fs, idx, v);
 }
 
+// Planes that hold a whole number of chunks (nx ny a multiple of kChunk) put node `off` of every
+// plane at the same byte offsets from the plane's first element, so a march forms them once:
+// .a for the quads (fields 0-3, and 4-7 at .a + 4 kChunk values) or FP64's first pair (pair j at
+// .a + 2 j kChunk values), .b for the 8-9 pair of the quad layout.
+struct LaneOffsets {
+    uint32_t a, b;
+};
+
+template <class S>
+__host__ __device__ inline LaneOffsets lane_offsets(uint32_t off) {
+    const uint32_t cb = (off / kChunk<S>) * (10u * kChunk<S>), l = off % kChunk<S>;
+    if constexpr (kQuads<S>)
+        return {uint32_t((cb + 4u * l) * sizeof(S)), uint32_t((cb + 8u * kChunk<S> + 2u * l) * sizeof(S))};
+    else
+        return {uint32_t((cb + 2u * l) * sizeof(S)), 0u};
+}
+
+// Addresses of a node's stored words: .a the first quad (FP32 and 16-bit storage; the second is
+// 4 kChunk values on) or the first pair (FP64; pair j is 2 j kChunk values on), .b the 8-9 pair of
+// the quad layout. From the node index, or from a plane of whole chunks and the node's offsets in it.
+struct NodePtrs {
+    const char *a, *b;
+};
+
+template <class S>
+__device__ __forceinline__ NodePtrs node_ptrs(const S *base, uint32_t idx) {
+    const char *cb = reinterpret_cast<const char *>(base + chunk_base<S>(idx));
+    const uint32_t l = idx % kChunk<S>;
+    if constexpr (kQuads<S>)
+        return {cb + size_t(4 * l) * sizeof(S), cb + size_t(8 * kChunk<S> + 2 * l) * sizeof(S)};
+    else
+        return {cb + size_t(2 * l) * sizeof(S), nullptr};
+}
+
+template <class S>
+__device__ __forceinline__ NodePtrs node_ptrs_at(const S *plane, LaneOffsets o) {
+    const char *b = reinterpret_cast<const char *>(plane);
+    return {b + o.a, b + o.b};
+}
+
+template <class S>
+__device__ __forceinline__ void load_raw_p(NodePtrs p, RawNode<S> &r) {
+    if constexpr (kQuads<S>) {
+        r.q0 = *reinterpret_cast<const quad_t<S> *>(p.a);
+        r.q1 = *reinterpret_cast<const quad_t<S> *>(p.a + 4 * kChunk<S> * sizeof(S));
+        r.p = *reinterpret_cast<const pair_t<S> *>(p.b);
+    } else {
+#pragma unroll
+        for (int j = 0; j < 5; ++j) r.p[j] = *reinterpret_cast<const pair_t<S> *>(p.a + size_t(j) * 2 * kChunk<S> * sizeof(S));
+    }
+}
+
+template <class S>
+__device__ __forceinline__ void store_raw_p(NodePtrs p, const S *v) {
+    char *a = const_cast<char *>(p.a);
+    if constexpr (kQuads<S>) {
+        *reinterpret_cast<quad_t<S> *>(a) = join_quad<S>(v);
+        *reinterpret_cast<quad_t<S> *>(a + 4 * kChunk<S> * sizeof(S)) = join_quad<S>(v + 4);
+        *reinterpret_cast<pair_t<S> *>(const_cast<char *>(p.b)) = join_pair<S>(v[8], v[9]);
+    } else {
+#pragma unroll
+        for (int j = 0; j < 5; ++j)
+            *reinterpret_cast<pair_t<S> *>(a + size_t(j) * 2 * kChunk<S> * sizeof(S)) = join_pair<S>(v[2 * j], v[2 * j + 1]);
+    }
+}
+
+template <class S>
+__device__ __forceinline__ void store_raw_p(NodePtrs p, const RawNode<S> &r) {
+    S v[10];
+    unpack_raw<S>(r, v);
+    store_raw_p<S>(p, v);
+}
+
 // Node-major buffers (the 10 values of node i at base[10 i .. 10 i + 9]): the donor snapshot,
 // read at scattered stencil corners, where one node's values then share a cache line. A node
 // starts on a pair-word boundary (10 i values) and moves as 5 pair words.
