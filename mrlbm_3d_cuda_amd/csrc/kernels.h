@@ -78,13 +78,17 @@ struct WallArgs {
     const uint32_t *__restrict__ node;
     const R *__restrict__ wnx;
     const R *__restrict__ wny;
-    // per-wall records, component-major (component j of wall i at j * nwall + i)
-    const R *__restrict__ coef;        // 3 extrapolation coefficients per wall
-    const R *__restrict__ wconst;      // WALL_CONSTS closure constants per wall
-    const uint32_t *__restrict__ in_mask;
+    const uint32_t *__restrict__ in_mask;  // per wall node
+    // per-wall records, component-major (component j of record r at j * nrec + r): one record per
+    // wall node, or, with col0 > 0 (z-extruded walls), one per wall of plane 0 -- col0 inner then
+    // col1 outer records -- whose node and stencil indices are offset by z * plane
+    const R *__restrict__ coef;        // 3 extrapolation coefficients
+    const R *__restrict__ wconst;      // WALL_CONSTS closure constants
     const uint32_t *__restrict__ out_mask;
-    const uint32_t *__restrict__ st_base;  // 2 per wall: index of the (x0, y0) stencil corner
-    const R *__restrict__ st_frac;         // 4 per wall: xd1, yd1, xd2, yd2
+    const uint32_t *__restrict__ st_base;  // 2: index of the (x0, y0) stencil corner
+    const R *__restrict__ st_frac;         // 4: xd1, yd1, xd2, yd2
+    int col0, col1;
+    uint32_t plane;                        // nx * ny
     int nx;                                // row pitch of the stencil corners
     const uint32_t *__restrict__ bcf_node;
     const uint8_t *__restrict__ bcf_tag;

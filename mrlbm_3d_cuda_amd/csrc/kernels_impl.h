@@ -971,8 +971,13 @@ __device__ __forceinline__ void interp_rotate(const WallArgs<R, S> &a, uint32_t 
     R q[4][9];
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
+        // the corner's ten stored values as five pair words (a node starts on a pair boundary)
+        const pair_t<S> *pw = reinterpret_cast<const pair_t<S> *>(a.donor + size_t(corner[j]) * 10);
+        S raw[10];
 #pragma unroll
-        for (int k = 0; k < 9; ++k) q[j][k] = stored_value<R, S>(a.donor[size_t(corner[j]) * 10 + 1 + k], k + 1, a.scale);
+        for (int w = 0; w < 5; ++w) split_pair<S>(pw[w], raw[2 * w], raw[2 * w + 1]);
+#pragma unroll
+        for (int k = 0; k < 9; ++k) q[j][k] = stored_value<R, S>(raw[1 + k], k + 1, a.scale);
         if constexpr (Conv<S>::scaled) {
             q[j][0] += a.scale.uref;
             // stored second moments are non-equilibrium parts: m = (m - u u) + u u is restored at
@@ -1031,33 +1036,44 @@ __device__ __forceinline__ void wall_body(const WallArgs<R, S> &a, int lb) {
         R M[10];
         load_node<R, S>(as_const(a.mom), idx, a.scale, a.rho0, M);  // rho_I, u_old, m'_I
         const R c = a.wnx[i], s = a.wny[i];
+        // record r of the node (component-major, nw records) and the plane offset of its stencils
+        int r = i, nw = a.nwall;
+        uint32_t zoff = 0;
+        if (a.col0 > 0) {
+            const bool outer = i >= a.nb_inner;
+            const int local = outer ? i - a.nb_inner : i;
+            const int nwp = outer ? a.col1 : a.col0;
+            const int z = local / nwp;
+            r = (outer ? a.col0 : 0) + (local - z * nwp);
+            nw = a.col0 + a.col1;
+            zoff = uint32_t(z) * a.plane;
+        }
         R vm1[9], vm2[9];
-        const int nw = a.nwall;  // per-wall records are component-major
-        interp_rotate(a, a.st_base[i], a.st_frac[i], a.st_frac[nw + i], c, s, vm1);
-        interp_rotate(a, a.st_base[nw + i], a.st_frac[2 * nw + i], a.st_frac[3 * nw + i], c, s, vm2);
+        interp_rotate(a, a.st_base[r] + zoff, a.st_frac[r], a.st_frac[nw + r], c, s, vm1);
+        interp_rotate(a, a.st_base[nw + r] + zoff, a.st_frac[2 * nw + r], a.st_frac[3 * nw + r], c, s, vm2);
         const int g = i < a.nb_inner ? 0 : 1;
         const R u0 = a.uw[g][0], u1 = a.uw[g][1], u2 = a.uw[g][2];
         // wall values u'_w and m'_w = u'_w u'_w (curvedLBM.cu:93-101)
         const R vw[7] = {u0, u1, u2, u0 * u0, u1 * u1, u2 * u2, u1 * u2};
         constexpr int sel[7] = {0, 1, 2, 3, 4, 5, 8};
-        const R c0 = a.coef[i], c1 = a.coef[nw + i], c2 = a.coef[2 * nw + i];
+        const R c0 = a.coef[r], c1 = a.coef[nw + r], c2 = a.coef[2 * nw + r];
         R p[7];
 #pragma unroll
         for (int j = 0; j < 7; ++j) p[j] = c0 * vw[j] + c1 * vm1[sel[j]] + c2 * vm2[sel[j]];
         R out[10];
         if (a.strong) {
-            wall_closure_strong(c, s, a.in_mask[i], a.out_mask[i], a.omega, p, M[0], M[7], M[8], out);
+            wall_closure_strong(c, s, a.in_mask[i], a.out_mask[r], a.omega, p, M[0], M[7], M[8], out);
         } else if constexpr (kExactOrder<R>) {
             wall_closure_loop(c, s, a.in_mask[i], p, M[0], M[7], M[8], out);
         } else {
             R k[WALL_CONSTS];
 #pragma unroll
-            for (int j = 0; j < WALL_CONSTS; ++j) k[j] = a.wconst[size_t(j) * nw + i];
+            for (int j = 0; j < WALL_CONSTS; ++j) k[j] = a.wconst[size_t(j) * nw + r];
             wall_closure(c, s, k, p, M[0], M[7], M[8], out);
         }
         if (a.forces) masked_force(out, a.in_mask[i], 1.0, F);
         collide(a.omega, out);
-        if (a.forces) masked_force(out, a.out_mask[i], -1.0, F);
+        if (a.forces) masked_force(out, a.out_mask[r], -1.0, F);
         uint32_t seed = 0;
         if constexpr (Conv<S>::scaled) seed = uint32_t(*a.step_counter) * 2u + 2u;
         store_node<R, S>(a.mom, idx, a.scale, a.rho0, out, seed);

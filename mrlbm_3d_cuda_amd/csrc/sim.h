@@ -279,12 +279,53 @@ class Sim final : public SimBase {
         std::vector<R> inlet(std::max<size_t>(g.inlet.size(), 1), R(0));
         for (size_t k = 0; k < g.inlet.size(); ++k) inlet[k] = R(g.inlet[k]);
 
-        // per-wall records go to the device component-major (entry j of wall i at j * nwall + i) so
+        // z-extruded walls (the cylinder and annulus cases): every plane lists the same wall columns
+        // in the same order with the same normals, depths and masks. The wall pass then reads the
+        // records of one plane's walls (a few hundred KB that stay in L2) and offsets the node and
+        // stencil indices by the plane, instead of reading one record per wall node and step.
+        wall_cols_[0] = wall_cols_[1] = 0;
+        {
+            bool cols = g.ngroups > 0;
+            const int64_t plane = int64_t(g.nx) * g.ny;
+            for (int gi = 0; gi < g.ngroups && cols; ++gi) {
+                const WallTable &w = g.wall[gi];
+                const size_t n = w.node.size();
+                cols = n % size_t(g.nz) == 0;
+                const size_t nwp = cols ? n / size_t(g.nz) : 1;
+                for (size_t k = nwp; k < n && cols; ++k) {
+                    const size_t k0 = k % nwp;
+                    cols = w.node[k] == w.node[k0] + int64_t(k / nwp) * plane && w.nx[k] == w.nx[k0] &&
+                           w.ny[k] == w.ny[k0] && w.delta[k] == w.delta[k0] && w.in_mask[k] == w.in_mask[k0] &&
+                           w.out_mask[k] == w.out_mask[k0];
+                }
+                wall_cols_[gi] = int(nwp);
+            }
+            if (!cols) wall_cols_[0] = wall_cols_[1] = 0;
+        }
+        // the records of the walls that the wall pass reads (every wall, or one plane's)
+        const int nrec = wall_cols_[0] > 0 ? wall_cols_[0] + wall_cols_[1] : nwall;
+        if (wall_cols_[0] > 0) {
+            auto pick = [&](const auto &v, int per) {
+                std::decay_t<decltype(v)> out;
+                for (int gi = 0; gi < 2; ++gi)
+                    for (int k = 0; k < wall_cols_[gi]; ++k) {
+                        const size_t i = size_t(gi == 0 ? 0 : nb_inner_) + size_t(k);
+                        for (int j = 0; j < per; ++j) out.push_back(v[i * per + j]);
+                    }
+                return out;
+            };
+            w_const = pick(w_const, WALL_CONSTS);
+            w_coef = pick(w_coef, 3);
+            st_frac = pick(st_frac, 4);
+            st_base = pick(st_base, 2);
+            w_out = pick(w_out, 1);
+        }
+        // per-wall records go to the device component-major (entry j of record i at j * nrec + i) so
         // that the wall pass reads them with coalesced loads
-        auto soa = [nwall](const auto &v, int per) {
+        auto soa = [nrec](const auto &v, int per) {
             std::decay_t<decltype(v)> out(v.size());
-            for (int i = 0; i < nwall; ++i)
-                for (int j = 0; j < per; ++j) out[size_t(j) * nwall + i] = v[size_t(i) * per + j];
+            for (int i = 0; i < nrec; ++i)
+                for (int j = 0; j < per; ++j) out[size_t(j) * nrec + i] = v[size_t(i) * per + j];
             return out;
         };
         w_const = soa(w_const, WALL_CONSTS);
@@ -672,6 +713,9 @@ class Sim final : public SimBase {
         w.st_base = d_stbase_;
         w.st_frac = d_stfrac_;
         w.nx = g.nx;
+        w.col0 = wall_cols_[0];
+        w.col1 = wall_cols_[1];
+        w.plane = uint32_t(int64_t(g.nx) * g.ny);
         w.bcf_node = d_bcfnode_;
         w.bcf_tag = d_bcftag_;
         w.bcf_in = d_bcfin_;
@@ -1140,6 +1184,7 @@ class Sim final : public SimBase {
     std::vector<std::vector<uint2>> lists_;
     LaunchConfig launch_;
     int nwall_ = 0, nb_inner_ = 0, nbcf_ = 0;
+    int wall_cols_[2] = {0, 0};  // wall records per plane of each group (0: a record per wall node)
     uint32_t *d_info_ = nullptr;
     uint2 *d_items_[3] = {nullptr, nullptr, nullptr};  // every item, interior items, wall-side items
     R *d_wconst_ = nullptr;
