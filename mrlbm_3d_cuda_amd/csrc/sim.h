@@ -462,12 +462,12 @@ class Sim final : public SimBase {
             }
         // y slabs: each XCD owns whole rows of tiles over the full z range and runs them in rounds
         // of one z chunk, so that a chunk's halo planes are still in that XCD's L2 from the round
-        // before (one z chunk for every tile: lz_special does not apply)
+        // before (boundary tiles cut their part of a round into chunks of lz_special)
         const bool yslab = lc.xcd_remap && lc.slab_axis == 1 && tiles_y >= 8;
         const int groups = yslab ? 8 : ((lc.xcd_remap && g.nz >= 32) ? 8 : 1);
         const int slab = yslab ? g.nz : (g.nz + groups - 1) / groups;
         const int lz = std::max(1, std::min(slab, lc.lz));
-        const int lzs = yslab ? lz : (lc.lz_special > 0 ? std::max(1, std::min(slab, lc.lz_special)) : lz);
+        const int lzs = lc.lz_special > 0 ? std::max(1, std::min(yslab ? lz : slab, lc.lz_special)) : lz;
         // lists[0]: every item; lists[1]: interior tiles; lists[2]: wall-side tiles
         std::vector<std::vector<uint2>> lists[3];
         for (auto &l : lists) l.resize(groups);
@@ -485,13 +485,18 @@ class Sim final : public SimBase {
         for (int gi = 0; gi < groups; ++gi) {
             if (yslab) {
                 const int ta = (gi * tiles_y / groups) * tiles_x, tb = ((gi + 1) * tiles_y / groups) * tiles_x;
+                // a tile's part [z, ze) of a round, in chunks of lz (lz_special for boundary tiles)
+                auto add_round = [&](int t, int z) {
+                    const int ze = std::min(g.nz, z + lz), step = tile_special[t] ? lzs : lz;
+                    for (int zz = z; zz < ze; zz += step) add(gi, t, zz, std::min(ze, zz + step));
+                };
                 for (int z = 0; z < g.nz; z += lz)  // the rounds: passes 0 and 1
                     for (int pass = 0; pass < 2; ++pass)
                         for (int t = ta; t < tb; ++t)
-                            if (pass_of(t) == pass && tile_live[t]) add(gi, t, z, std::min(g.nz, z + lz));
+                            if (pass_of(t) == pass && tile_live[t]) add_round(t, z);
                 for (int z = 0; z < g.nz; z += lz)  // then pass 2, after every round
                     for (int t = ta; t < tb; ++t)
-                        if (pass_of(t) == 2 && tile_live[t]) add(gi, t, z, std::min(g.nz, z + lz));
+                        if (pass_of(t) == 2 && tile_live[t]) add_round(t, z);
                 continue;
             }
             const int za = gi * slab, zb = std::min(g.nz, za + slab);
