@@ -180,11 +180,13 @@ __device__ __forceinline__ void boundary_node(const FusedArgs<R, S> &a, R *f, co
             for (int k = 0; k < 10; ++k) M[k] = old[k];
             store = code == NC_KEEP;  // frozen: both buffers already hold its value
         } else if (code == NC_WALL) {
-            // u keeps its old value; rho and m become the incoming rotated moments
+            // u keeps its old value; rho and m become the incoming rotated moments, handed to the
+            // wall pass through the node's donor snapshot (the wall pass stores the node)
 #pragma unroll
             for (int k = 0; k < 10; ++k) M[k] = old[k];
             incoming_rotated(t.wnx, t.wny, t.win, f, M);
-            do_collide = false;  // finished by wall_step
+            do_collide = false;
+            store = false;
         } else if (code == NC_BCF) {
             bc_fluid(payload, t.bcfin, a.omega, a.bcf_strong != 0, f, M);
         } else if (code == NC_WEST) {
@@ -231,7 +233,7 @@ __device__ __forceinline__ void boundary_node(const FusedArgs<R, S> &a, R *f, co
             bc_solid(payload, a.omega, f, M);  // NC_BCS
         }
     }
-    donor = store && (info & INFO_DONOR);
+    donor = (store || code == NC_WALL) && (info & INFO_DONOR);
 }
 
 // State and steps of the boundary-tile path shared by both marches. Per thread: the stored
@@ -1034,7 +1036,7 @@ __device__ __forceinline__ void wall_body(const WallArgs<R, S> &a, int lb) {
     if (i < a.nwall) {
         const uint32_t idx = a.node[i];
         R M[10];
-        load_node<R, S>(as_const(a.mom), idx, a.scale, a.rho0, M);  // rho_I, u_old, m'_I
+        load_node_nm<R, S>(a.donor, idx, a.scale, a.rho0, M);  // rho_I, u_old, m'_I (the fused pass's snapshot)
         const R c = a.wnx[i], s = a.wny[i];
         // record r of the node (component-major, nw records) and the plane offset of its stencils
         int r = i, nw = a.nwall;
