@@ -385,12 +385,6 @@ class Sim final : public SimBase {
         constexpr size_t kLastLevelCacheBytes = size_t(256) << 20;
         const bool cache_resident = 2 * field_buffer_elems<S>(g.n) * sizeof(S) <= kLastLevelCacheBytes;
         if (sizeof(R) == 4 && !Conv<S>::scaled && cache_resident) lc.bx = 16;
-        // FP32 storage streaming from HBM is bound by its traffic: its boundary tiles keep their
-        // place in tile order, so an inlet or outlet column runs beside its neighbours and they
-        // share ring lines in L2 (512x256x256 wake: 9% fewer L2 read requests, +2.5% MLUPS); the
-        // arithmetic-bound forms run the slower boundary tiles first, so that they form no tail
-        // (16-bit storage on the same wake: first +2% over tile order)
-        if (sizeof(R) == 4 && !Conv<S>::scaled && !cache_resident) lc.special_order = 1;
         // register-window march for FP32 arithmetic: with 16-bit storage one LDS plane, whose
         // four blocks per CU hide the decode and encode work better than the two-plane form's two;
         // with FP32 storage two LDS planes (one barrier per plane) on grids that stream from HBM
@@ -401,11 +395,18 @@ class Sim final : public SimBase {
         // latency-bound, so small grids need many short chunks), at most 32 planes
         const int64_t tiles = int64_t((g.nx + lc.bx - 1) / lc.bx) * ((g.ny + lc.by - 1) / lc.by);
         lc.lz = int(std::max<int64_t>(1, std::min<int64_t>(32, tiles * g.nz / 1024)));
-        // boundary tiles in chunks of the same length, except for FP32 storage streaming from HBM,
-        // whose boundary tiles (in tile order) march half chunks, which finish nearer their
-        // neighbours (512x256x256 wake: fused pass 608 -> 601 us)
-        lc.lz_special = (sizeof(R) == 4 && !Conv<S>::scaled && !cache_resident) ? std::max(1, lc.lz / 2) : 0;
+        lc.lz_special = 0;  // boundary tiles in chunks of the same length
         lc.xcd_remap = true;
+        // FP32 arithmetic on grids that stream from HBM: y slabs -- each XCD marches a band of
+        // tile rows (cut to equal estimated work) through z in rounds, so a chunk's halo planes
+        // were loaded by the same XCD one round before -- with boundary tiles first in each round
+        // in half chunks (512x256x256 wake, A/B on one box: FP32 fused pass 608.5 -> 592.1 us,
+        // BF16 387.4 -> 382.3 us against z slabs)
+        if (sizeof(R) == 4 && !cache_resident && (g.ny + lc.by - 1) / lc.by >= 8) {
+            lc.slab_axis = 1;
+            lc.special_order = 0;
+            lc.lz_special = std::max(1, lc.lz / 2);
+        }
         return lc;
     }
 
@@ -466,7 +467,7 @@ class Sim final : public SimBase {
                         tile_wall[xx / lc.bx + tiles_x * (yy / lc.by)] = 1;
                     }
             }
-        // y slabs: each XCD owns whole rows of tiles over the full z range and runs them in rounds
+        // y slabs: each XCD owns a band of tile rows over the full z range and runs it in rounds
         // of one z chunk, so that a chunk's halo planes are still in that XCD's L2 from the round
         // before (boundary tiles cut their part of a round into chunks of lz_special)
         const bool yslab = lc.xcd_remap && lc.slab_axis == 1 && tiles_y >= 8;
@@ -488,9 +489,27 @@ class Sim final : public SimBase {
         // pass of a tile in its slab (or z round): boundary tiles in pass 0 (first), in pass 1 with
         // the others (tile order), or in pass 2 (last; in a y slab after every round)
         auto pass_of = [&](int t) { return tile_special[t] ? lc.special_order : 1; };
+        // y slabs: the tiles, in row-major order, cut into one contiguous range per XCD of equal
+        // estimated work, a tile running the boundary path counting 1.5 plain tiles (FP32 storage)
+        // or 2 (16-bit storage and FP64), as measured per plane; rows holding a curved wall would
+        // otherwise load their XCDs alone
+        std::vector<int> yrange(groups + 1, 0);
+        if (yslab) {
+            const double ws = (sizeof(R) == 4 && !Conv<S>::scaled) ? 1.5 : 2.0;
+            std::vector<double> prefix(ntiles + 1, 0.0);
+            for (int t = 0; t < ntiles; ++t)
+                prefix[t + 1] = prefix[t] + (!tile_live[t] ? 0.0 : (tile_special[t] && face_kind(t) == FACE_NONE ? ws : 1.0));
+            for (int k = 1; k < groups; ++k) {
+                const double goal = prefix[ntiles] * k / groups;
+                int t = yrange[k - 1];
+                while (t < ntiles && prefix[t + 1] <= goal) ++t;
+                yrange[k] = t;
+            }
+            yrange[groups] = ntiles;
+        }
         for (int gi = 0; gi < groups; ++gi) {
             if (yslab) {
-                const int ta = (gi * tiles_y / groups) * tiles_x, tb = ((gi + 1) * tiles_y / groups) * tiles_x;
+                const int ta = yrange[gi], tb = yrange[gi + 1];
                 // a tile's part [z, ze) of a round, in chunks of lz (lz_special for boundary tiles)
                 auto add_round = [&](int t, int z) {
                     const int ze = std::min(g.nz, z + lz), step = tile_special[t] ? lzs : lz;
