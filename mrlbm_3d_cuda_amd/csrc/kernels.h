@@ -266,9 +266,12 @@ inline bool fused_tile_supported(int bx, int by) {
 // variant 1), or one / two planes of all 27 pulled populations (register-window march,
 // variants 3 / 4); then the moments of the tile's west ring column for three planes.
 inline size_t fused_lds_bytes(int bx, int by, size_t real_bytes, int variant) {
-    const size_t planes = variant == 3 ? 27 : 54;
+    // ring march: 54 population slots of every frame position; register-window march: one or two
+    // planes of nine (c_x, c_z) blocks of three population rows-sets and a spare row (WindowLds)
+    const size_t values = variant == 1 ? 54 * size_t(bx + 2) * size_t(by + 2)
+                                       : size_t(variant == 3 ? 1 : 2) * 9 * size_t(bx + 2) * size_t(3 * by + 1);
     // + the west ring column's moments of three planes (boundary tiles)
-    return (planes * size_t(bx + 2) * size_t(by + 2) + 30 * size_t(by)) * real_bytes;
+    return (values + 30 * size_t(by)) * real_bytes;
 }
 
 }  // namespace mrlbm

@@ -450,6 +450,35 @@ __device__ __forceinline__ void fused_march(const FusedArgs<R, S> &a, R *lds, in
     if constexpr (!SPECIAL) store_collided<R, S>(a.dst, idxp, a.scale, a.rho0, Mp, seed);
 }
 
+// Registers are capped at four waves per SIMD for the one-plane march of 16-bit storage in 32 x 8
+// tiles (128 VGPRs: the plain and face marches fit, the boundary path spills about 44 bytes per
+// lane), three waves for the other one-plane marches (about 156 VGPRs in use, no spills: capped at
+// four, the boundary-heavy 16 x 8 FP32 and batched forms spilled 64-132 bytes per lane and ran up to
+// 60% longer) and two for the two-plane march (512-thread blocks: one per CU). A cap of five waves
+// (96 VGPRs) made the plain march spill too.
+template <class S, int BX, int BY, int NBUF>
+constexpr int window_min_blocks() {
+    return BX * BY > 256 ? 1 : (NBUF == 2 ? 2 : ((Conv<S>::scaled && BX * BY == 256) ? 4 : 3));
+}
+
+// LDS plane of the register-window march, laid out by the rows each population is pulled from:
+// for every (c_x, c_z) a block of the c_y = +1 populations (frame rows 0..BY-1), one spare row, the
+// c_y = -1 populations (rows 2..BY+1) and the c_y = 0 populations (rows 1..BY). Population q of
+// frame position p sits at off(q) + p. Writes that no node pulls -- the c_y = +1 populations of the
+// last tile row and of the west / east ring column's last row, the c_y = -1 ones of the first rows --
+// land in the spare row between the c_y = +1 and c_y = -1 groups of the block.
+template <int BX, int BY>
+struct WindowLds {
+    static constexpr int W = BX + 2;
+    static constexpr int NP = BY * W;     // one population's values of a plane
+    static constexpr int B = 3 * NP + W;  // one (c_x, c_z) block
+    static constexpr int PLANE = 9 * B;   // one LDS plane
+    static constexpr int cyoff(int cy) { return cy > 0 ? 0 : (cy < 0 ? NP - W : 2 * NP); }
+    static constexpr int off(int q) {
+        return (3 * (LATTICE.cz[q] + 1) + (LATTICE.cx[q] + 1)) * B + cyoff(LATTICE.cy[q]);
+    }
+};
+
 // Register-window march: LDS holds NBUF planes of the 27 populations pulled in one iteration;
 // every position keeps in registers the c_z=+1 populations of its last two planes and the c_z=0
 // populations of its last plane, and writes them into LDS in the iteration that pulls them.
@@ -468,7 +497,9 @@ __device__ __forceinline__ void window_march(const FusedArgs<R, S> &a, R *lds, i
     using F = Frame<BX, BY>;
     constexpr int W = F::W;
     constexpr int NH = F::NH;
-    R *const wold = lds + 27 * NBUF * NH;
+    using L = WindowLds<BX, BY>;
+    constexpr bool kReloadOwn = window_min_blocks<S, BX, BY, NBUF>() == 4;
+    R *const wold = lds + NBUF * L::PLANE;
 
     const int x0 = (tile % a.tiles_x) * BX;
     const int y0 = (tile / a.tiles_x) * BY;
@@ -489,21 +520,24 @@ __device__ __forceinline__ void window_march(const FusedArgs<R, S> &a, R *lds, i
     // FP32 arithmetic: a ring position reconstructs only the nine populations that stream from it
     // into the tile -- c_y = +1 from the south row, -1 from the north row (corners included), c_x =
     // +1 / -1 from the west / east column -- and keeps windows of three per c_z group. roff[j]: its
-    // LDS offsets of the c_z = -1 populations with c_b = j - 1 (lds_slot order; + 9 NH per c_z step).
+    // LDS offset of the c_z = -1 population with c_b = j - 1 (+ 3 B per c_z step).
     constexpr bool RING9 = !kExactOrder<R>;
     constexpr int RW = RING9 ? 3 : 9;
     const bool ring_row = hy1 == 0 || hy1 == BY + 1;
     const int ring_si = ring_row ? (hy1 == 0 ? 1 : -1) : (hx1 == 0 ? 1 : -1);
     int roff[3];
 #pragma unroll
-    for (int j = 0; j < 3; ++j) roff[j] = (ring_row ? 3 * (ring_si + 1) + j : 3 * j + (ring_si + 1)) * NH + p1;
+    for (int j = 0; j < 3; ++j)
+        roff[j] = (ring_row ? j * L::B + L::cyoff(ring_si) : (ring_si + 1) * L::B + L::cyoff(j - 1)) + p1;
 
     // stored values of the plane in flight, per position (set B only holds plane z0 in the
     // prologue, which loads it together with plane z0-1)
     RawNode<S> ra0, ra1, rb0, rb1;
     // planes of whole chunks and a grid of whole tiles (every thread's node inside): loads and the
     // plain march's stores address a position by its fixed offsets from the plane's first element
-    const bool fast = a.plane_aligned && a.nx % BX == 0 && a.ny % BY == 0;
+    // (two-plane march only: in the one-plane march the offsets took 23 more registers and gained
+    // nothing)
+    const bool fast = NBUF == 2 && a.plane_aligned && a.nx % BX == 0 && a.ny % BY == 0;
     const size_t plane_elems = size_t(plane) * 10;
     const LaneOffsets lo0 = lane_offsets<S>(off0), lo1 = lane_offsets<S>(off1);
     auto load_plane_to = [&](int z, RawNode<S> &d0, RawNode<S> &d1) {
@@ -587,7 +621,7 @@ __device__ __forceinline__ void window_march(const FusedArgs<R, S> &a, R *lds, i
         }
         if (has1) recon_ring(rb1, west1, z0, pb1, q1, nullptr);
         if constexpr (SPECIAL) {
-            bp.copy_own(rb0, bp.own_cur);
+            if constexpr (!kReloadOwn) bp.copy_own(rb0, bp.own_cur);
             bp.load_info(a, off0 + uint32_t(z0) * plane, bp.info_cur);
         }
     }
@@ -599,10 +633,13 @@ __device__ __forceinline__ void window_march(const FusedArgs<R, S> &a, R *lds, i
     if constexpr (Conv<S>::scaled) {
         if (a.step_counter) seed = uint32_t(*a.step_counter) * 2u + 1u;
     }
+    // the plain march stores a plane's results at the start of the next iteration (after the wait
+    // for the prefetched plane, before the next prefetch)
+    constexpr bool kDelayStore = true;
     R Mp[10];
     uint32_t idxp = a.pad;
     int zprev = z0;  // plane of Mp
-    // the plain march's store of the previous plane's results (none before the first plane)
+    // the plain march's store of the last results (with kDelayStore: none before the first plane)
     auto store_prev = [&](bool any) {
         if (fast && !any) return;
         NodePtrs q = fast ? node_ptrs_at(static_cast<const S *>(a.dst.base + size_t(zprev) * plane_elems), lo0)
@@ -620,14 +657,18 @@ __device__ __forceinline__ void window_march(const FusedArgs<R, S> &a, R *lds, i
                             : FACE == FACE_NORTH ? y == a.ny - 1
                                                  : y == 0);
     // one position: reconstruct plane z+1, write the pulled plane's groups, advance the windows
-    auto put = [&](const RawNode<S> &raw, int p, R *G, R *pold, R *q, bool west, int zp, R *keep = nullptr) {
+    // whether a ring position hands population q to the tile (the other populations of a ring
+    // position are never pulled, and the LDS plane has no room for them: WindowLds)
+    auto ring_takes = [&](int qd) { return ring_row ? LATTICE.cy[qd] == ring_si : LATTICE.cx[qd] == ring_si; };
+    auto put = [&](const RawNode<S> &raw, int p, R *G, R *pold, R *q, bool west, int zp, R *keep = nullptr,
+                   bool ring = false) {
         R f[Q];
         recon(raw, f, west, zp, keep);
 #pragma unroll
         for (int i = 0; i < 9; ++i) {
-            G[lds_slot(GROUP_ZP[i]) * NH + p] = pold[i];  // plane z-1
-            G[lds_slot(GROUP_Z0[i]) * NH + p] = q[i];     // plane z
-            G[lds_slot(GROUP_ZM[i]) * NH + p] = f[GROUP_ZM[i]];  // plane z+1
+            if (!ring || ring_takes(GROUP_ZP[i])) G[L::off(GROUP_ZP[i]) + p] = pold[i];          // plane z-1
+            if (!ring || ring_takes(GROUP_Z0[i])) G[L::off(GROUP_Z0[i]) + p] = q[i];             // plane z
+            if (!ring || ring_takes(GROUP_ZM[i])) G[L::off(GROUP_ZM[i]) + p] = f[GROUP_ZM[i]];   // plane z+1
         }
 #pragma unroll
         for (int i = 0; i < 9; ++i) {
@@ -642,8 +683,8 @@ __device__ __forceinline__ void window_march(const FusedArgs<R, S> &a, R *lds, i
             recon_ring(raw, west, zp, gzp, gz0, gzm);
 #pragma unroll
             for (int j = 0; j < 3; ++j) {
-                G[roff[j] + 18 * NH] = pold[j];
-                G[roff[j] + 9 * NH] = q[j];
+                G[roff[j] + 6 * L::B] = pold[j];
+                G[roff[j] + 3 * L::B] = q[j];
                 G[roff[j]] = gzm[j];
             }
 #pragma unroll
@@ -652,7 +693,7 @@ __device__ __forceinline__ void window_march(const FusedArgs<R, S> &a, R *lds, i
                 q[j] = gz0[j];
             }
         } else {
-            put(raw, p1, G, pold, q, west, zp);
+            put(raw, p1, G, pold, q, west, zp, nullptr, true);
         }
     };
     auto iteration = [&](int z, R *G, R *w0, R *w1, bool stored_before) {
@@ -660,10 +701,17 @@ __device__ __forceinline__ void window_march(const FusedArgs<R, S> &a, R *lds, i
         put(ra0, h, G, w0, q0, false, z + 1, fo_nx);
         if (has1) put_ring(ra1, G, w1, q1, west1, z + 1);
         if constexpr (SPECIAL) {
-            bp.copy_own(ra0, bp.own_nx);
+            // the own node's stored values of plane z: kept from the prefetch of plane z (two
+            // planes of them held in registers), or, in the register-capped march, loaded again
+            // (an L2 hit: loaded two iterations ago) before the prefetch, so that the wait for
+            // them does not cover it
+            if constexpr (kReloadOwn)
+                load_raw(a.src, off0 + uint32_t(z) * plane, bp.own_cur);
+            else
+                bp.copy_own(ra0, bp.own_nx);
             load_tabs(a, bp.info_cur, uint32_t(y) + uint32_t(a.ny) * uint32_t(z), bp.tab);
             bp.load_info(a, off0 + uint32_t(pmod(z + 1, a.nz)) * plane, bp.info_nx);
-        } else {
+        } else if constexpr (kDelayStore) {
             store_prev(stored_before);
         }
         // the inlet velocity of the plane (every thread loads an in-range entry)
@@ -674,11 +722,11 @@ __device__ __forceinline__ void window_march(const FusedArgs<R, S> &a, R *lds, i
         lds_barrier();
         R f[Q];
 #pragma unroll
-        for (int q = 0; q < Q; ++q) f[q] = G[lds_slot(q) * NH + h - LATTICE.cx[q] - LATTICE.cy[q] * W];
+        for (int q = 0; q < Q; ++q) f[q] = G[L::off(q) + h - LATTICE.cx[q] - LATTICE.cy[q] * W];
         const uint32_t node = off0 + uint32_t(z) * plane;
         if constexpr (SPECIAL) {
             bp.update(a, f, tx, ty, inside, east_tile, node, wold + west_slot(z) * 10 * BY, seed);
-            bp.copy_own(bp.own_nx, bp.own_cur);
+            if constexpr (!kReloadOwn) bp.copy_own(bp.own_nx, bp.own_cur);
             bp.info_cur = bp.info_nx;
         } else {
             moments_lds(f, Mp);
@@ -709,6 +757,7 @@ __device__ __forceinline__ void window_march(const FusedArgs<R, S> &a, R *lds, i
             watch_finite(a, Mp[0]);
             idxp = inside ? node : a.pad;
             zprev = z;
+            if constexpr (!kDelayStore) store_prev(true);
         }
         if constexpr (FACE == FACE_WEST || FACE == FACE_EAST) {
 #pragma unroll
@@ -716,7 +765,7 @@ __device__ __forceinline__ void window_march(const FusedArgs<R, S> &a, R *lds, i
         }
     };
     R *const G0 = lds;
-    R *const G1 = NBUF == 2 ? lds + 27 * NH : lds;
+    R *const G1 = NBUF == 2 ? lds + L::PLANE : lds;
     int z = z0;
     iteration(z, G0, pa0, pa1, false);
     ++z;
@@ -726,7 +775,7 @@ __device__ __forceinline__ void window_march(const FusedArgs<R, S> &a, R *lds, i
         iteration(z, G0, pa0, pa1, true);
         ++z;
     }
-    if constexpr (!SPECIAL) store_prev(true);
+    if constexpr (!SPECIAL && kDelayStore) store_prev(true);
     if (NBUF == 1) lds_barrier();  // the block's LDS is free again for the next item
 }
 
@@ -763,10 +812,8 @@ __device__ __forceinline__ void window_item(const FusedArgs<R, S> &a, R *lds, ui
 // PART: which launch of the fused pass (PART_ALL, PART_INTERIOR, PART_WALL_SIDE). The interior
 // launch has no boundary tile, so the boundary path is not compiled into it; the two parts are
 // separate instantiations so that profiles tell the launches apart.
-// One LDS plane leaves room for three blocks per CU, two planes for two: registers are capped to
-// match (blocks of 512 threads: one per CU).
 template <class R, class S, int BX, int BY, int NBUF, int PART>
-__global__ __launch_bounds__(BX *BY, (BX * BY > 256 ? 1 : (NBUF == 1 ? 3 : 2))) void fused_window_kernel(FusedArgs<R, S> a) {
+__global__ __launch_bounds__(BX *BY, (window_min_blocks<S, BX, BY, NBUF>())) void fused_window_kernel(FusedArgs<R, S> a) {
     extern __shared__ unsigned char lds_raw[];
     R *lds = reinterpret_cast<R *>(lds_raw);
     int b = blockIdx.x;

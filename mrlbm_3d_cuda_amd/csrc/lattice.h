@@ -55,9 +55,4 @@ constexpr int GROUP_ZP[9] = {5, 9, 11, 16, 18, 19, 22, 23, 25};
 constexpr int GROUP_Z0[9] = {0, 1, 2, 3, 4, 7, 8, 13, 14};
 constexpr int GROUP_ZM[9] = {6, 10, 12, 15, 17, 20, 21, 24, 26};
 
-// Slot of direction q in an LDS plane of the register-window march: 9 (cz + 1) + 3 (cy + 1) +
-// (cx + 1), so that the nine populations with one velocity component fixed sit at slots of one
-// arithmetic pattern (what a ring position of a tile writes).
-constexpr int lds_slot(int q) { return 9 * (LATTICE.cz[q] + 1) + 3 * (LATTICE.cy[q] + 1) + (LATTICE.cx[q] + 1); }
-
 }  // namespace mrlbm
