@@ -970,7 +970,7 @@ __device__ __forceinline__ void batch_item(const FusedArgs<R, S> *__restrict__ c
 // block stops at the end of the list; no block waits for another). Then one block per case that
 // files the case's previous record.
 template <class R, class S, int BX, int BY, int VARIANT>
-__global__ __launch_bounds__(BX *BY, (VARIANT == 3 ? 3 : 2)) void fused_batch_kernel(
+__global__ __launch_bounds__(BX *BY, (VARIANT == 3 ? window_min_blocks<S, BX, BY, 1>() : 2)) void fused_batch_kernel(
     const FusedArgs<R, S> *__restrict__ cases, const uint4 *__restrict__ items, int nitems, int groups, BatchQueue q) {
     extern __shared__ unsigned char lds_raw[];
     R *lds = reinterpret_cast<R *>(lds_raw);
