@@ -635,11 +635,10 @@ __device__ __forceinline__ void window_march(const FusedArgs<R, S> &a, R *lds, i
     }
     // the plain march stores a plane's results at the start of the next iteration (after the wait
     // for the prefetched plane, before the next prefetch)
-    constexpr bool kDelayStore = true;
     R Mp[10];
     uint32_t idxp = a.pad;
     int zprev = z0;  // plane of Mp
-    // the plain march's store of the last results (with kDelayStore: none before the first plane)
+    // the plain march's store of the last results (none before the first plane)
     auto store_prev = [&](bool any) {
         if (fast && !any) return;
         NodePtrs q = fast ? node_ptrs_at(static_cast<const S *>(a.dst.base + size_t(zprev) * plane_elems), lo0)
@@ -711,7 +710,7 @@ __device__ __forceinline__ void window_march(const FusedArgs<R, S> &a, R *lds, i
                 bp.copy_own(ra0, bp.own_nx);
             load_tabs(a, bp.info_cur, uint32_t(y) + uint32_t(a.ny) * uint32_t(z), bp.tab);
             bp.load_info(a, off0 + uint32_t(pmod(z + 1, a.nz)) * plane, bp.info_nx);
-        } else if constexpr (kDelayStore) {
+        } else {
             store_prev(stored_before);
         }
         // the inlet velocity of the plane (every thread loads an in-range entry)
@@ -757,7 +756,6 @@ __device__ __forceinline__ void window_march(const FusedArgs<R, S> &a, R *lds, i
             watch_finite(a, Mp[0]);
             idxp = inside ? node : a.pad;
             zprev = z;
-            if constexpr (!kDelayStore) store_prev(true);
         }
         if constexpr (FACE == FACE_WEST || FACE == FACE_EAST) {
 #pragma unroll
@@ -775,7 +773,7 @@ __device__ __forceinline__ void window_march(const FusedArgs<R, S> &a, R *lds, i
         iteration(z, G0, pa0, pa1, true);
         ++z;
     }
-    if constexpr (!SPECIAL && kDelayStore) store_prev(true);
+    if constexpr (!SPECIAL) store_prev(true);
     if (NBUF == 1) lds_barrier();  // the block's LDS is free again for the next item
 }
 
