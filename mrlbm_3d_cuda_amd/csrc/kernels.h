@@ -270,8 +270,8 @@ inline size_t fused_lds_bytes(int bx, int by, size_t real_bytes, int variant) {
     // planes of nine (c_x, c_z) blocks of three population rows-sets and a spare row (WindowLds)
     const size_t values = variant == 1 ? 54 * size_t(bx + 2) * size_t(by + 2)
                                        : size_t(variant == 3 ? 1 : 2) * 9 * size_t(bx + 2) * size_t(3 * by + 1);
-    // + the west ring column's moments of three planes (boundary tiles)
-    return (values + 30 * size_t(by)) * real_bytes;
+    // + the west ring column's density and velocity of three planes (boundary tiles)
+    return (values + 12 * size_t(by)) * real_bytes;
 }
 
 }  // namespace mrlbm

@@ -241,8 +241,8 @@ __device__ __forceinline__ void boundary_node(const FusedArgs<R, S> &a, R *f, co
 // planes and the table values of the plane being updated. The results of a plane are stored as
 // soon as they are computed (the bulk path keeps its results in registers until the next
 // iteration; here that would hold 20 more values through the boundary closed forms, which then
-// spill). The moments of the west ring column go through LDS (wold, three planes) so that a node
-// in the tile's first column can read its x-1 neighbour.
+// spill). The density and velocity of the west ring column go through LDS (wold, three planes) so
+// that a node in the tile's first column can read its x-1 neighbour.
 template <class R, class S, int BX, int BY>
 struct BoundaryPath {
     static constexpr int W = BX + 2;
@@ -252,11 +252,11 @@ struct BoundaryPath {
 
     __device__ void load_info(const FusedArgs<R, S> &a, uint32_t node, uint32_t &out) { out = a.info[node]; }
     __device__ void copy_own(const RawNode<S> &raw, RawNode<S> &dst) { dst = raw; }
-    // the x-1 neighbour's moments at the plane being updated: lane - 1 inside the tile row, the
-    // west ring column (LDS) for the first column
+    // the x-1 neighbour's density and velocity at the plane being updated (all the outlet closures
+    // read of it): lane - 1 inside the tile row, the west ring column (LDS) for the first column
     __device__ void neighbour_xm1(const R *old, int tx, int ty, const R *wold, R *oxm) {
 #pragma unroll
-        for (int k = 0; k < 10; ++k) {
+        for (int k = 0; k < 4; ++k) {
             const R v = __shfl(old[k], int(threadIdx.x) - 1, 64);
             oxm[k] = tx > 0 ? v : wold[k * BY + ty];
         }
@@ -269,15 +269,17 @@ struct BoundaryPath {
                            const R *wold, uint32_t seed) {
         // the node's own and x-1 moments are only read by boundary nodes: waves made of bulk
         // nodes skip their decode
-        R old[10], oxm[10];
+        R old[10], oxm[4];
         if (__any((info_cur & INFO_CODE_MASK) != NC_BULK)) {
             decode_node<R, S>(own_cur, a.scale, a.rho0, old);
 #pragma unroll
-            for (int k = 0; k < 10; ++k) oxm[k] = old[k];
+            for (int k = 0; k < 4; ++k) oxm[k] = old[k];
             if (east_tile) neighbour_xm1(old, tx, ty, wold, oxm);  // uniform per block
         } else {
 #pragma unroll
-            for (int k = 0; k < 10; ++k) old[k] = oxm[k] = R(0);
+            for (int k = 0; k < 10; ++k) old[k] = R(0);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) oxm[k] = R(0);
         }
         R M[10];
         bool store, donor, do_collide;
@@ -349,7 +351,7 @@ __device__ __forceinline__ void fused_march(const FusedArgs<R, S> &a, R *lds, in
         decode_node<R, S>(raw, a.scale, a.rho0, m);
         if (SPECIAL && west) {
 #pragma unroll
-            for (int k = 0; k < 10; ++k) wold[(west_slot(zp) * 10 + k) * BY + (hy1 - 1)] = m[k];
+            for (int k = 0; k < 4; ++k) wold[(west_slot(zp) * 4 + k) * BY + (hy1 - 1)] = m[k];
         }
         reconstruct_lds(m, f);
 #pragma unroll
@@ -433,7 +435,7 @@ __device__ __forceinline__ void fused_march(const FusedArgs<R, S> &a, R *lds, in
         pull(f);
         const uint32_t node = off0 + uint32_t(z) * plane;
         if constexpr (SPECIAL) {
-            bp.update(a, f, tx, ty, inside, east_tile, node, wold + west_slot(z) * 10 * BY, seed);
+            bp.update(a, f, tx, ty, inside, east_tile, node, wold + west_slot(z) * 4 * BY, seed);
         } else {
             moments_lds(f, Mp);
             collide_for_store<R, S>(a.omega, true, Mp);
@@ -568,7 +570,7 @@ __device__ __forceinline__ void window_march(const FusedArgs<R, S> &a, R *lds, i
         }
         if (SPECIAL && west) {
 #pragma unroll
-            for (int k = 0; k < 10; ++k) wold[(west_slot(zp) * 10 + k) * BY + (hy1 - 1)] = m[k];
+            for (int k = 0; k < 4; ++k) wold[(west_slot(zp) * 4 + k) * BY + (hy1 - 1)] = m[k];
         }
     };
     auto recon = [&](const RawNode<S> &raw, R *f, bool west, int zp, R *keep = nullptr) {
@@ -724,19 +726,19 @@ __device__ __forceinline__ void window_march(const FusedArgs<R, S> &a, R *lds, i
         for (int q = 0; q < Q; ++q) f[q] = G[L::off(q) + h - LATTICE.cx[q] - LATTICE.cy[q] * W];
         const uint32_t node = off0 + uint32_t(z) * plane;
         if constexpr (SPECIAL) {
-            bp.update(a, f, tx, ty, inside, east_tile, node, wold + west_slot(z) * 10 * BY, seed);
+            bp.update(a, f, tx, ty, inside, east_tile, node, wold + west_slot(z) * 4 * BY, seed);
             if constexpr (!kReloadOwn) bp.copy_own(bp.own_nx, bp.own_cur);
             bp.info_cur = bp.info_nx;
         } else {
             moments_lds(f, Mp);
             if constexpr (FACE != FACE_NONE) {
                 if (__any(face_lane)) {  // the face's closed form (cases/cylinder/boundaries.cuh:201-315)
-                    R g[Q], Mf[10], old[10], oxm[10];
+                    R g[Q], Mf[10], old[10], oxm[4];
 #pragma unroll
                     for (int q = 0; q < Q; ++q) g[q] = f[q];
                     lds_to_pop(g);
 #pragma unroll
-                    for (int k = 0; k < 10; ++k) old[k] = oxm[k] = R(0);
+                    for (int k = 0; k < 10; ++k) old[k] = R(0);
 #pragma unroll
                     for (int k = 0; k < 4; ++k) {
                         old[k] = fo_cur[k];
