@@ -253,6 +253,7 @@ struct LaunchConfig {
     int slab_axis = 2;      // axis cut into the XCD slabs: 2 = z, 1 = y (rows of tiles, z chunks in rounds)
     bool face_tiles = true;  // face marches for tiles whose only boundary nodes form one face
     int special_order = 0;  // boundary tiles: 0 first, 1 in tile order, 2 last (in a z slab; in a y slab per z round, or after every round)
+    double special_weight = 0;  // y slabs: plain tiles' worth of a boundary-path tile (0: by storage)
 };
 
 // Tile shapes (BX x BY) the fused kernel is instantiated for.
@@ -264,7 +265,8 @@ inline bool fused_tile_supported(int bx, int by) {
 // Bytes of dynamic LDS one fused block uses over the tile and its ring: three planes of the
 // c_z=+1 population group, two of the c_z=0 group and one of the c_z=-1 group (ring march,
 // variant 1), or one / two planes of all 27 pulled populations (register-window march,
-// variants 3 / 4); then the moments of the tile's west ring column for three planes.
+// variants 3 / 4); then the density and velocity of the tile's west ring column for three
+// planes.
 inline size_t fused_lds_bytes(int bx, int by, size_t real_bytes, int variant) {
     // ring march: 54 population slots of every frame position; register-window march: one or two
     // planes of nine (c_x, c_z) blocks of three population rows-sets and a spare row (WindowLds)

@@ -161,6 +161,7 @@ class Sim final : public SimBase {
         if (const char *e = std::getenv("MRLBM_XCD_REMAP")) lc.xcd_remap = std::atoi(e) != 0;
         if (const char *e = std::getenv("MRLBM_SLAB_AXIS")) lc.slab_axis = (e[0] == 'y' || e[0] == '1') ? 1 : 2;
         if (const char *e = std::getenv("MRLBM_FACE_TILES")) lc.face_tiles = std::atoi(e) != 0;
+        if (const char *e = std::getenv("MRLBM_SPECIAL_WEIGHT")) lc.special_weight = std::max(0.0, std::atof(e));
         if (const char *e = std::getenv("MRLBM_SPECIAL_ORDER")) lc.special_order = std::max(0, std::min(2, std::atoi(e)));
         if (const char *e = std::getenv("MRLBM_STOCHASTIC_ROUNDING")) stochastic_ = Conv<S>::scaled && std::atoi(e) != 0;
         if (const char *e = std::getenv("MRLBM_GRAPH_STEPS")) graph_steps_ = std::max(0, std::atoi(e) / 2 * 2);
@@ -399,9 +400,10 @@ class Sim final : public SimBase {
         lc.xcd_remap = true;
         // FP32 arithmetic on grids that stream from HBM: y slabs -- each XCD marches a band of
         // tile rows (cut to equal estimated work) through z in rounds, so a chunk's halo planes
-        // were loaded by the same XCD one round before -- with boundary tiles first in each round
-        // in half chunks (512x256x256 wake, A/B on one box: FP32 fused pass 608.5 -> 592.1 us,
-        // BF16 387.4 -> 382.3 us against z slabs)
+        // were loaded by the same XCD one round before -- with boundary tiles first in each round,
+        // those on the boundary path in half chunks (512x256x256 wake, A/B on one box: FP32 fused
+        // pass 608.5 -> 592.1 us, BF16 387.4 -> 382.3 us against z slabs; then whole chunks for
+        // face marches: FP32 591.5 -> 582.3 us, BF16 368.2 -> 361.2 us)
         if (sizeof(R) == 4 && !cache_resident && (g.ny + lc.by - 1) / lc.by >= 8) {
             lc.slab_axis = 1;
             lc.special_order = 0;
@@ -413,7 +415,7 @@ class Sim final : public SimBase {
     // Tile shape, march variant and work items of the fused pass. The grid is cut into one slab
     // per XCD (blocks are dispatched round-robin over the XCDs, so slab j runs on XCD j and its
     // tiles share that L2): z slabs, inside which every tile is cut into z chunks of lz planes
-    // (lz_special for tiles that hold boundary or donor nodes), or y slabs marched in z rounds.
+    // (lz_special for tiles on the boundary path), or y slabs marched in z rounds.
     // The boundary tiles go first.
     int configure_launch(const LaunchConfig &want) {
         LaunchConfig lc = want;
@@ -486,16 +488,22 @@ class Sim final : public SimBase {
             lists[0][gi].push_back(it);
             lists[part][gi].push_back(it);
         };
+        // tiles marched in chunks of lz_special: those on the boundary path. A face march costs
+        // about a plain march, and in a y slab its whole chunks keep step with the plain tiles'
+        // (512x256x256 wake with a D=8 cylinder, FP32: 574 us with every boundary tile in half
+        // chunks, 561 us in whole chunks)
+        auto short_chunks = [&](int t) { return tile_special[t] && face_kind(t) == FACE_NONE; };
         // pass of a tile in its slab (or z round): boundary tiles in pass 0 (first), in pass 1 with
         // the others (tile order), or in pass 2 (last; in a y slab after every round)
         auto pass_of = [&](int t) { return tile_special[t] ? lc.special_order : 1; };
         // y slabs: the tiles, in row-major order, cut into one contiguous range per XCD of equal
         // estimated work, a tile running the boundary path counting 1.5 plain tiles (FP32 storage)
-        // or 2 (16-bit storage and FP64), as measured per plane; rows holding a curved wall would
-        // otherwise load their XCDs alone
+        // or 2 (16-bit storage and FP64), as measured per plane (MRLBM_SPECIAL_WEIGHT overrides;
+        // 512x256x256 wake, FP32: 1.0 / 1.25 / 1.5 / 2.0 gave 586 / 584 / 582 / 591 us); rows
+        // holding a curved wall would otherwise load their XCDs alone
         std::vector<int> yrange(groups + 1, 0);
         if (yslab) {
-            const double ws = (sizeof(R) == 4 && !Conv<S>::scaled) ? 1.5 : 2.0;
+            const double ws = lc.special_weight > 0 ? lc.special_weight : (sizeof(R) == 4 && !Conv<S>::scaled) ? 1.5 : 2.0;
             std::vector<double> prefix(ntiles + 1, 0.0);
             for (int t = 0; t < ntiles; ++t)
                 prefix[t + 1] = prefix[t] + (!tile_live[t] ? 0.0 : (tile_special[t] && face_kind(t) == FACE_NONE ? ws : 1.0));
@@ -510,9 +518,10 @@ class Sim final : public SimBase {
         for (int gi = 0; gi < groups; ++gi) {
             if (yslab) {
                 const int ta = yrange[gi], tb = yrange[gi + 1];
-                // a tile's part [z, ze) of a round, in chunks of lz (lz_special for boundary tiles)
+                // a tile's part [z, ze) of a round, in chunks of lz (lz_special for tiles on the
+                // boundary path)
                 auto add_round = [&](int t, int z) {
-                    const int ze = std::min(g.nz, z + lz), step = tile_special[t] ? lzs : lz;
+                    const int ze = std::min(g.nz, z + lz), step = short_chunks(t) ? lzs : lz;
                     for (int zz = z; zz < ze; zz += step) add(gi, t, zz, std::min(ze, zz + step));
                 };
                 for (int z = 0; z < g.nz; z += lz)  // the rounds: passes 0 and 1
@@ -528,7 +537,7 @@ class Sim final : public SimBase {
             for (int pass = 0; pass < 3; ++pass)
                 for (int t = 0; t < ntiles; ++t) {
                     if (pass_of(t) != pass || !tile_live[t]) continue;
-                    const int step = tile_special[t] ? lzs : lz;
+                    const int step = short_chunks(t) ? lzs : lz;
                     for (int z = za; z < zb; z += step) add(gi, t, z, std::min(zb, z + step));
                 }
         }
