@@ -383,8 +383,9 @@ int mrlbm_sim_info(mrlbm_sim *sim, char *buf, int64_t cap, int64_t *len);
  * checkpoint_write work on it; init, step, time_steps, checkpoint_read and destroy are refused.
  * mrlbm_batch_autotune times trial_steps steps of each launch configuration -- tile, march
  * variant and z chunk, then for the fastest of those the cases' assignment to the CUs (blocks
- * per CU of a persistent launch, cases cut into XCD slabs or kept whole, slab axis) -- keeps the
- * fastest and re-initialises every case.
+ * per CU of a persistent launch, cases cut into XCD slabs or kept whole, slab axis) -- times the
+ * four fastest again as the run steps (graph replays), keeps the fastest and re-initialises
+ * every case.
  */
 typedef struct mrlbm_batch mrlbm_batch;
 

@@ -6,6 +6,10 @@
 // uneven cases there is a sequence of runs. Here each case keeps its own buffers, tables, force
 // ring and step counter (the same objects a standalone simulation uses), and only the launches
 // are shared.
+#include <algorithm>
+#include <utility>
+#include <vector>
+
 #include "sim.h"
 
 using mrlbm::set_error;
@@ -228,8 +232,12 @@ class Batch final : public BatchBase {
     // register-window march x z chunks of 4, 8, 16, 32 planes; then, around the fastest of those,
     // the assignment of the cases to the CUs -- one block per work item or a persistent launch with
     // 1, 2 or 3 blocks per CU, cases cut into XCD slabs (z or y) or kept whole on the least loaded
-    // XCD. The cases' states are scrambled by the trial steps: the tuner re-initialises them at
-    // the end.
+    // XCD. Trials run a few eager steps, which differ by a few percent from run to run and from
+    // graph replays (on 64 annuli a z chunk of 16 won the eager trials and one of 8 the
+    // replays), so the four fastest configurations of both stages are timed again as the run
+    // steps -- two replays of the step graph (or twice the trial steps where graphs are off) --
+    // twice each in turn, and the best of those times decides. The cases' states are scrambled by the
+    // trial steps: the tuner re-initialises them at the end.
     int autotune(int trial_steps, mrlbm_tune_result *out) override {
         if (trial_steps <= 0) return set_error(MRLBM_ERR_INVALID, "trial steps must be positive");
         const LaunchConfig def = launch;  // the configuration in use is timed first
@@ -247,15 +255,22 @@ class Batch final : public BatchBase {
         double best = 1e300, at_default = 0.0;
         LaunchConfig best_lc = def;
         int tried = 0;
-        auto trial = [&](const LaunchConfig &lc) -> int {
+        std::vector<std::pair<double, LaunchConfig>> timed;
+        auto time_config = [&](const LaunchConfig &lc, int nsteps, double &ms) -> int {
             if (int rc = configure(lc)) return rc;
+            // two eager steps, and one untimed replay (capture, instantiation, upload) of the
+            // step graph where graphs are on
+            int rc = step(2 + graph_steps_, MRLBM_STEP_FORCES);
+            if (!rc) rc = time_steps(nsteps, MRLBM_STEP_FORCES, &ms);
+            ms /= nsteps;
+            return rc;
+        };
+        auto trial = [&](const LaunchConfig &lc) -> int {
             double ms = 0.0;
-            int rc = step(2, MRLBM_STEP_FORCES);
-            if (!rc) rc = time_steps(trial_steps, MRLBM_STEP_FORCES, &ms);
-            if (rc) return rc;
-            ms /= trial_steps;
+            if (int rc = time_config(lc, trial_steps, ms)) return rc;
             ++tried;
             if (tried == 1) at_default = ms;
+            timed.emplace_back(ms, lc);
             if (ms < best) {
                 best = ms;
                 best_lc = lc;
@@ -298,6 +313,24 @@ class Batch final : public BatchBase {
                         return rc;
                     }
                 }
+        // third stage: the four fastest again, as the run steps, in turn, twice
+        graph_steps_ = graph_steps;
+        std::stable_sort(timed.begin(), timed.end(),
+                         [](const auto &p, const auto &q) { return p.first < q.first; });
+        timed.resize(std::min<size_t>(timed.size(), 4));
+        std::vector<double> again(timed.size(), 1e300);
+        const int n3 = graph_steps_ > 0 ? 2 * graph_steps_ : 2 * trial_steps;
+        for (int pass = 0; pass < 2; ++pass)
+            for (size_t k = 0; k < timed.size(); ++k) {
+                double ms = 0.0;
+                if (int rc = time_config(timed[k].second, n3, ms)) return rc;
+                again[k] = std::min(again[k], ms);
+            }
+        for (size_t k = 0; k < timed.size(); ++k)
+            if (k == 0 || again[k] < best) {
+                best = again[k];
+                best_lc = timed[k].second;
+            }
         graph_steps_ = graph_steps;
         if (int rc = configure(best_lc)) return rc;
         if (int rc = init()) return rc;
