@@ -498,7 +498,6 @@ __device__ __forceinline__ void window_march(const FusedArgs<R, S> &a, R *lds, i
     static_assert(!SPECIAL || FACE == FACE_NONE, "a face march runs the plain path");
     using F = Frame<BX, BY>;
     constexpr int W = F::W;
-    constexpr int NH = F::NH;
     using L = WindowLds<BX, BY>;
     constexpr bool kReloadOwn = window_min_blocks<S, BX, BY, NBUF>() == 4;
     R *const wold = lds + NBUF * L::PLANE;
