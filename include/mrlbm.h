@@ -401,6 +401,7 @@ typedef struct mrlbm_tune_result {
     int32_t packing;        /* cases on the XCDs: 0 every case of 32 planes or more cut into one z
                                slab per XCD, 1 whole cases on the least loaded XCD, largest first */
     int32_t slab_axis;      /* XCD slabs of a cut case: 2 z slabs, 1 y slabs (rows of tiles) */
+    int32_t lz_special;     /* z chunk of the tiles on the boundary path (0: lz) */
 } mrlbm_tune_result;
 
 int mrlbm_batch_plan(const mrlbm_case_desc *descs, int32_t n, uint64_t hbm_budget_bytes, int32_t *admitted,

@@ -134,6 +134,7 @@ class TuneResult(C.Structure):
         ("ms_per_step_default", C.c_double),
         ("packing", C.c_int32),
         ("slab_axis", C.c_int32),
+        ("lz_special", C.c_int32),
     ]
 
 

@@ -232,7 +232,7 @@ class Batch final : public BatchBase {
     // register-window march x z chunks of 4, 8, 16, 32 planes; then, around the fastest of those,
     // the assignment of the cases to the CUs -- one block per work item or a persistent launch with
     // 1, 2 or 3 blocks per CU, cases cut into XCD slabs (z or y) or kept whole on the least loaded
-    // XCD. Trials run a few eager steps, which differ by a few percent from run to run and from
+    // XCD, and (one block per item) boundary-path tiles in whole or half z chunks. Trials run a few eager steps, which differ by a few percent from run to run and from
     // graph replays (on 64 annuli a z chunk of 16 won the eager trials and one of 8 the
     // replays), so the four fastest configurations of both stages are timed again as the run
     // steps -- two replays of the step graph (or twice the trial steps where graphs are off) --
@@ -301,18 +301,21 @@ class Batch final : public BatchBase {
         const LaunchConfig base = best_lc;
         for (int packing : {0, 1})
             for (int axis : {2, 1})
-                for (int bpc : {0, 1, 2, 3}) {
-                    if (packing == 1 && axis == 1) continue;  // whole cases are not cut into slabs
-                    LaunchConfig lc = base;
-                    lc.packing = packing;
-                    lc.slab_axis = axis;
-                    lc.blocks_per_cu = bpc;
-                    if (same(lc, base)) continue;
-                    if (int rc = trial(lc)) {
-                        graph_steps_ = graph_steps;
-                        return rc;
+                for (int bpc : {0, 1, 2, 3})
+                    for (int lzs : {0, base.lz / 2}) {
+                        if (packing == 1 && axis == 1) continue;  // whole cases are not cut into slabs
+                        if (lzs > 0 && bpc > 0) continue;         // half chunks: with one block per item
+                        LaunchConfig lc = base;
+                        lc.packing = packing;
+                        lc.slab_axis = axis;
+                        lc.blocks_per_cu = bpc;
+                        lc.lz_special = lzs;
+                        if (same(lc, base) || (lzs > 0 && lzs == base.lz)) continue;
+                        if (int rc = trial(lc)) {
+                            graph_steps_ = graph_steps;
+                            return rc;
+                        }
                     }
-                }
         // third stage: the four fastest again, as the run steps, in turn, twice
         graph_steps_ = graph_steps;
         std::stable_sort(timed.begin(), timed.end(),
@@ -345,6 +348,7 @@ class Batch final : public BatchBase {
             out->ms_per_step_default = at_default;
             out->packing = best_lc.packing;
             out->slab_axis = best_lc.slab_axis;
+            out->lz_special = best_lc.lz_special;
         }
         return MRLBM_OK;
     }
@@ -562,6 +566,7 @@ int mrlbm_batch_get_launch(mrlbm_batch *b, mrlbm_tune_result *out) {
     out->blocks_per_cu = b->impl->launch.blocks_per_cu;
     out->packing = b->impl->launch.packing;
     out->slab_axis = b->impl->launch.slab_axis;
+    out->lz_special = b->impl->launch.lz_special;
     return MRLBM_OK;
 }
 

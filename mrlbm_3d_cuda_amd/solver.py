@@ -357,7 +357,7 @@ class Batch:
         r = TuneResult()
         _check(self.lib.mrlbm_batch_get_launch(self.h, C.byref(r)))
         return {"bx": r.bx, "by": r.by, "variant": r.variant, "lz": r.lz, "blocks_per_cu": r.blocks_per_cu,
-                "packing": r.packing, "slab_axis": r.slab_axis}
+                "packing": r.packing, "slab_axis": r.slab_axis, "lz_special": r.lz_special}
 
 
 def algorithmic_bytes_per_update(precision):

@@ -110,7 +110,8 @@ def test_batch_autotune_keeps_results(monkeypatch):
         assert r["candidates"] >= 24
         assert (r["bx"], r["by"]) in [(32, 8), (32, 4), (16, 8)] and r["variant"] in (1, 3, 4)
         assert 0 < r["ms_per_step"] <= r["ms_per_step_default"] * 1.0001
-        assert b.launch() == {k: r[k] for k in ("bx", "by", "variant", "lz", "blocks_per_cu", "packing", "slab_axis")}
+        assert b.launch() == {k: r[k] for k in ("bx", "by", "variant", "lz", "blocks_per_cu", "packing", "slab_axis",
+                                                "lz_special")}
         assert all(c.info().steps_done == 0 for c in b.cases)
         b.step(20, forces=True)
         got = [(c.fields(), c.forces(0, 20)) for c in b.cases]
