@@ -396,7 +396,12 @@ class Sim final : public SimBase {
         // latency-bound, so small grids need many short chunks), at most 32 planes
         const int64_t tiles = int64_t((g.nx + lc.bx - 1) / lc.bx) * ((g.ny + lc.by - 1) / lc.by);
         lc.lz = int(std::max<int64_t>(1, std::min<int64_t>(32, tiles * g.nz / 1024)));
-        lc.lz_special = 0;  // boundary tiles in chunks of the same length
+        // boundary-path tiles in chunks of the same length (FP64), or of a quarter of it on
+        // cache-resident grids in FP32 arithmetic, whose boundary tiles bound the pass: shorter
+        // items of the slower tiles leave a shorter tail (256x128x64 cylinder, A/B on one box:
+        // FP32 storage 47,920-48,090 -> 48,602-48,820 MLUPS, BF16 50,944-50,983 -> 51,172-51,481,
+        // FP16 45,867-46,072 -> 47,065-47,195; FP64 12,289-12,300 -> 12,056-12,063, so not there)
+        lc.lz_special = (sizeof(R) == 4 && cache_resident) ? std::max(1, lc.lz / 4) : 0;
         lc.xcd_remap = true;
         // FP32 arithmetic on grids that stream from HBM: y slabs -- each XCD marches a band of
         // tile rows (cut to equal estimated work) through z in rounds, so a chunk's halo planes
