@@ -289,9 +289,9 @@ def test_two_stream_schedule_is_bitwise_neutral(kind, precision):
 @pytest.mark.parametrize("kind", ["channel", "annulus", "closed_channel"])
 def test_slab_axis_is_bitwise_neutral(kind, precision):
     """Tile work items cut into y slabs (rows of tiles marched in z rounds) instead of z slabs,
-    boundary tiles first, in tile order or last: the same state, force records, plane means and wall pressure,
-    bit for bit, over eager steps, graph replays and statistics steps; FP64 also equals the
-    oracle."""
+    boundary tiles first, in tile order or last, in whole z chunks or in shorter ones: the same
+    state, force records, plane means and wall pressure, bit for bit, over eager steps, graph
+    replays and statistics steps; FP64 also equals the oracle."""
     from mrlbm_3d_cuda_amd.solver import Simulation
     from oracle.binding import Oracle
 
@@ -303,17 +303,23 @@ def test_slab_axis_is_bitwise_neutral(kind, precision):
         if kind == "closed_channel":
             d = closed_z(d)
     out = []
-    for axis, lz, order in (("z", None, "0"), ("y", "8", "0"), ("y", "64", "1"), ("z", None, "2"), ("y", "16", "2")):
-        print(f"[gpu parity] {kind} slab axis {axis} lz {lz} boundary tile order {order}", flush=True)
+    for axis, lz, order, lzs in (("z", None, "0", None), ("y", "8", "0", None), ("y", "64", "1", None),
+                                 ("z", None, "2", None), ("y", "16", "2", None), ("z", "8", "0", "1"),
+                                 ("z", "8", "0", "8"), ("y", "16", "0", "5")):
+        print(f"[gpu parity] {kind} slab axis {axis} lz {lz} boundary tile order {order} boundary chunk {lzs}",
+              flush=True)
         os.environ["MRLBM_SLAB_AXIS"] = axis
         os.environ["MRLBM_SPECIAL_ORDER"] = order
         if lz:
             os.environ["MRLBM_LZ"] = lz
+        if lzs:
+            os.environ["MRLBM_LZ_SPECIAL"] = lzs
         try:
             s = Simulation(d).init()
         finally:
             del os.environ["MRLBM_SLAB_AXIS"], os.environ["MRLBM_SPECIAL_ORDER"]
             os.environ.pop("MRLBM_LZ", None)
+            os.environ.pop("MRLBM_LZ_SPECIAL", None)
         s.step(3, forces=True).step(32, forces=True).step(17, forces=True, stats=True).step(8)
         r = [s.fields(), s.forces(0, 52), s.surface_pressure()[0]]
         if kind != "annulus":
